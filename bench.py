@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""SPPM pass throughput on one MI355X.
+
+A step is one full SPPM pass (trace → emit → hash/scan/sort → gather) of the
+workload, with the scene already resident in HBM.  The metric is
+photons+rays/s per pass = (pixels·1 camera ray + photons per pass) / T_pass.
+
+Default workload: the caustic stress scene (Cornell box + two 20480-triangle
+GGX spheres, 1024², 2^21 photons/pass, r0 = 0.01).
+
+Prints one JSON line.  With --gpus > 1 it exits: this build measures one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def gather_bytes(n_pixels: int, n_photons: int, n_buckets: int) -> int:
+    """Algorithmic HBM bytes of one gather launch: per pixel 8 float4 + 1 f32
+    read and 2 float4 + 1 f32 written (168 B), every stored photon record read
+    once (24 B), cell start/end read once (8 B per bucket)."""
+    return 168 * n_pixels + 24 * n_photons + 8 * n_buckets
+
+
+def read_traffic(kernel: str, workload: str):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+    f = ROOT / "profiles" / f"{kernel}_{workload}_pmc.txt"
+    if not f.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", f.read_text())
+    return float(m.group(1)) if m else None
+
+
+def run_cpu_baseline(cfg, scene, passes: int, threads: int):
+    import oracle
+
+    oracle.lib().ref_set_num_threads(threads)
+    ref = oracle.RefRenderer(cfg, scene)
+    ref.next_seed()  # time pass 2 of the seed schedule, as the GPU median skips pass 1
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        rc = ref.run_pass()
+        if rc != 0:
+            raise RuntimeError(f"oracle pass failed: {rc}")
+    dt = time.perf_counter() - t0
+    stages = ref.stage_ms()
+    ref.close()
+    work = (cfg.width * cfg.height + cfg.photons_per_round) * passes
+    return work / dt, dt, stages
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=12)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="caustic", choices=["caustic", "cornell", "atrium"])
+    ap.add_argument("--cpu-passes", type=int, default=1, help="oracle passes timed for the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0 = all cores of this host)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the pass is single-GPU; sharding is described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import phosphor_amd as P
+
+    cfg, scene, _rounds = P.config_for(args.workload)
+    n_px = cfg.width * cfg.height
+    print(f"[bench] workload={args.workload} tris={scene.n_triangles} {cfg.width}x{cfg.height} "
+          f"photons/pass={cfg.photons_per_round} grid={cfg.grid_res}", file=sys.stderr, flush=True)
+    r = P.Renderer(cfg, scene)
+    for _ in range(args.warmup):
+        r.run_pass()
+    r.synchronize()
+    stage = []
+    counts = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r.run_pass()
+        stage.append(r.stage_ms())
+        counts.append(r.last_photon_count)
+    r.synchronize()
+    dt = time.perf_counter() - t0
+    stage = np.array(stage)
+    ms_per_step = dt * 1e3 / args.steps
+    value = (n_px + cfg.photons_per_round) * args.steps / dt
+    mean_stage = stage.mean(axis=0)
+    n_photons = int(np.mean([min(c, cfg.photons_per_round * cfg.max_photon_bounces) for c in counts]))
+    buckets = cfg.grid_res ** 3 + 1
+    gbytes = gather_bytes(n_px, n_photons, buckets)
+    g_ms = float(mean_stage[3])
+    achieved = gbytes / (g_ms * 1e-3) / 1e9
+    print(f"[bench] {ms_per_step:.2f} ms/pass, stages trace/emit/map/gather = "
+          f"{', '.join(f'{x:.2f}' for x in mean_stage)} ms, deposits/pass={n_photons}", file=sys.stderr, flush=True)
+    r.close()
+
+    cpu = None
+    if args.cpu_passes > 0:
+        threads = args.cpu_threads or cpu_cores()
+        try:
+            cpu_value, cpu_dt, cpu_stages = run_cpu_baseline(cfg, scene, args.cpu_passes, threads)
+            cpu = {"value": cpu_value, "unit": "photons+rays/s", "cores": threads, "kind": "port",
+                   "sample": f"{args.cpu_passes} full pass(es) of the {args.workload} workload "
+                             f"({cpu_dt:.1f} s, stages {', '.join(f'{x:.0f}' for x in cpu_stages)} ms) "
+                             f"on {cpu_model()}"}
+            print(f"[bench] cpu oracle: {cpu_value:.3e} photons+rays/s over {cpu_dt:.1f}s on {threads} threads",
+                  file=sys.stderr, flush=True)
+        except Exception as e:  # the CPU leg is a reported baseline, not the measurement
+            print(f"[bench] cpu baseline skipped: {e}", file=sys.stderr)
+
+    line = {
+        "metric": "SPPM photons+rays/s per pass",
+        "value": value,
+        "unit": "photons+rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "triangles": scene.n_triangles, "width": cfg.width,
+                   "height": cfg.height, "photons_per_pass": cfg.photons_per_round,
+                   "max_bounces": cfg.max_photon_bounces, "grid_res": cfg.grid_res,
+                   "initial_radius": cfg.initial_radius, "direct_samples": cfg.direct_samples,
+                   "parallelism": "single-gpu"},
+        "stage_ms": {"trace": float(mean_stage[0]), "emit": float(mean_stage[1]),
+                     "hash_scan_sort": float(mean_stage[2]), "gather": float(mean_stage[3])},
+        "roofline": {"kernel": "k_gather", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": read_traffic("k_gather", args.workload)},
+        "traversal": {"kernel": "k_trace_rays", "camera_rays_per_s": n_px / (float(mean_stage[0]) * 1e-3),
+                      "photons_per_s": cfg.photons_per_round / (float(mean_stage[1]) * 1e-3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,329 @@
+/*
+ * phosphor_hip.h — C ABI of the MI355X (gfx950) SPPM pass kernels.
+ *
+ * Every entry point replaces one OpenCL enqueue of alfazet/phosphor's pass loop
+ * (src/renderer/renderer.cpp:90-161).  Arguments keep the order and meaning of
+ * the `.cl` kernel signature they replace: a `cl::Buffer` becomes a device
+ * pointer, a by-value struct (CameraParams, PhotonHashData, float4) becomes a
+ * `const *` to host memory holding the same bytes, scalars stay u32/f32, and a
+ * trailing `stream` (a hipStream_t; NULL = the null stream) orders the launch.
+ *
+ * All launches are asynchronous on `stream`.  Every function returns 0 on
+ * success or a PHX_ERR_* / hipError_t code and never throws; the message of
+ * the last failure on the calling thread is returned by phx_last_error().
+ *
+ * The structs below restate the reference's host/device POD layouts byte for
+ * byte (the structs of include/common/); their sizes and field offsets are checked by
+ * static assertions in phosphor_amd/csrc/abi.cpp and by the CPU tests.
+ */
+#ifndef PHOSPHOR_HIP_H
+#define PHOSPHOR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PHX_ALIGN16 __attribute__((aligned(16)))
+
+/* ---- scalar/vector types (include/common/typedefs.h:4-27) ---- */
+typedef struct PHX_ALIGN16 phx_float4 { float x, y, z, w; } phx_float4;
+typedef struct phx_float2 { float x, y; } phx_float2;
+
+/* ---- boundary structs (byte-exact with the reference) ---- */
+
+/* include/common/camera.h:7-22 — 112 bytes */
+typedef struct PHX_ALIGN16 PhxCameraParams {
+    phx_float4 position;
+    phx_float4 lower_left_corner;
+    phx_float4 horizontal;
+    phx_float4 vertical;
+    phx_float4 defocus_disk_u;
+    phx_float4 defocus_disk_v;
+    float defocus_angle;
+    float focus_distance;
+    uint8_t _padding[8];
+} PhxCameraParams;
+
+/* include/common/triangle.h:7-28 — 48/48/48/32 bytes */
+typedef struct PHX_ALIGN16 PhxTriangleVertices { phx_float4 v0, v1, v2; } PhxTriangleVertices;
+typedef struct PHX_ALIGN16 PhxTriangleNormals { phx_float4 n0, n1, n2; } PhxTriangleNormals;
+typedef struct PHX_ALIGN16 PhxTriangleTangents { phx_float4 t0, t1, t2; } PhxTriangleTangents;
+typedef struct PHX_ALIGN16 PhxTriangleUvs {
+    phx_float2 uv0, uv1, uv2;
+    uint8_t _padding[8];
+} PhxTriangleUvs;
+
+/* include/common/bvh_node.h:6-8 + bounding_box.h:7-10 — 32 bytes.
+ * min.w = as_float(triangle index), 0xFFFFFFFF for an interior node;
+ * max.w = as_float(index of the right child) (src/renderer/bvh.cpp:59-72). */
+typedef struct PHX_ALIGN16 PhxBvhNode { phx_float4 min, max; } PhxBvhNode;
+
+/* include/common/material.h:7-27 — 32 / 192 bytes */
+typedef struct PHX_ALIGN16 PhxUvTransform {
+    phx_float2 uv_offset;
+    phx_float2 uv_scale;
+    float uv_rotation;
+    uint8_t _padding[12];
+} PhxUvTransform;
+typedef struct PHX_ALIGN16 PhxUvTransformBlock {
+    PhxUvTransform diff, emis, norm, occlusion, metal_rough, transmission;
+} PhxUvTransformBlock;
+
+/* include/common/material.h:36-59 — 96 bytes */
+typedef struct PHX_ALIGN16 PhxMaterial {
+    phx_float4 base_color;
+    phx_float4 emissive;
+    phx_float4 att_color;
+    float metallic, roughness, transmission, ior, att_dist, thickness;
+    uint32_t diff_index, emis_index, norm_index, occlusion_index, metal_rough_index, transmission_index;
+} PhxMaterial;
+
+/* include/common/texture_meta.h:7-15 — 16 bytes */
+typedef struct PHX_ALIGN16 PhxTextureMeta {
+    uint32_t atlas_offset, width, height, channels;
+} PhxTextureMeta;
+
+/* include/common/light.h:9-31 — 128 bytes */
+#define PHX_LIGHT_POINT 0
+#define PHX_LIGHT_SPOT 1
+#define PHX_LIGHT_DIRECTIONAL 2
+#define PHX_LIGHT_TEXTURED 3
+typedef struct PHX_ALIGN16 PhxLight {
+    phx_float4 position, power, direction, tangent, bitangent, origin, aux;
+    uint8_t kind;
+    uint8_t _padding[15];
+} PhxLight;
+
+/* include/common/photon_hash.h:11-19 — 48 bytes */
+typedef struct PHX_ALIGN16 PhxPhotonHashData {
+    phx_float4 origin;
+    phx_float4 cell_sizes;
+    uint32_t grid_res;
+    uint8_t _padding[12];
+} PhxPhotonHashData;
+
+/* ---- status codes ---- */
+#define PHX_OK 0
+#define PHX_ERR_INVALID_ARG 10001
+#define PHX_ERR_NO_PHOTONS 10002     /* renderer.cpp:117-118 "no photon hit" */
+#define PHX_ERR_WORKSPACE 10003      /* workspace smaller than the *_workspace_size query */
+#define PHX_ERR_NO_DEVICE 10004
+#define PHX_ERR_TOO_LARGE 10005
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *phx_last_error(void);
+/* Library version string. */
+const char *phx_version(void);
+
+/* ======================================================================
+ * Kernel-level entry points (one per reference kernel)
+ * ====================================================================== */
+
+/* Replaces kernels/trace_rays.cl:14-40 (enqueued at renderer.cpp:90-95,
+ * arguments bound at src/buffers/rendering.cpp:32-42).  One work item per
+ * pixel: camera ray → first diffuse hitpoint, NEE direct light, specular
+ * emission, envmap on miss.  Writes hp_pos/hp_normal/hp_throughput/
+ * hp_base_color/hp_direct/hp_emission (float4 per pixel). */
+int phx_trace_rays(const PhxCameraParams *camera, uint32_t image_width, uint32_t image_height, uint32_t seed,
+                   uint32_t direct_samples,
+                   const PhxTriangleVertices *tri_verts, const PhxTriangleNormals *tri_normals,
+                   const PhxTriangleTangents *tri_tangents, const PhxTriangleUvs *tri_uvs,
+                   const PhxBvhNode *bvh_nodes, const uint32_t *tri_mat_index, uint32_t n_triangles,
+                   const PhxMaterial *materials, const PhxUvTransformBlock *uv_transforms,
+                   const PhxTextureMeta *tex_meta, const uint8_t *tex_atlas,
+                   const PhxLight *lights, uint32_t n_lights, const float *lum_pref_sum, float total_lum,
+                   const phx_float4 *scene_center, float scene_radius,
+                   const PhxTriangleVertices *etri_verts, const PhxTriangleNormals *etri_normals,
+                   const PhxTriangleUvs *etri_uvs, const uint32_t *etri_mat_index,
+                   const float *envmap_data, uint32_t envmap_width, uint32_t envmap_height, uint32_t has_envmap,
+                   phx_float4 *hp_pos, phx_float4 *hp_normal, phx_float4 *hp_throughput,
+                   phx_float4 *hp_base_color, phx_float4 *hp_direct, phx_float4 *hp_emission,
+                   void *stream);
+
+/* Device scratch needed by phx_emit_photons for one batch. */
+size_t phx_emit_photons_workspace_size(uint32_t photons_to_emit, uint32_t max_photon_bounces);
+
+/* Replaces kernels/emit_photons.cl:12-32 (renderer.cpp:97-121, arguments bound
+ * at rendering.cpp:44-56).  Photon `offset + i` (i < photons_to_emit) is traced
+ * with PCG stream `seed + offset + i`; every diffuse deposit is appended at
+ * index *n_photons (a device counter the caller resets per pass, as
+ * renderer.cpp:99 does).  Unlike the reference's atomic_inc order, deposits
+ * land in the canonical order (photon index, bounce), so repeated runs are
+ * bit-identical; the counter counts every deposit, and deposits at or beyond
+ * max_photons are dropped (emit_photons.cl:98-105).  `workspace` must hold
+ * phx_emit_photons_workspace_size(photons_to_emit, max_photon_bounces) bytes. */
+int phx_emit_photons(const PhxLight *lights, uint32_t n_lights, uint32_t max_photons, uint32_t offset,
+                     uint32_t photons_to_emit, uint32_t max_photon_bounces, uint32_t seed,
+                     const PhxBvhNode *tree, const PhxTriangleVertices *tri_verts,
+                     const PhxTriangleNormals *tri_normals, const PhxTriangleTangents *tri_tangents,
+                     const PhxTriangleUvs *tri_uvs, const uint32_t *tri_mat_index, uint32_t n_tris,
+                     const PhxTriangleVertices *etri_verts, const PhxTriangleNormals *etri_normals,
+                     const PhxTriangleUvs *etri_uvs, const uint32_t *etri_mat_index,
+                     const PhxMaterial *materials, const PhxUvTransformBlock *uv_transforms,
+                     const PhxTextureMeta *tex_meta, const uint8_t *tex_atlas, const float *lum_pref_sum,
+                     float total_lum, const phx_float4 *scene_center, float scene_radius,
+                     phx_float4 *photon_pos, uint32_t *photon_power, uint32_t *photon_dir, uint32_t *n_photons,
+                     void *workspace, size_t workspace_bytes, void *stream);
+
+/* Replaces kernels/hash_photons.cl:4-5 (renderer.cpp:128-132).  photon_hashes[i]
+ * = get_hash(photon_pos[i]) (photon_hash.h:21-37, 0 = outside the grid) and
+ * cell_counts[h] += 1.  Like the reference, cell_counts is accumulated into:
+ * the caller zeroes it first (renderer.cpp:128). */
+int phx_hash_photons(const phx_float4 *photon_pos, uint32_t n_photons, const PhxPhotonHashData *data,
+                     uint32_t *photon_hashes, uint32_t *cell_counts, void *stream);
+
+/* Device scratch needed by phx_scan_cells for n_cells elements. */
+size_t phx_scan_cells_workspace_size(uint32_t n_cells);
+
+/* Replaces kernels/scan_small_chunks.cl + scan_big_chunks.cl +
+ * scan_add_offsets.cl and the cell_start→cell_end copy (renderer.cpp:134-147)
+ * with one single-pass scan.  Postcondition: cell_start = exclusive prefix sum
+ * of cell_counts and cell_end = cell_start.  No 4096² bucket limit. */
+int phx_scan_cells(const uint32_t *cell_counts, uint32_t n_cells, uint32_t *cell_start, uint32_t *cell_end,
+                   void *workspace, size_t workspace_bytes, void *stream);
+
+/* Replaces kernels/scatter_photons.cl:4-7 with its exact semantics:
+ * dst = atomic_inc(&cell_end[h]); records copied to dst.  Order inside a cell
+ * follows atomic arrival (nondeterministic, as in the reference). */
+int phx_scatter_photons(const phx_float4 *photon_pos_in, const uint32_t *photon_power_in,
+                        const uint32_t *photon_dir_in, const uint32_t *photon_hashes, uint32_t *cell_end,
+                        uint32_t n_photons, phx_float4 *out_photon_pos, uint32_t *out_photon_power,
+                        uint32_t *out_photon_dir, void *stream);
+
+/* Device scratch needed by phx_sort_photons. */
+size_t phx_sort_photons_workspace_size(uint32_t n_photons, uint32_t n_cells);
+
+/* Deterministic counterpart of scatter_photons.cl: a stable counting sort by
+ * cell.  Same inputs and the same postcondition (records grouped by cell,
+ * cell_end[h] = cell_start[h] + count(h)), but inside a cell records keep
+ * their input order, so the sorted photon map is canonical.  cell_start must
+ * hold the exclusive scan of the counts and cell_end must equal cell_start on
+ * entry (the phx_scan_cells postcondition). */
+int phx_sort_photons(const phx_float4 *photon_pos_in, const uint32_t *photon_power_in,
+                     const uint32_t *photon_dir_in, const uint32_t *photon_hashes, const uint32_t *cell_start,
+                     uint32_t *cell_end, uint32_t n_photons, uint32_t n_cells, phx_float4 *out_photon_pos,
+                     uint32_t *out_photon_power, uint32_t *out_photon_dir, void *workspace,
+                     size_t workspace_bytes, void *stream);
+
+/* Replaces kernels/gather_flux.cl:5-22 (renderer.cpp:155-161, arguments bound
+ * at rendering.cpp:71-77): per-pixel 27-cell Gaussian photon gather and the
+ * SPPM radius/flux update (photon_hash.h:56-103, gather_flux.cl:28-77). */
+int phx_gather_flux(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const phx_float4 *hp_throughput,
+                    const phx_float4 *hp_base_color, const phx_float4 *hp_direct, const phx_float4 *hp_emission,
+                    uint32_t n_pixels, const phx_float4 *photon_pos, const uint32_t *photon_power,
+                    const uint32_t *photon_dir, uint32_t n_photons, const uint32_t *cell_start,
+                    const uint32_t *cell_end, const PhxPhotonHashData *data, float sppm_alpha,
+                    phx_float4 *flux, float *photon_weight, phx_float4 *radiance, void *stream);
+
+/* ======================================================================
+ * Device-memory plumbing (no reference counterpart: lets hosts without their
+ * own HIP allocator, such as the Python tooling, own device buffers through
+ * the same HIP runtime as the kernels)
+ * ====================================================================== */
+int phx_device_count(void);
+int phx_device_malloc(void **out, size_t bytes);
+int phx_device_free(void *p);
+int phx_memcpy_htod(void *dst, const void *src, size_t bytes);
+int phx_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int phx_memset_device(void *dst, int value, size_t bytes);
+int phx_device_synchronize(void);
+
+/* ======================================================================
+ * Host helpers the pass driver needs (src/renderer/bvh.cpp, photon_hash.h)
+ * ====================================================================== */
+
+/* src/renderer/bvh.cpp:18-84: median split on the longest axis of the node
+ * bbox, one triangle per leaf, DFS-flattened (left child = index + 1).
+ * `tri_verts` is host memory with n_tris entries; `out_nodes` must hold
+ * 2*n_tris-1 nodes.  Returns the node count (0 on error). */
+uint32_t phx_build_bvh_median(const PhxTriangleVertices *tri_verts, uint32_t n_tris, PhxBvhNode *out_nodes);
+
+/* photon_hash.h:41-50 build_photon_hash_info(bbox, grid_res), written to *out. */
+int phx_build_photon_hash_info(const PhxBvhNode *root, uint32_t grid_res, PhxPhotonHashData *out);
+
+/* ======================================================================
+ * Pass driver (src/renderer/renderer.cpp Renderer::set_scene / run)
+ * ====================================================================== */
+
+/* Host-side scene description (SceneData, include/host/scene_data.hpp:21-42,
+ * already split to SoA as GeometryBuffers::upload does).  All pointers are
+ * host memory; the renderer copies them to the device. */
+typedef struct PhxSceneDesc {
+    const PhxTriangleVertices *tri_verts;
+    const PhxTriangleNormals *tri_normals;
+    const PhxTriangleTangents *tri_tangents;
+    const PhxTriangleUvs *tri_uvs;
+    const uint32_t *tri_mat_index;
+    uint32_t n_triangles;
+    const PhxTriangleVertices *etri_verts;
+    const PhxTriangleNormals *etri_normals;
+    const PhxTriangleUvs *etri_uvs;
+    const uint32_t *etri_mat_index;
+    uint32_t n_e_triangles;
+    const PhxMaterial *materials;
+    const PhxUvTransformBlock *uv_transforms;
+    uint32_t n_materials;
+    const PhxLight *lights;
+    uint32_t n_lights;
+    const PhxTextureMeta *tex_meta;
+    uint32_t n_textures;
+    const uint8_t *tex_atlas;
+    uint64_t tex_atlas_bytes;
+    const float *envmap; /* 3 floats per texel, NULL if none */
+    uint32_t envmap_width, envmap_height;
+    const PhxCameraParams *camera;
+    /* optional prebuilt BVH (NULL → median-split build as bvh.cpp does) */
+    const PhxBvhNode *bvh_nodes;
+    uint32_t n_bvh_nodes;
+} PhxSceneDesc;
+
+/* The subset of ArgsList (include/host/cmd_args.hpp:36-65) the pass reads. */
+typedef struct PhxRenderArgs {
+    uint32_t width, height;
+    uint32_t photons_per_round;     /* -p */
+    uint32_t max_photons_per_batch; /* --batch-size */
+    uint32_t max_photon_bounces;    /* --max-bounces */
+    uint32_t grid_res;              /* --grid-res */
+    uint32_t direct_samples;        /* --direct-samples */
+    uint32_t seed;                  /* --seed */
+    float sppm_alpha;               /* --sppm-alpha */
+    float initial_radius;           /* 0 → reference rule (cell half-diagonal, gather_flux.cl:51-56) */
+} PhxRenderArgs;
+
+typedef struct PhxRenderer PhxRenderer;
+
+/* Renderer::Renderer + set_scene (renderer.cpp:9-57). */
+int phx_renderer_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, PhxRenderer **out);
+void phx_renderer_destroy(PhxRenderer *r);
+/* One SPPM round of Renderer::run (renderer.cpp:64-78): round_seed from the
+ * host PCG, then trace → emit → hash/scan/sort → gather.  Asynchronous; the
+ * per-pass deposit count is read back once (renderer.cpp:113-118). */
+int phx_renderer_run_pass(PhxRenderer *r);
+/* Block until all enqueued passes have finished. */
+int phx_renderer_synchronize(PhxRenderer *r);
+/* Stage timings of the last pass in milliseconds: trace, emit, hash+scan+sort, gather. */
+int phx_renderer_last_stage_ms(PhxRenderer *r, float out_ms[4]);
+/* Number of deposited photons in the last pass (before the clamp). */
+uint32_t phx_renderer_last_photon_count(PhxRenderer *r);
+uint32_t phx_renderer_rounds_done(PhxRenderer *r);
+/* Copy per-pixel state to host: flux (τ, r²), photon_weight N, radiance sum. */
+int phx_renderer_read_state(PhxRenderer *r, phx_float4 *flux, float *photon_weight, phx_float4 *radiance);
+/* Copy the last pass's hitpoint buffers (6 × float4 per pixel, hp_pos … hp_emission order). */
+int phx_renderer_read_hitpoints(PhxRenderer *r, phx_float4 *hp6);
+/* Copy the last pass's sorted photon map (n = last clamped count) and cell ranges. */
+int phx_renderer_read_photon_map(PhxRenderer *r, phx_float4 *pos, uint32_t *power, uint32_t *dir,
+                                 uint32_t *cell_start, uint32_t *cell_end);
+/* Device pointers of the renderer's buffers (for the Python tooling). */
+int phx_renderer_device_buffers(PhxRenderer *r, void **ptrs, uint32_t n_ptrs);
+/* Final linear estimate (output.cpp:52-83 before exposure/gamma):
+ * L = radiance/rounds + flux.xyz/(π r² photons_emitted). */
+int phx_renderer_estimate(PhxRenderer *r, float *rgb_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHOSPHOR_HIP_H */

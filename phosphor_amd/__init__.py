@@ -1,0 +1,182 @@
+"""phosphor_amd — MI355X (gfx950) SPPM pass loop with the reference's pass interface.
+
+The compute path is the HIP library ``build/libphosphor_hip.so`` (C ABI in
+include/phosphor_hip.h).  There is no CPU fallback: loading fails loudly if the
+library is missing, and the renderer fails if no HIP device is present.
+
+``Renderer`` mirrors src/renderer/renderer.cpp: construct it with the pass
+arguments and a scene (Renderer::Renderer + set_scene), then call
+``run_pass()`` per SPPM round or ``run(rounds)`` (Renderer::run).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from . import abi
+from .abi import RenderArgs, ptr
+
+_LIB = None
+LIB_PATH = Path(__file__).resolve().parent / "build" / "libphosphor_hip.so"
+
+
+class PhosphorError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def lib():
+    """The loaded HIP library (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        path = Path(os.environ.get("PHOSPHOR_HIP_LIB", LIB_PATH))
+        if not path.exists():
+            raise RuntimeError(f"{path} is missing: build it with `python -m phosphor_amd.build` "
+                               "(there is no CPU fallback for the pass kernels)")
+        _LIB = abi.bind(C.CDLL(str(path)))
+    return _LIB
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().phx_last_error().decode(errors="replace")
+        raise PhosphorError(rc, f"{what}: {msg}" if what else msg)
+
+
+@dataclass
+class PassConfig:
+    """The ArgsList fields one SPPM render uses (cmd_args.hpp defaults)."""
+
+    width: int = 1024
+    height: int = 1024
+    photons_per_round: int = 1 << 20
+    max_photons_per_batch: int = 1 << 20
+    max_photon_bounces: int = 16
+    grid_res: int = 64
+    direct_samples: int = 16
+    seed: int = 2137
+    sppm_alpha: float = 0.7
+    initial_radius: float = 0.0
+
+    def to_c(self) -> RenderArgs:
+        return RenderArgs(self.width, self.height, self.photons_per_round, self.max_photons_per_batch,
+                          self.max_photon_bounces, self.grid_res, self.direct_samples, self.seed,
+                          self.sppm_alpha, self.initial_radius)
+
+
+class Renderer:
+    """HIP pass driver (renderer.cpp:9-161) behind the C ABI."""
+
+    def __init__(self, config: PassConfig, scene):
+        self.config = config
+        self.scene = scene
+        self._args = config.to_c()
+        self._desc = scene.desc()
+        h = C.c_void_p()
+        check(lib().phx_renderer_create(C.byref(self._args), C.byref(self._desc), C.byref(h)), "renderer_create")
+        self._h = h
+        self.n_pixels = config.width * config.height
+        self.bucket_count = config.grid_res ** 3 + 1
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().phx_renderer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def run_pass(self) -> None:
+        check(lib().phx_renderer_run_pass(self._h), "run_pass")
+
+    def run(self, rounds: int) -> None:
+        for _ in range(rounds):
+            self.run_pass()
+        self.synchronize()
+
+    def synchronize(self) -> None:
+        check(lib().phx_renderer_synchronize(self._h), "synchronize")
+
+    def stage_ms(self) -> np.ndarray:
+        out = np.zeros(4, np.float32)
+        check(lib().phx_renderer_last_stage_ms(self._h, ptr(out)), "stage_ms")
+        return out
+
+    @property
+    def last_photon_count(self) -> int:
+        return int(lib().phx_renderer_last_photon_count(self._h))
+
+    @property
+    def rounds_done(self) -> int:
+        return int(lib().phx_renderer_rounds_done(self._h))
+
+    def read_state(self):
+        flux = np.zeros((self.n_pixels, 4), np.float32)
+        weight = np.zeros(self.n_pixels, np.float32)
+        rad = np.zeros((self.n_pixels, 4), np.float32)
+        check(lib().phx_renderer_read_state(self._h, ptr(flux), ptr(weight), ptr(rad)), "read_state")
+        return flux, weight, rad
+
+    def read_hitpoints(self) -> np.ndarray:
+        hp = np.zeros((self.n_pixels, 6, 4), np.float32)
+        check(lib().phx_renderer_read_hitpoints(self._h, ptr(hp)), "read_hitpoints")
+        return hp
+
+    def read_photon_map(self):
+        n = min(self.last_photon_count, self.config.photons_per_round * self.config.max_photon_bounces)
+        pos = np.zeros((n, 4), np.float32)
+        power = np.zeros(n, np.uint32)
+        dirs = np.zeros(n, np.uint32)
+        cs = np.zeros(self.bucket_count, np.uint32)
+        ce = np.zeros(self.bucket_count, np.uint32)
+        check(lib().phx_renderer_read_photon_map(self._h, ptr(pos), ptr(power), ptr(dirs), ptr(cs), ptr(ce)),
+              "read_photon_map")
+        return pos, power, dirs, cs, ce
+
+    def estimate(self) -> np.ndarray:
+        """Linear radiance estimate (output.cpp:52-83 before tonemapping), (H, W, 3)."""
+        rgb = np.zeros((self.n_pixels, 3), np.float32)
+        check(lib().phx_renderer_estimate(self._h, ptr(rgb)), "estimate")
+        return rgb.reshape(self.config.height, self.config.width, 3)
+
+    def device_buffers(self) -> list[int]:
+        out = (C.c_void_p * 20)()
+        check(lib().phx_renderer_device_buffers(self._h, out, 20), "device_buffers")
+        return [int(v or 0) for v in out]
+
+
+def config_for(name: str, **overrides) -> tuple[PassConfig, object, int]:
+    """PassConfig, scene and round count of a named workload (phosphor_amd.scenes.CONFIGS)."""
+    from . import scenes
+
+    spec = dict(scenes.CONFIGS[name])
+    spec.update(overrides)
+    scene = spec["scene"](spec["width"], spec["height"])
+    lo, hi = scene.bounds()
+    extent = float((hi - lo).max()) + 2e-3
+    radius = spec.get("radius")
+    grid = spec.get("grid_res") or (scenes.grid_res_for_radius(extent, radius) if radius else 64)
+    cfg = PassConfig(width=spec["width"], height=spec["height"], photons_per_round=spec["photons_per_round"],
+                     max_photons_per_batch=min(spec["photons_per_round"], 1 << 20),
+                     max_photon_bounces=spec["max_photon_bounces"], grid_res=grid,
+                     direct_samples=spec.get("direct_samples", 16), seed=spec["seed"],
+                     sppm_alpha=spec["sppm_alpha"], initial_radius=float(radius or 0.0))
+    return cfg, scene, spec["rounds"]
+
+
+__all__ = ["Renderer", "PassConfig", "PhosphorError", "lib", "check", "config_for", "abi"]

@@ -1,0 +1,95 @@
+/*
+ * internal.h — launchers and workspace layouts shared by the C ABI, the pass
+ * driver and the kernels.  Not part of the public interface.
+ */
+#ifndef PHX_INTERNAL_H
+#define PHX_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/phosphor_hip.h"
+
+namespace phx {
+
+/* Pointers of one scene on the device (kernel argument block). */
+struct SceneArgs {
+    const PhxTriangleVertices *tv;
+    const PhxTriangleNormals *tn;
+    const PhxTriangleTangents *tt;
+    const PhxTriangleUvs *tuv;
+    const uint32_t *tmat;
+    const PhxBvhNode *nodes;
+    uint32_t n_tris;
+    const PhxTriangleVertices *etv;
+    const PhxTriangleNormals *etn;
+    const PhxTriangleUvs *etuv;
+    const uint32_t *etmat;
+    const PhxMaterial *materials;
+    const PhxUvTransformBlock *uvt;
+    const PhxTextureMeta *tex_meta;
+    const uint8_t *atlas;
+    const PhxLight *lights;
+    uint32_t n_lights;
+    const float *lum_pref_sum;
+    float total_lum;
+    phx_float4 scene_center;
+    float scene_radius;
+    const float *envmap;
+    uint32_t env_w, env_h, has_env;
+};
+
+/* ---- scan (single pass, decoupled look-back) ---- */
+constexpr uint32_t SCAN_TILE = 4096; /* 256 threads × 16 elements */
+size_t scan_workspace_bytes(uint32_t n);
+/* out = exclusive scan of in; out2 (optional) receives a copy; total
+ * (optional, device) receives the sum.  `ws` holds scan_workspace_bytes(n). */
+hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *total, void *ws,
+                       hipStream_t s);
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
+
+hipError_t launch_trace(const PhxCameraParams &cam, uint32_t w, uint32_t h, uint32_t seed, uint32_t direct_samples,
+                        const SceneArgs &sc, phx_float4 *hp_pos, phx_float4 *hp_normal, phx_float4 *hp_throughput,
+                        phx_float4 *hp_base_color, phx_float4 *hp_direct, phx_float4 *hp_emission, hipStream_t s);
+
+size_t emit_workspace_bytes(uint32_t photons_to_emit, uint32_t max_bounces);
+hipError_t launch_emit(const SceneArgs &sc, uint32_t max_photons, uint32_t offset, uint32_t photons_to_emit,
+                       uint32_t max_bounces, uint32_t seed, phx_float4 *photon_pos, uint32_t *photon_power,
+                       uint32_t *photon_dir, uint32_t *n_photons, void *ws, hipStream_t s);
+
+hipError_t launch_hash(const phx_float4 *pos, uint32_t n, const PhxPhotonHashData &d, uint32_t *hashes,
+                       uint32_t *counts, hipStream_t s);
+/* Same with the photon count read on the device (clamped to max_n). */
+hipError_t launch_hash_dev(const phx_float4 *pos, const uint32_t *d_n, uint32_t max_n, const PhxPhotonHashData &d,
+                           uint32_t *hashes, uint32_t *counts, hipStream_t s);
+
+hipError_t launch_scatter_atomic(const phx_float4 *pos_in, const uint32_t *pow_in, const uint32_t *dir_in,
+                                 const uint32_t *hashes, uint32_t *cell_end, uint32_t n, phx_float4 *pos_out,
+                                 uint32_t *pow_out, uint32_t *dir_out, hipStream_t s);
+
+size_t sort_workspace_bytes(uint32_t n, uint32_t n_cells);
+hipError_t launch_sort(const phx_float4 *pos_in, const uint32_t *pow_in, const uint32_t *dir_in,
+                       const uint32_t *hashes, const uint32_t *cell_start, uint32_t *cell_end, uint32_t n,
+                       uint32_t n_cells, phx_float4 *pos_out, uint32_t *pow_out, uint32_t *dir_out, void *ws,
+                       hipStream_t s);
+
+hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const phx_float4 *hp_throughput,
+                         const phx_float4 *hp_base_color, const phx_float4 *hp_direct, const phx_float4 *hp_emission,
+                         uint32_t n_pixels, const phx_float4 *ppos, const uint32_t *ppow, const uint32_t *pdir,
+                         const uint32_t *cell_start, const uint32_t *cell_end, const PhxPhotonHashData &d,
+                         float alpha, phx_float4 *flux, float *photon_weight, phx_float4 *radiance, hipStream_t s);
+
+/* error bookkeeping of the C ABI */
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_status(hipError_t e, const char *what);
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+} // namespace phx
+
+#endif /* PHX_INTERNAL_H */

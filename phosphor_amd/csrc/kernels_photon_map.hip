@@ -1,0 +1,516 @@
+/*
+ * kernels_photon_map.hip — spatial-grid photon map: hash, scan, scatter/sort,
+ * and the per-pixel flux gather with the SPPM update.
+ *
+ * scan: one kernel replaces scan_small_chunks + scan_big_chunks +
+ *   scan_add_offsets + the cell_start→cell_end copy (renderer.cpp:134-147).
+ *   256-thread tiles of 4096 elements; a thread scans 16 consecutive values,
+ *   a wave64 shuffle scan combines thread totals, LDS carries combine the 4
+ *   waves, and tiles are chained with a decoupled look-back over a
+ *   (flag, value) status word per tile.  Tiles are numbered by an atomic
+ *   ticket, so a tile only ever waits on tiles that started before it.  There
+ *   is no 4096² element limit.
+ * sort: stable LSD radix sort of the cell keys (8-bit digits), the payload
+ *   being the input index.  Digit ranks inside a tile come from wave64
+ *   ballot matching (peers with the same digit) plus per-wave running counts
+ *   in LDS, so equal keys keep their input order; the last pass moves the
+ *   24-byte photon records.  Result: the scatter of scatter_photons.cl with
+ *   records inside a cell in input order.
+ * gather: gather_flux.cl + photon_hash.h:56-103, one lane per pixel.
+ */
+#include "phx_device.h"
+
+namespace phx {
+
+/* ---------------- fill ---------------- */
+__global__ void k_fill_u32(u32 *p, u32 v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride)
+        p[i] = v;
+}
+
+hipError_t launch_fill_u32(u32 *p, u32 v, size_t n, hipStream_t s) {
+    if (n == 0)
+        return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096)
+        blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3((u32)blocks), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+/* ---------------- scan ---------------- */
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr unsigned long long FLAG_AGG = 1ull << 32;
+constexpr unsigned long long FLAG_PREFIX = 2ull << 32;
+
+__device__ __forceinline__ u32 pad_idx(u32 i) { return i + (i >> 4); }
+
+__device__ __forceinline__ u32 wave_incl_scan(u32 x, u32 lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        u32 y = __shfl_up(x, d, 64);
+        if (lane >= (u32)d)
+            x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ u32 wave_sum(u32 x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        x += __shfl_xor(x, d, 64);
+    return x;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan(const u32 *__restrict__ in, u32 n, u32 *__restrict__ out,
+                                                      u32 *__restrict__ out2, u32 *__restrict__ total,
+                                                      unsigned long long *status, u32 *ticket, u32 n_tiles) {
+    __shared__ u32 s[SCAN_TILE + SCAN_TILE / 16];
+    __shared__ u32 s_wave[SCAN_BLOCK / 64];
+    __shared__ u32 s_tile, s_prefix;
+    u32 t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    if (t == 0)
+        s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    u32 tile = s_tile;
+    u32 base = tile * SCAN_TILE;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        u32 i = (u32)k * SCAN_BLOCK + t;
+        u32 g = base + i;
+        s[pad_idx(i)] = g < n ? in[g] : 0u;
+    }
+    __syncthreads();
+    u32 v[SCAN_ITEMS];
+    u32 sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        v[k] = s[pad_idx(t * SCAN_ITEMS + k)];
+        sum += v[k];
+    }
+    u32 incl = wave_incl_scan(sum, lane);
+    if (lane == 63)
+        s_wave[w] = incl;
+    __syncthreads();
+    u32 wpre = 0, btotal = 0;
+#pragma unroll
+    for (u32 i = 0; i < SCAN_BLOCK / 64; i++) {
+        u32 c = s_wave[i];
+        wpre += (i < w) ? c : 0u;
+        btotal += c;
+    }
+    u32 texcl = wpre + incl - sum;
+    if (w == 0) {
+        u32 excl = 0;
+        if (tile == 0) {
+            if (lane == 0)
+                __hip_atomic_store(&status[0], FLAG_PREFIX | btotal, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (lane == 0)
+                __hip_atomic_store(&status[tile], FLAG_AGG | btotal, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int pred = (int)tile - 1;
+            for (;;) {
+                int idx = pred - (int)lane;
+                unsigned long long st_ =
+                    (idx >= 0) ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : FLAG_PREFIX;
+                u32 flag = (u32)(st_ >> 32);
+                unsigned long long pmask = __ballot(flag == 2u);
+                unsigned long long zmask = __ballot(flag == 0u);
+                int first = pmask ? (__ffsll((long long)pmask) - 1) : 64;
+                unsigned long long need = (first >= 63) ? ~0ull : ((2ull << first) - 1ull);
+                if (zmask & need)
+                    continue;
+                u32 val = ((int)lane <= first) ? (u32)st_ : 0u;
+                excl += wave_sum(val);
+                if (first < 64)
+                    break;
+                pred -= 64;
+            }
+            if (lane == 0)
+                __hip_atomic_store(&status[tile], FLAG_PREFIX | (u32)(excl + btotal), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) {
+            s_prefix = excl;
+            if (total && tile == n_tiles - 1)
+                *total = excl + btotal;
+        }
+    }
+    __syncthreads();
+    u32 run = s_prefix + texcl;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        s[pad_idx(t * SCAN_ITEMS + k)] = run;
+        run += v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        u32 i = (u32)k * SCAN_BLOCK + t;
+        u32 g = base + i;
+        if (g < n) {
+            u32 r = s[pad_idx(i)];
+            out[g] = r;
+            if (out2)
+                out2[g] = r;
+        }
+    }
+}
+
+__global__ void k_scan_init(u32 *ticket, unsigned long long *status, u32 n_tiles, u32 *total) {
+    u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        *ticket = 0u;
+        if (total)
+            *total = 0u;
+    }
+    for (; i < n_tiles; i += gridDim.x * blockDim.x)
+        status[i] = 0ull;
+}
+
+size_t scan_workspace_bytes(u32 n) {
+    size_t tiles = ((size_t)n + SCAN_TILE - 1) / SCAN_TILE;
+    return 256 + align_up(tiles * sizeof(unsigned long long), 256);
+}
+
+hipError_t launch_scan(const u32 *in, u32 n, u32 *out, u32 *out2, u32 *total, void *ws, hipStream_t s) {
+    u32 n_tiles = (u32)(((size_t)n + SCAN_TILE - 1) / SCAN_TILE);
+    u32 *ticket = (u32 *)ws;
+    unsigned long long *status = (unsigned long long *)((char *)ws + 256);
+    u32 init_blocks = n_tiles ? (n_tiles + 255) / 256 : 1;
+    if (init_blocks > 1024)
+        init_blocks = 1024;
+    hipLaunchKernelGGL(k_scan_init, dim3(init_blocks), dim3(256), 0, s, ticket, status, n_tiles, total);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || n == 0)
+        return e;
+    hipLaunchKernelGGL(k_scan, dim3(n_tiles), dim3(SCAN_BLOCK), 0, s, in, n, out, out2, total, status, ticket,
+                       n_tiles);
+    return hipGetLastError();
+}
+
+/* ---------------- hash (hash_photons.cl) ---------------- */
+__global__ __launch_bounds__(256) void k_hash(const phx_float4 *__restrict__ pos, u32 n, const u32 *d_n, u32 max_n,
+                                              PhxPhotonHashData d, u32 *__restrict__ hashes,
+                                              u32 *__restrict__ counts) {
+    if (d_n) {
+        u32 c = *d_n;
+        n = c < max_n ? c : max_n;
+    }
+    for (u32 i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        phx_float4 p = pos[i];
+        u32 h = get_hash(p.x, p.y, p.z, d);
+        hashes[i] = h;
+        atomicAdd(&counts[h], 1u);
+    }
+}
+
+hipError_t launch_hash(const phx_float4 *pos, u32 n, const PhxPhotonHashData &d, u32 *hashes, u32 *counts,
+                       hipStream_t s) {
+    if (n == 0)
+        return hipSuccess;
+    u32 blocks = (n + 255) / 256;
+    if (blocks > 8192)
+        blocks = 8192;
+    hipLaunchKernelGGL(k_hash, dim3(blocks), dim3(256), 0, s, pos, n, (const u32 *)nullptr, 0u, d, hashes, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_hash_dev(const phx_float4 *pos, const u32 *d_n, u32 max_n, const PhxPhotonHashData &d, u32 *hashes,
+                           u32 *counts, hipStream_t s) {
+    u32 blocks = (max_n + 255) / 256;
+    if (blocks > 8192)
+        blocks = 8192;
+    if (blocks == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_hash, dim3(blocks), dim3(256), 0, s, pos, 0u, d_n, max_n, d, hashes, counts);
+    return hipGetLastError();
+}
+
+/* ---------------- scatter (scatter_photons.cl semantics) ---------------- */
+__global__ __launch_bounds__(256) void k_scatter_atomic(const phx_float4 *__restrict__ pos_in,
+                                                        const u32 *__restrict__ pow_in, const u32 *__restrict__ dir_in,
+                                                        const u32 *__restrict__ hashes, u32 *cell_end, u32 n,
+                                                        phx_float4 *__restrict__ pos_out, u32 *__restrict__ pow_out,
+                                                        u32 *__restrict__ dir_out) {
+    u32 i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    u32 dst = atomicAdd(&cell_end[hashes[i]], 1u);
+    pos_out[dst] = pos_in[i];
+    pow_out[dst] = pow_in[i];
+    dir_out[dst] = dir_in[i];
+}
+
+hipError_t launch_scatter_atomic(const phx_float4 *pos_in, const u32 *pow_in, const u32 *dir_in, const u32 *hashes,
+                                 u32 *cell_end, u32 n, phx_float4 *pos_out, u32 *pow_out, u32 *dir_out,
+                                 hipStream_t s) {
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_atomic, dim3((n + 255) / 256), dim3(256), 0, s, pos_in, pow_in, dir_in, hashes,
+                       cell_end, n, pos_out, pow_out, dir_out);
+    return hipGetLastError();
+}
+
+/* ---------------- stable radix sort by cell ---------------- */
+constexpr int RADIX_BITS = 8;
+constexpr int RADIX = 1 << RADIX_BITS;
+constexpr int SORT_BLOCK = 256;
+constexpr int SORT_ITEMS = 16;
+constexpr u32 SORT_TILE = SORT_BLOCK * SORT_ITEMS; /* 4096 */
+constexpr int SORT_WAVES = SORT_BLOCK / 64;
+
+__global__ __launch_bounds__(SORT_BLOCK) void k_radix_hist(const u32 *__restrict__ keys, u32 n, u32 shift,
+                                                           u32 n_tiles, u32 *__restrict__ hist) {
+    __shared__ u32 h[RADIX];
+    u32 t = threadIdx.x, tile = blockIdx.x;
+    h[t] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; k++) {
+        u32 e = tile * SORT_TILE + (u32)k * SORT_BLOCK + t;
+        if (e < n)
+            atomicAdd(&h[(keys[e] >> shift) & (RADIX - 1)], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)t * n_tiles + tile] = h[t];
+}
+
+/* Element order inside a tile is (wave, item, lane): element
+ * tile·4096 + wave·1024 + item·64 + lane, so ranks computed item by item
+ * within a wave, then offset by the counts of earlier waves, are stable. */
+__global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(
+    const u32 *__restrict__ keys_in, const u32 *__restrict__ vals_in, u32 n, u32 shift, u32 n_tiles,
+    const u32 *__restrict__ offs, u32 *__restrict__ keys_out, u32 *__restrict__ vals_out, int final_pass,
+    const phx_float4 *__restrict__ pos_in, const u32 *__restrict__ pow_in, const u32 *__restrict__ dir_in,
+    phx_float4 *__restrict__ pos_out, u32 *__restrict__ pow_out, u32 *__restrict__ dir_out) {
+    __shared__ u32 cnt[SORT_WAVES][RADIX];
+    u32 t = threadIdx.x, lane = t & 63u, w = t >> 6, tile = blockIdx.x;
+    for (u32 i = t; i < SORT_WAVES * RADIX; i += SORT_BLOCK)
+        (&cnt[0][0])[i] = 0u;
+    __syncthreads();
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    u32 key[SORT_ITEMS], val[SORT_ITEMS], rank[SORT_ITEMS];
+    u32 ebase = tile * SORT_TILE + w * (SORT_ITEMS * 64);
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; k++) {
+        u32 e = ebase + (u32)k * 64 + lane;
+        bool valid = e < n;
+        key[k] = valid ? keys_in[e] : 0u;
+        val[k] = valid ? (vals_in ? vals_in[e] : e) : 0u;
+        u32 d = (key[k] >> shift) & (RADIX - 1);
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < RADIX_BITS; b++) {
+            bool bit = (d >> b) & 1u;
+            unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        u32 below = (u32)__popcll(peers & lt_mask);
+        u32 c = cnt[w][d];
+        rank[k] = c + below;
+        if (valid && below == 0)
+            cnt[w][d] = c + (u32)__popcll(peers);
+    }
+    __syncthreads();
+    {
+        /* cnt[w][d] ← global offset of digit d for this tile + counts of earlier waves */
+        u32 d = t;
+        u32 run = offs[(size_t)d * n_tiles + tile];
+#pragma unroll
+        for (int ww = 0; ww < SORT_WAVES; ww++) {
+            u32 c = cnt[ww][d];
+            cnt[ww][d] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; k++) {
+        u32 e = ebase + (u32)k * 64 + lane;
+        if (e >= n)
+            continue;
+        u32 d = (key[k] >> shift) & (RADIX - 1);
+        u32 dst = cnt[w][d] + rank[k];
+        if (final_pass) {
+            u32 src = val[k];
+            pos_out[dst] = pos_in[src];
+            pow_out[dst] = pow_in[src];
+            dir_out[dst] = dir_in[src];
+        } else {
+            keys_out[dst] = key[k];
+            vals_out[dst] = val[k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cell_end(const u32 *__restrict__ cell_start, u32 *__restrict__ cell_end,
+                                                  u32 n_cells, u32 n) {
+    for (u32 h = blockIdx.x * 256 + threadIdx.x; h < n_cells; h += gridDim.x * 256)
+        cell_end[h] = (h + 1 < n_cells) ? cell_start[h + 1] : n;
+}
+
+static u32 key_bits(u32 n_cells) {
+    u32 maxkey = n_cells > 0 ? n_cells - 1 : 0;
+    u32 bits = 0;
+    while (bits < 32 && (maxkey >> bits) != 0)
+        bits++;
+    return bits < 1 ? 1 : bits;
+}
+
+size_t sort_workspace_bytes(u32 n, u32 n_cells) {
+    (void)n_cells;
+    size_t tiles = ((size_t)n + SORT_TILE - 1) / SORT_TILE;
+    size_t b = 0;
+    b += 4 * align_up((size_t)n * sizeof(u32), 256);        /* keys/vals ping-pong */
+    b += 2 * align_up(tiles * RADIX * sizeof(u32), 256);     /* hist, offs */
+    b += align_up(scan_workspace_bytes((u32)(tiles * RADIX)), 256);
+    return b;
+}
+
+hipError_t launch_sort(const phx_float4 *pos_in, const u32 *pow_in, const u32 *dir_in, const u32 *hashes,
+                       const u32 *cell_start, u32 *cell_end, u32 n, u32 n_cells, phx_float4 *pos_out, u32 *pow_out,
+                       u32 *dir_out, void *ws, hipStream_t s) {
+    hipError_t e;
+    if (n > 0) {
+        u32 n_tiles = (u32)(((size_t)n + SORT_TILE - 1) / SORT_TILE);
+        char *p = (char *)ws;
+        u32 *kbuf[2], *vbuf[2];
+        kbuf[0] = (u32 *)p;
+        p += align_up((size_t)n * sizeof(u32), 256);
+        kbuf[1] = (u32 *)p;
+        p += align_up((size_t)n * sizeof(u32), 256);
+        vbuf[0] = (u32 *)p;
+        p += align_up((size_t)n * sizeof(u32), 256);
+        vbuf[1] = (u32 *)p;
+        p += align_up((size_t)n * sizeof(u32), 256);
+        u32 *hist = (u32 *)p;
+        p += align_up((size_t)n_tiles * RADIX * sizeof(u32), 256);
+        u32 *offs = (u32 *)p;
+        p += align_up((size_t)n_tiles * RADIX * sizeof(u32), 256);
+        void *scan_ws = p;
+        u32 passes = (key_bits(n_cells) + RADIX_BITS - 1) / RADIX_BITS;
+        const u32 *kin = hashes;
+        const u32 *vin = nullptr;
+        for (u32 pass = 0; pass < passes; pass++) {
+            u32 shift = pass * RADIX_BITS;
+            bool last = pass + 1 == passes;
+            hipLaunchKernelGGL(k_radix_hist, dim3(n_tiles), dim3(SORT_BLOCK), 0, s, kin, n, shift, n_tiles, hist);
+            if ((e = hipGetLastError()) != hipSuccess)
+                return e;
+            if ((e = launch_scan(hist, n_tiles * RADIX, offs, nullptr, nullptr, scan_ws, s)) != hipSuccess)
+                return e;
+            u32 *kout = kbuf[pass & 1], *vout = vbuf[pass & 1];
+            hipLaunchKernelGGL(k_radix_scatter, dim3(n_tiles), dim3(SORT_BLOCK), 0, s, kin, vin, n, shift, n_tiles,
+                               offs, kout, vout, last ? 1 : 0, pos_in, pow_in, dir_in, pos_out, pow_out, dir_out);
+            if ((e = hipGetLastError()) != hipSuccess)
+                return e;
+            kin = kout;
+            vin = vout;
+        }
+    }
+    if (n_cells > 0) {
+        u32 blocks = (n_cells + 255) / 256;
+        if (blocks > 4096)
+            blocks = 4096;
+        hipLaunchKernelGGL(k_cell_end, dim3(blocks), dim3(256), 0, s, cell_start, cell_end, n_cells, n);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
+/* ---------------- gather (gather_flux.cl + photon_hash.h:56-103) ---------------- */
+struct GatherArgs {
+    const phx_float4 *hp_pos, *hp_normal, *hp_throughput, *hp_base_color, *hp_direct, *hp_emission;
+    u32 n_pixels;
+    const phx_float4 *ppos;
+    const u32 *ppow, *pdir, *cell_start, *cell_end;
+    PhxPhotonHashData d;
+    f32 alpha;
+    phx_float4 *flux;
+    f32 *photon_weight;
+    phx_float4 *radiance;
+};
+
+__global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
+    u32 tid = blockIdx.x * 256 + threadIdx.x;
+    if (tid >= a.n_pixels)
+        return;
+    F4 ge = ld(a.hp_emission + tid);
+    F4 rsf = ld(a.radiance + tid);
+    F4 r = mk(ge.x, ge.y, ge.z, 0.0f);
+    if (ge.w == 0.0f) {
+        st(a.radiance + tid, rsf + r);
+        return;
+    }
+    F4 dr = ld(a.hp_direct + tid);
+    r.x += dr.x;
+    r.y += dr.y;
+    r.z += dr.z;
+    st(a.radiance + tid, rsf + r);
+    F4 fsf = ld(a.flux + tid);
+    f32 wsf = a.photon_weight[tid];
+    f32 r_sq = fsf.w;
+    const PhxPhotonHashData &d = a.d;
+    if (r_sq < EPS) {
+        f32 hx = d.cell_sizes.x * 0.5f, hy = d.cell_sizes.y * 0.5f, hz = d.cell_sizes.z * 0.5f;
+        r_sq = hx * hx + hy * hy + hz * hz;
+    }
+    F4 nf = BLACK;
+    f32 found = 0.0f;
+    F4 pos = ld(a.hp_pos + tid), sn = ld(a.hp_normal + tid);
+    f32 two_r_sq = 2.0f * r_sq;
+    for (u32 i = 0; i < 27; i++) {
+        i32 cx = (i32)(i % 3) - 1, cy = (i32)((i / 3) % 3) - 1, cz = (i32)((i / 9) % 3) - 1;
+        F4 p2 = pos;
+        p2.x += (f32)cx * d.cell_sizes.x;
+        p2.y += (f32)cy * d.cell_sizes.y;
+        p2.z += (f32)cz * d.cell_sizes.z;
+        u32 h = get_hash(p2.x, p2.y, p2.z, d);
+        if (h == 0)
+            continue;
+        u32 s0 = a.cell_start[h], e0 = a.cell_end[h];
+        for (u32 k = s0; k < e0; k++) {
+            F4 dd = pos - ld(a.ppos + k);
+            f32 dsq = dot(dd, dd);
+            if (dsq >= r_sq)
+                continue;
+            F4 pd = decode_oct(a.pdir[k]);
+            if (dot(pd, sn) > 0.0f)
+                continue;
+            f32 wgt = GAUSS_ALPHA * (1.0f - (1.0f - phx_expf(-GAUSS_BETA * dsq / two_r_sq)) / GAUSS_DENOM);
+            nf = nf + wgt * decode_rgbe(a.ppow[k]);
+            found += wgt;
+        }
+    }
+    if (found > 0.0f) {
+        f32 npw = wsf + a.alpha * found;
+        f32 ratio = npw / (wsf + found);
+        f32 nr = r_sq * ratio;
+        F4 albedo = ld(a.hp_base_color + tid);
+        F4 contrib = ld(a.hp_throughput + tid) * (albedo / PI) * nf;
+        F4 tmp = (fsf + contrib) * ratio;
+        st(a.flux + tid, mk(tmp.x, tmp.y, tmp.z, nr));
+        a.photon_weight[tid] = npw;
+    }
+}
+
+hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const phx_float4 *hp_throughput,
+                         const phx_float4 *hp_base_color, const phx_float4 *hp_direct, const phx_float4 *hp_emission,
+                         u32 n_pixels, const phx_float4 *ppos, const u32 *ppow, const u32 *pdir, const u32 *cell_start,
+                         const u32 *cell_end, const PhxPhotonHashData &d, f32 alpha, phx_float4 *flux,
+                         f32 *photon_weight, phx_float4 *radiance, hipStream_t s) {
+    if (n_pixels == 0)
+        return hipSuccess;
+    GatherArgs a{hp_pos, hp_normal, hp_throughput, hp_base_color, hp_direct, hp_emission, n_pixels, ppos, ppow,
+                 pdir,   cell_start, cell_end,   d,             alpha,     flux,      photon_weight, radiance};
+    hipLaunchKernelGGL(k_gather, dim3((n_pixels + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+} // namespace phx

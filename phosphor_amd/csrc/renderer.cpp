@@ -1,0 +1,594 @@
+/*
+ * renderer.cpp — the SPPM pass driver on HIP.
+ *
+ * Mirrors src/renderer/renderer.cpp (Renderer::set_scene/run/trace_rays/
+ * emit_photons/hash_and_sort_photons/gather_flux) and the buffer owners of
+ * include/buffers/*.hpp (GeometryBuffers, ShadingBuffers, PhotonBuffers,
+ * ImageBuffers, RenderingBuffers), with cl::Buffer replaced by RAII device
+ * allocations and the OpenCL enqueues replaced by the kernels of this
+ * library on one HIP stream.
+ *
+ * Differences from the reference, all internal: the scan is one kernel, the
+ * scatter is the stable sort (canonical photon order), and the deposit count
+ * is read back once per pass exactly where the reference reads it
+ * (renderer.cpp:113-118).
+ */
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "phx_math.h"
+
+namespace phx {
+
+/* cl::Buffer stand-in: one device allocation, freed on destruction */
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() {
+        if (p)
+            (void)hipFree(p);
+    }
+    hipError_t alloc(size_t n) {
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+        bytes = n < 16 ? 16 : n;
+        return hipMalloc(&p, bytes);
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+/* rendering.cpp:3-13: zero items still allocate one item */
+static hipError_t copy_to_dev_buf(DevBuf &b, const void *src, size_t count, size_t item, hipStream_t s) {
+    hipError_t e = b.alloc(count ? count * item : item);
+    if (e != hipSuccess)
+        return e;
+    if (count && src)
+        return hipMemcpyAsync(b.p, src, count * item, hipMemcpyHostToDevice, s);
+    return hipMemsetAsync(b.p, 0, b.bytes, s);
+}
+
+struct GeometryBuffers { /* include/buffers/geometry.hpp */
+    DevBuf d_tri_verts, d_tri_normals, d_tri_tangents, d_tri_uvs, d_tri_mat_index;
+    uint32_t n_triangles = 0;
+    DevBuf d_etri_verts, d_etri_normals, d_etri_uvs, d_etri_mat_index;
+    uint32_t n_e_triangles = 0;
+    DevBuf d_bvh_nodes;
+};
+
+struct ShadingBuffers { /* include/buffers/shading.hpp */
+    DevBuf d_materials, d_uv_transforms;
+    uint32_t n_materials = 0;
+    DevBuf d_lights, d_lum_pref_sum;
+    uint32_t n_lights = 0;
+    float total_lum = 0.0f;
+    DevBuf d_tex_atlas, d_tex_meta;
+    uint32_t n_textures = 0;
+    phx_float4 scene_center{};
+    float scene_radius = 0.0f;
+    DevBuf d_envmap_data;
+    uint32_t envmap_width = 0, envmap_height = 0, has_envmap = 0;
+};
+
+struct PhotonBuffers { /* include/buffers/photon.hpp (sorted photon map) */
+    DevBuf d_photon_pos, d_photon_power, d_photon_dir, d_photon_hashes;
+    DevBuf d_cell_counts, d_cell_start, d_cell_end;
+    uint32_t n_photons = 0;
+};
+
+struct ImageBuffers { /* include/buffers/image.hpp */
+    DevBuf d_hp_pos, d_hp_normal, d_hp_throughput, d_hp_base_color, d_hp_direct, d_hp_emission;
+    DevBuf d_flux, d_photon_weight, d_radiance;
+    PhxCameraParams camera{};
+    uint32_t image_width = 0, image_height = 0, n_pixels = 0;
+};
+
+struct RenderingBuffers {
+    GeometryBuffers geometry_buf;
+    ShadingBuffers shading_buf;
+    PhotonBuffers photon_buf;
+    ImageBuffers image_buf;
+
+    SceneArgs scene_args() const {
+        const GeometryBuffers &g = geometry_buf;
+        const ShadingBuffers &s = shading_buf;
+        SceneArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.tv = g.d_tri_verts.as<PhxTriangleVertices>();
+        a.tn = g.d_tri_normals.as<PhxTriangleNormals>();
+        a.tt = g.d_tri_tangents.as<PhxTriangleTangents>();
+        a.tuv = g.d_tri_uvs.as<PhxTriangleUvs>();
+        a.tmat = g.d_tri_mat_index.as<uint32_t>();
+        a.nodes = g.d_bvh_nodes.as<PhxBvhNode>();
+        a.n_tris = g.n_triangles;
+        a.etv = g.d_etri_verts.as<PhxTriangleVertices>();
+        a.etn = g.d_etri_normals.as<PhxTriangleNormals>();
+        a.etuv = g.d_etri_uvs.as<PhxTriangleUvs>();
+        a.etmat = g.d_etri_mat_index.as<uint32_t>();
+        a.materials = s.d_materials.as<PhxMaterial>();
+        a.uvt = s.d_uv_transforms.as<PhxUvTransformBlock>();
+        a.tex_meta = s.d_tex_meta.as<PhxTextureMeta>();
+        a.atlas = s.d_tex_atlas.as<uint8_t>();
+        a.lights = s.d_lights.as<PhxLight>();
+        a.n_lights = s.n_lights;
+        a.lum_pref_sum = s.d_lum_pref_sum.as<float>();
+        a.total_lum = s.total_lum;
+        a.scene_center = s.scene_center;
+        a.scene_radius = s.scene_radius;
+        a.envmap = s.d_envmap_data.as<float>();
+        a.env_w = s.envmap_width;
+        a.env_h = s.envmap_height;
+        a.has_env = s.has_envmap;
+        return a;
+    }
+};
+
+} // namespace phx
+
+using namespace phx;
+
+struct PhxRenderer {
+    PhxRenderArgs args{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[5] = {};
+    RenderingBuffers buffers;
+    PhxPhotonHashData photon_hash_info{};
+    uint32_t bucket_count = 0;
+    uint32_t photon_buf_size = 0;
+    /* Renderer's own unsorted photon buffers (renderer.hpp:48-51) */
+    DevBuf d_photon_pos, d_photon_power, d_photon_dir, d_batch_size;
+    DevBuf d_emit_ws, d_scan_ws, d_sort_ws;
+    uint32_t rng = 0;
+    uint32_t rounds = 0;
+    uint64_t total_photons_emitted = 0;
+    uint32_t last_count = 0;
+    bool timed = false;
+
+    ~PhxRenderer() {
+        if (stream)
+            (void)hipStreamSynchronize(stream);
+        for (hipEvent_t &e : ev)
+            if (e)
+                (void)hipEventDestroy(e);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+
+    int set_scene(const PhxSceneDesc &scene);
+    int run_pass();
+    int trace_rays(uint32_t seed);
+    int emit_photons(uint32_t seed);
+    int hash_and_sort_photons();
+    int gather_flux();
+};
+
+static uint32_t host_pcg_seed(uint32_t seed) { /* random.h:16-24 */
+    uint32_t s = 0u;
+    s = s * 747796405u + 2891336453u;
+    s += seed;
+    s = s * 747796405u + 2891336453u;
+    return s;
+}
+
+static uint32_t host_random_u32(uint32_t &state) { /* random.h:29-34 */
+    uint32_t old = state;
+    state = state * 747796405u + 2891336453u;
+    return ((old >> ((old >> 28u) + 4u)) ^ old) * 277803737u;
+}
+
+#define PHX_TRY(expr, what)                                                                                            \
+    do {                                                                                                               \
+        hipError_t e_ = (expr);                                                                                        \
+        if (e_ != hipSuccess)                                                                                          \
+            return hip_status(e_, what);                                                                               \
+    } while (0)
+
+/* renderer.cpp:22-57 + rendering.cpp:15-30 + geometry.cpp / shading.cpp / image.cpp / photon.cpp */
+int PhxRenderer::set_scene(const PhxSceneDesc &sc) {
+    hipStream_t s = stream;
+    const PhxRenderArgs &a = args;
+    uint32_t n = sc.n_triangles, ne = sc.n_e_triangles;
+
+    /* Bvh bvh(scene.triangles) */
+    std::vector<PhxBvhNode> nodes;
+    if (sc.bvh_nodes && sc.n_bvh_nodes) {
+        nodes.assign(sc.bvh_nodes, sc.bvh_nodes + sc.n_bvh_nodes);
+    } else {
+        nodes.resize(2 * (size_t)n - 1);
+        uint32_t got = phx_build_bvh_median(sc.tri_verts, n, nodes.data());
+        if (got != nodes.size())
+            return fail(PHX_ERR_INVALID_ARG, "set_scene: BVH build failed");
+    }
+    const PhxBvhNode &root = nodes[0];
+
+    GeometryBuffers &g = buffers.geometry_buf;
+    g.n_triangles = n;
+    g.n_e_triangles = ne;
+    PHX_TRY(copy_to_dev_buf(g.d_tri_verts, sc.tri_verts, n, sizeof(PhxTriangleVertices), s), "upload");
+    PHX_TRY(copy_to_dev_buf(g.d_tri_normals, sc.tri_normals, n, sizeof(PhxTriangleNormals), s), "upload");
+    PHX_TRY(copy_to_dev_buf(g.d_tri_tangents, sc.tri_tangents, n, sizeof(PhxTriangleTangents), s), "upload");
+    PHX_TRY(copy_to_dev_buf(g.d_tri_uvs, sc.tri_uvs, n, sizeof(PhxTriangleUvs), s), "upload");
+    PHX_TRY(copy_to_dev_buf(g.d_tri_mat_index, sc.tri_mat_index, n, sizeof(uint32_t), s), "upload");
+    PHX_TRY(copy_to_dev_buf(g.d_etri_verts, sc.etri_verts, ne, sizeof(PhxTriangleVertices), s), "upload");
+    PHX_TRY(copy_to_dev_buf(g.d_etri_normals, sc.etri_normals, ne, sizeof(PhxTriangleNormals), s), "upload");
+    PHX_TRY(copy_to_dev_buf(g.d_etri_uvs, sc.etri_uvs, ne, sizeof(PhxTriangleUvs), s), "upload");
+    PHX_TRY(copy_to_dev_buf(g.d_etri_mat_index, sc.etri_mat_index, ne, sizeof(uint32_t), s), "upload");
+    PHX_TRY(copy_to_dev_buf(g.d_bvh_nodes, nodes.data(), nodes.size(), sizeof(PhxBvhNode), s), "upload");
+
+    ShadingBuffers &sh = buffers.shading_buf;
+    sh.n_materials = sc.n_materials;
+    sh.n_lights = sc.n_lights;
+    sh.n_textures = sc.n_textures;
+    PHX_TRY(copy_to_dev_buf(sh.d_materials, sc.materials, sc.n_materials, sizeof(PhxMaterial), s), "upload");
+    PHX_TRY(copy_to_dev_buf(sh.d_uv_transforms, sc.uv_transforms, sc.n_materials, sizeof(PhxUvTransformBlock), s),
+            "upload");
+    PHX_TRY(copy_to_dev_buf(sh.d_lights, sc.lights, sc.n_lights, sizeof(PhxLight), s), "upload");
+    /* scene_data.cpp:14-23 luminance prefix sum; shading.cpp:21-23 total */
+    std::vector<float> lum;
+    float running = 0.0f;
+    for (uint32_t i = 0; i < sc.n_lights; i++) {
+        const PhxLight &l = sc.lights[i];
+        float v = l.power.x * 0.2126f + l.power.y * 0.7152f + l.power.z * 0.0722f;
+        running += std::max(v, 0.0f);
+        lum.push_back(running);
+    }
+    sh.total_lum = lum.empty() ? 0.0f : lum.back();
+    PHX_TRY(copy_to_dev_buf(sh.d_lum_pref_sum, lum.data(), lum.size(), sizeof(float), s), "upload");
+    /* shading.cpp:25-43 texture atlas (never empty) */
+    if (sc.n_textures) {
+        PHX_TRY(copy_to_dev_buf(sh.d_tex_meta, sc.tex_meta, sc.n_textures, sizeof(PhxTextureMeta), s), "upload");
+        PHX_TRY(copy_to_dev_buf(sh.d_tex_atlas, sc.tex_atlas, (size_t)sc.tex_atlas_bytes, 1, s), "upload");
+    } else {
+        PhxTextureMeta m{};
+        uint8_t zero = 0;
+        PHX_TRY(copy_to_dev_buf(sh.d_tex_meta, &m, 1, sizeof(PhxTextureMeta), s), "upload");
+        PHX_TRY(copy_to_dev_buf(sh.d_tex_atlas, &zero, 1, 1, s), "upload");
+    }
+    /* shading.cpp:45-58 bounding sphere of the root bbox */
+    float cx = 0.5f * (root.min.x + root.max.x), cy = 0.5f * (root.min.y + root.max.y),
+          cz = 0.5f * (root.min.z + root.max.z);
+    sh.scene_center = phx_float4{cx, cy, cz, 0.0f};
+    float radius = 0.0f;
+    for (int c = 0; c < 8; c++) {
+        float dx = ((c & 1) ? root.max.x : root.min.x) - cx;
+        float dy = ((c & 2) ? root.max.y : root.min.y) - cy;
+        float dz = ((c & 4) ? root.max.z : root.min.z) - cz;
+        radius = std::max(radius, phx_sqrtf(dx * dx + dy * dy + dz * dz));
+    }
+    sh.scene_radius = radius;
+    /* shading.cpp:60-71 envmap */
+    if (sc.envmap && sc.envmap_width && sc.envmap_height) {
+        sh.has_envmap = 1;
+        sh.envmap_width = sc.envmap_width;
+        sh.envmap_height = sc.envmap_height;
+        PHX_TRY(copy_to_dev_buf(sh.d_envmap_data, sc.envmap, 3ull * sc.envmap_width * sc.envmap_height, sizeof(float),
+                                s),
+                "upload");
+    } else {
+        sh.has_envmap = 0;
+        sh.envmap_width = sh.envmap_height = 0;
+        PHX_TRY(copy_to_dev_buf(sh.d_envmap_data, nullptr, 0, sizeof(float), s), "upload");
+    }
+
+    /* alloc_image_buffers (image.cpp:11-21, zero-filled) */
+    ImageBuffers &im = buffers.image_buf;
+    im.camera = *sc.camera;
+    im.image_width = a.width;
+    im.image_height = a.height;
+    im.n_pixels = a.width * a.height;
+    size_t np = im.n_pixels;
+    DevBuf *f4bufs[] = {&im.d_hp_pos,   &im.d_hp_normal, &im.d_hp_throughput, &im.d_hp_base_color,
+                        &im.d_hp_direct, &im.d_hp_emission, &im.d_flux,        &im.d_radiance};
+    for (DevBuf *b : f4bufs) {
+        PHX_TRY(b->alloc(np * sizeof(phx_float4)), "alloc image");
+        PHX_TRY(hipMemsetAsync(b->p, 0, b->bytes, s), "clear image");
+    }
+    PHX_TRY(im.d_photon_weight.alloc(np * sizeof(float)), "alloc image");
+    PHX_TRY(hipMemsetAsync(im.d_photon_weight.p, 0, im.d_photon_weight.bytes, s), "clear image");
+    if (a.initial_radius > 0.0f) {
+        /* explicit initial radius: flux.w starts at r0² instead of 0 */
+        std::vector<phx_float4> f(np, phx_float4{0.0f, 0.0f, 0.0f, a.initial_radius * a.initial_radius});
+        PHX_TRY(hipMemcpyAsync(im.d_flux.p, f.data(), np * sizeof(phx_float4), hipMemcpyHostToDevice, s), "init flux");
+        PHX_TRY(hipStreamSynchronize(s), "init flux");
+    }
+
+    /* renderer.cpp:37-52 */
+    phx_build_photon_hash_info(&root, a.grid_res, &photon_hash_info);
+    bucket_count = a.grid_res * a.grid_res * a.grid_res + 1;
+    uint64_t buf = (uint64_t)a.photons_per_round * a.max_photon_bounces;
+    if (buf > 0xFFFFFFFFull)
+        return fail(PHX_ERR_TOO_LARGE, "set_scene: photons_per_round * max_bounces overflows u32");
+    photon_buf_size = (uint32_t)buf;
+    PHX_TRY(d_photon_pos.alloc((size_t)photon_buf_size * sizeof(phx_float4)), "alloc photons");
+    PHX_TRY(d_photon_power.alloc((size_t)photon_buf_size * sizeof(uint32_t)), "alloc photons");
+    PHX_TRY(d_photon_dir.alloc((size_t)photon_buf_size * sizeof(uint32_t)), "alloc photons");
+    PHX_TRY(d_batch_size.alloc(sizeof(uint32_t)), "alloc counter");
+    PhotonBuffers &p = buffers.photon_buf;
+    PHX_TRY(p.d_photon_pos.alloc((size_t)photon_buf_size * sizeof(phx_float4)), "alloc photon map");
+    PHX_TRY(p.d_photon_power.alloc((size_t)photon_buf_size * sizeof(uint32_t)), "alloc photon map");
+    PHX_TRY(p.d_photon_dir.alloc((size_t)photon_buf_size * sizeof(uint32_t)), "alloc photon map");
+    PHX_TRY(p.d_photon_hashes.alloc((size_t)photon_buf_size * sizeof(uint32_t)), "alloc photon map");
+    PHX_TRY(p.d_cell_counts.alloc((size_t)bucket_count * sizeof(uint32_t)), "alloc cells");
+    PHX_TRY(p.d_cell_start.alloc((size_t)bucket_count * sizeof(uint32_t)), "alloc cells");
+    PHX_TRY(p.d_cell_end.alloc((size_t)bucket_count * sizeof(uint32_t)), "alloc cells");
+    uint32_t per_batch = std::min(a.photons_per_round, a.max_photons_per_batch ? a.max_photons_per_batch : 1u);
+    PHX_TRY(d_emit_ws.alloc(emit_workspace_bytes(per_batch, a.max_photon_bounces)), "alloc emit workspace");
+    PHX_TRY(d_scan_ws.alloc(scan_workspace_bytes(bucket_count)), "alloc scan workspace");
+    PHX_TRY(d_sort_ws.alloc(sort_workspace_bytes(photon_buf_size, bucket_count)), "alloc sort workspace");
+
+    rng = host_pcg_seed(a.seed);
+    PHX_TRY(hipStreamSynchronize(s), "set_scene");
+    return PHX_OK;
+}
+
+/* renderer.cpp:90-95 */
+int PhxRenderer::trace_rays(uint32_t seed) {
+    const ImageBuffers &i = buffers.image_buf;
+    return hip_status(launch_trace(i.camera, i.image_width, i.image_height, seed, args.direct_samples,
+                                   buffers.scene_args(), i.d_hp_pos.as<phx_float4>(), i.d_hp_normal.as<phx_float4>(),
+                                   i.d_hp_throughput.as<phx_float4>(), i.d_hp_base_color.as<phx_float4>(),
+                                   i.d_hp_direct.as<phx_float4>(), i.d_hp_emission.as<phx_float4>(), stream),
+                      "trace_rays");
+}
+
+/* renderer.cpp:97-121 */
+int PhxRenderer::emit_photons(uint32_t seed) {
+    PHX_TRY(launch_fill_u32(d_batch_size.as<uint32_t>(), 0u, 1, stream), "emit_photons: reset counter");
+    uint32_t ppr = args.photons_per_round;
+    uint32_t per_batch = std::min(ppr, args.max_photons_per_batch ? args.max_photons_per_batch : 1u);
+    SceneArgs sc = buffers.scene_args();
+    for (uint32_t off = 0; off < ppr; off += per_batch) {
+        uint32_t to_emit = std::min(per_batch, ppr - off);
+        PHX_TRY(launch_emit(sc, photon_buf_size, off, to_emit, args.max_photon_bounces, seed,
+                            d_photon_pos.as<phx_float4>(), d_photon_power.as<uint32_t>(), d_photon_dir.as<uint32_t>(),
+                            d_batch_size.as<uint32_t>(), d_emit_ws.p, stream),
+                "emit_photons");
+    }
+    uint32_t count = 0;
+    PHX_TRY(hipMemcpyAsync(&count, d_batch_size.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream), "emit readback");
+    PHX_TRY(hipStreamSynchronize(stream), "emit readback");
+    last_count = count;
+    uint32_t n = std::min(count, photon_buf_size);
+    if (n == 0)
+        return fail(PHX_ERR_NO_PHOTONS, "no photon hit"); /* renderer.cpp:117-118 */
+    buffers.photon_buf.n_photons = n;
+    return PHX_OK;
+}
+
+/* renderer.cpp:123-153 */
+int PhxRenderer::hash_and_sort_photons() {
+    PhotonBuffers &p = buffers.photon_buf;
+    uint32_t n = p.n_photons;
+    PHX_TRY(launch_fill_u32(p.d_cell_counts.as<uint32_t>(), 0u, bucket_count, stream), "clear cell counts");
+    PHX_TRY(launch_hash(d_photon_pos.as<phx_float4>(), n, photon_hash_info, p.d_photon_hashes.as<uint32_t>(),
+                        p.d_cell_counts.as<uint32_t>(), stream),
+            "hash_photons");
+    PHX_TRY(launch_scan(p.d_cell_counts.as<uint32_t>(), bucket_count, p.d_cell_start.as<uint32_t>(),
+                        p.d_cell_end.as<uint32_t>(), nullptr, d_scan_ws.p, stream),
+            "scan_cells");
+    PHX_TRY(launch_sort(d_photon_pos.as<phx_float4>(), d_photon_power.as<uint32_t>(), d_photon_dir.as<uint32_t>(),
+                        p.d_photon_hashes.as<uint32_t>(), p.d_cell_start.as<uint32_t>(),
+                        p.d_cell_end.as<uint32_t>(), n, bucket_count, p.d_photon_pos.as<phx_float4>(),
+                        p.d_photon_power.as<uint32_t>(), p.d_photon_dir.as<uint32_t>(), d_sort_ws.p, stream),
+            "sort_photons");
+    return PHX_OK;
+}
+
+/* renderer.cpp:155-161 */
+int PhxRenderer::gather_flux() {
+    const ImageBuffers &i = buffers.image_buf;
+    const PhotonBuffers &p = buffers.photon_buf;
+    return hip_status(launch_gather(i.d_hp_pos.as<phx_float4>(), i.d_hp_normal.as<phx_float4>(),
+                                    i.d_hp_throughput.as<phx_float4>(), i.d_hp_base_color.as<phx_float4>(),
+                                    i.d_hp_direct.as<phx_float4>(), i.d_hp_emission.as<phx_float4>(), i.n_pixels,
+                                    p.d_photon_pos.as<phx_float4>(), p.d_photon_power.as<uint32_t>(),
+                                    p.d_photon_dir.as<uint32_t>(), p.d_cell_start.as<uint32_t>(),
+                                    p.d_cell_end.as<uint32_t>(), photon_hash_info, args.sppm_alpha,
+                                    i.d_flux.as<phx_float4>(), i.d_photon_weight.as<float>(),
+                                    i.d_radiance.as<phx_float4>(), stream),
+                      "gather_flux");
+}
+
+/* renderer.cpp:64-78, one round */
+int PhxRenderer::run_pass() {
+    uint32_t round_seed = host_random_u32(rng);
+    int rc;
+    PHX_TRY(hipEventRecord(ev[0], stream), "event");
+    if ((rc = trace_rays(round_seed)) != PHX_OK)
+        return rc;
+    PHX_TRY(hipEventRecord(ev[1], stream), "event");
+    if ((rc = emit_photons(round_seed)) != PHX_OK)
+        return rc;
+    PHX_TRY(hipEventRecord(ev[2], stream), "event");
+    if ((rc = hash_and_sort_photons()) != PHX_OK)
+        return rc;
+    PHX_TRY(hipEventRecord(ev[3], stream), "event");
+    if ((rc = gather_flux()) != PHX_OK)
+        return rc;
+    PHX_TRY(hipEventRecord(ev[4], stream), "event");
+    timed = true;
+    rounds++;
+    total_photons_emitted += args.photons_per_round;
+    return PHX_OK;
+}
+
+extern "C" {
+
+int phx_renderer_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, PhxRenderer **out) {
+    if (!args || !scene || !out)
+        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: null argument");
+    *out = nullptr;
+    if (scene->n_triangles == 0 || !scene->tri_verts || !scene->tri_normals || !scene->tri_tangents ||
+        !scene->tri_uvs || !scene->tri_mat_index || !scene->materials || scene->n_materials == 0 || !scene->camera)
+        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: empty or incomplete scene (main.cpp:40-47)");
+    if (args->width == 0 || args->height == 0 || args->photons_per_round == 0)
+        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: zero image or photon count");
+    if (args->grid_res == 0 || args->grid_res > 255)
+        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: grid_res outside 1..255 (cmd_args.hpp:51)");
+    if (args->max_photon_bounces == 0 || args->max_photon_bounces > 32)
+        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: max_bounces outside 1..32 (cmd_args.hpp:49)");
+    if (args->direct_samples == 0 || args->direct_samples > 128)
+        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: direct_samples outside 1..128 (cmd_args.hpp:59)");
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
+        return fail(PHX_ERR_NO_DEVICE, "phx_renderer_create: no HIP device");
+    std::unique_ptr<PhxRenderer> r(new PhxRenderer());
+    r->args = *args;
+    hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+    if (e != hipSuccess)
+        return hip_status(e, "phx_renderer_create: stream");
+    for (hipEvent_t &ev : r->ev)
+        if ((e = hipEventCreate(&ev)) != hipSuccess)
+            return hip_status(e, "phx_renderer_create: event");
+    int rc = r->set_scene(*scene);
+    if (rc != PHX_OK)
+        return rc;
+    *out = r.release();
+    return PHX_OK;
+}
+
+void phx_renderer_destroy(PhxRenderer *r) { delete r; }
+
+int phx_renderer_run_pass(PhxRenderer *r) {
+    if (!r)
+        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_run_pass: null renderer");
+    return r->run_pass();
+}
+
+int phx_renderer_synchronize(PhxRenderer *r) {
+    if (!r)
+        return fail(PHX_ERR_INVALID_ARG, "null renderer");
+    return hip_status(hipStreamSynchronize(r->stream), "phx_renderer_synchronize");
+}
+
+int phx_renderer_last_stage_ms(PhxRenderer *r, float out_ms[4]) {
+    if (!r || !out_ms)
+        return fail(PHX_ERR_INVALID_ARG, "null argument");
+    if (!r->timed) {
+        for (int k = 0; k < 4; k++)
+            out_ms[k] = 0.0f;
+        return PHX_OK;
+    }
+    hipError_t e = hipEventSynchronize(r->ev[4]);
+    if (e != hipSuccess)
+        return hip_status(e, "stage timing");
+    for (int k = 0; k < 4; k++)
+        if ((e = hipEventElapsedTime(&out_ms[k], r->ev[k], r->ev[k + 1])) != hipSuccess)
+            return hip_status(e, "stage timing");
+    return PHX_OK;
+}
+
+uint32_t phx_renderer_last_photon_count(PhxRenderer *r) { return r ? r->last_count : 0u; }
+uint32_t phx_renderer_rounds_done(PhxRenderer *r) { return r ? r->rounds : 0u; }
+
+int phx_renderer_read_state(PhxRenderer *r, phx_float4 *flux, float *photon_weight, phx_float4 *radiance) {
+    if (!r)
+        return fail(PHX_ERR_INVALID_ARG, "null renderer");
+    const ImageBuffers &i = r->buffers.image_buf;
+    size_t np = i.n_pixels;
+    hipStream_t s = r->stream;
+    if (flux)
+        PHX_TRY(hipMemcpyAsync(flux, i.d_flux.p, np * sizeof(phx_float4), hipMemcpyDeviceToHost, s), "read state");
+    if (photon_weight)
+        PHX_TRY(hipMemcpyAsync(photon_weight, i.d_photon_weight.p, np * sizeof(float), hipMemcpyDeviceToHost, s),
+                "read state");
+    if (radiance)
+        PHX_TRY(hipMemcpyAsync(radiance, i.d_radiance.p, np * sizeof(phx_float4), hipMemcpyDeviceToHost, s),
+                "read state");
+    return hip_status(hipStreamSynchronize(s), "read state");
+}
+
+int phx_renderer_read_hitpoints(PhxRenderer *r, phx_float4 *hp6) {
+    if (!r || !hp6)
+        return fail(PHX_ERR_INVALID_ARG, "null argument");
+    const ImageBuffers &i = r->buffers.image_buf;
+    size_t np = i.n_pixels;
+    std::vector<phx_float4> tmp(np);
+    const DevBuf *b[6] = {&i.d_hp_pos,        &i.d_hp_normal, &i.d_hp_throughput,
+                          &i.d_hp_base_color, &i.d_hp_direct, &i.d_hp_emission};
+    for (int k = 0; k < 6; k++) {
+        PHX_TRY(hipMemcpyAsync(tmp.data(), b[k]->p, np * sizeof(phx_float4), hipMemcpyDeviceToHost, r->stream),
+                "read hitpoints");
+        PHX_TRY(hipStreamSynchronize(r->stream), "read hitpoints");
+        for (size_t p = 0; p < np; p++)
+            hp6[p * 6 + k] = tmp[p];
+    }
+    return PHX_OK;
+}
+
+int phx_renderer_read_photon_map(PhxRenderer *r, phx_float4 *pos, uint32_t *power, uint32_t *dir,
+                                 uint32_t *cell_start, uint32_t *cell_end) {
+    if (!r)
+        return fail(PHX_ERR_INVALID_ARG, "null renderer");
+    const PhotonBuffers &p = r->buffers.photon_buf;
+    size_t n = p.n_photons, nb = r->bucket_count;
+    hipStream_t s = r->stream;
+    if (pos)
+        PHX_TRY(hipMemcpyAsync(pos, p.d_photon_pos.p, n * sizeof(phx_float4), hipMemcpyDeviceToHost, s), "read map");
+    if (power)
+        PHX_TRY(hipMemcpyAsync(power, p.d_photon_power.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read map");
+    if (dir)
+        PHX_TRY(hipMemcpyAsync(dir, p.d_photon_dir.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read map");
+    if (cell_start)
+        PHX_TRY(hipMemcpyAsync(cell_start, p.d_cell_start.p, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+                "read map");
+    if (cell_end)
+        PHX_TRY(hipMemcpyAsync(cell_end, p.d_cell_end.p, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+                "read map");
+    return hip_status(hipStreamSynchronize(s), "read map");
+}
+
+int phx_renderer_device_buffers(PhxRenderer *r, void **ptrs, uint32_t n_ptrs) {
+    if (!r || !ptrs)
+        return fail(PHX_ERR_INVALID_ARG, "null argument");
+    const ImageBuffers &i = r->buffers.image_buf;
+    const PhotonBuffers &p = r->buffers.photon_buf;
+    void *all[] = {i.d_hp_pos.p,       i.d_hp_normal.p,     i.d_hp_throughput.p, i.d_hp_base_color.p,
+                   i.d_hp_direct.p,    i.d_hp_emission.p,   i.d_flux.p,          i.d_photon_weight.p,
+                   i.d_radiance.p,     p.d_photon_pos.p,    p.d_photon_power.p,  p.d_photon_dir.p,
+                   p.d_cell_start.p,   p.d_cell_end.p,      r->d_photon_pos.p,   r->d_photon_power.p,
+                   r->d_photon_dir.p,  p.d_photon_hashes.p, p.d_cell_counts.p,   (void *)r->stream};
+    uint32_t total = (uint32_t)(sizeof(all) / sizeof(all[0]));
+    for (uint32_t k = 0; k < n_ptrs && k < total; k++)
+        ptrs[k] = all[k];
+    return PHX_OK;
+}
+
+int phx_renderer_estimate(PhxRenderer *r, float *rgb) {
+    if (!r || !rgb)
+        return fail(PHX_ERR_INVALID_ARG, "null argument");
+    size_t np = r->buffers.image_buf.n_pixels;
+    std::vector<phx_float4> flux(np), rad(np);
+    int rc = phx_renderer_read_state(r, flux.data(), nullptr, rad.data());
+    if (rc != PHX_OK)
+        return rc;
+    /* output.cpp:52-83 */
+    const float PI_F = 3.141592654f;
+    float inv_ph = r->total_photons_emitted > 0 ? 1.0f / (float)r->total_photons_emitted : 0.0f;
+    float inv_rd = r->rounds > 0 ? 1.0f / (float)r->rounds : 0.0f;
+    for (size_t p = 0; p < np; p++) {
+        float area = PI_F * flux[p].w;
+        float ir = 0.0f, ig = 0.0f, ib = 0.0f;
+        if (area > 0.0f) {
+            float pa = inv_ph / area;
+            ir = flux[p].x * pa;
+            ig = flux[p].y * pa;
+            ib = flux[p].z * pa;
+        }
+        rgb[3 * p + 0] = rad[p].x * inv_rd + ir;
+        rgb[3 * p + 1] = rad[p].y * inv_rd + ig;
+        rgb[3 * p + 2] = rad[p].z * inv_rd + ib;
+    }
+    return PHX_OK;
+}
+
+} /* extern "C" */
