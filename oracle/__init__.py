@@ -44,6 +44,7 @@ _SIGS = {
     "ref_emit_range": (U32, VP, U32, U32, U32, VP, VP, VP, U32),
     "ref_renderer_run_pass": (INT, VP),
     "ref_renderer_next_seed": (U32, VP),
+    "ref_renderer_sample_light": (U32, VP, U32, U32, VP, VP, VP),
     "ref_renderer_stage_ms": (None, VP, VP),
     "ref_renderer_visits": (None, VP, VP),
     "ref_renderer_last_photon_count": (U32, VP),
@@ -118,6 +119,11 @@ class RefRenderer:
 
     def emit_range(self, seed: int, first: int, count: int) -> int:
         return int(lib().ref_emit_range(self._h, seed, first, count, None, None, None, 0))
+
+    def sample_light(self, seed: int, tid: int):
+        o, d, p = (np.zeros(4, np.float32) for _ in range(3))
+        lib().ref_renderer_sample_light(self._h, seed, tid, _p(o), _p(d), _p(p))
+        return o, d, p
 
     def stage_ms(self) -> np.ndarray:
         out = np.zeros(4, np.float64)

@@ -1712,6 +1712,18 @@ int ref_renderer_run_pass(RefRenderer *r) {
     return 0;
 }
 
+/* sample_light (light_sampling.h:200-227) of photon `tid` with pass seed `seed`:
+ * writes origin, dir, power (4 floats each); returns the RNG state after. */
+u32 ref_renderer_sample_light(RefRenderer *r, u32 seed, u32 tid, f32 *o4, f32 *d4, f32 *p4) {
+    Rng rng = pcg_seed(seed + tid);
+    V4 o, d, p;
+    sample_light(&rng, r->sc, &o, &d, &p);
+    std::memcpy(o4, &o, 16);
+    std::memcpy(d4, &d, 16);
+    std::memcpy(p4, &p, 16);
+    return rng.state;
+}
+
 /* Skip a round's seed without rendering (keeps the seed schedule aligned). */
 u32 ref_renderer_next_seed(RefRenderer *r) { return random_u32(&r->rng); }
 

@@ -230,6 +230,14 @@ class SceneBuilder:
         tris = (v[f] * f32(radius) + np.asarray(center, f32)).astype(f32)
         self.add_mesh(tris, mat, normals=v[f])
 
+    def add_point_light(self, position, power) -> None:
+        """light.cpp:4-11 make_point_light."""
+        light = np.zeros(1, LIGHT_DTYPE)[0]
+        light["kind"] = 0
+        light["position"][:3] = position
+        light["power"][:3] = power
+        self.lights.append(light)
+
     def add_directional_light(self, direction, power) -> None:
         """light.cpp:24-36 make_directional_light (tangent frame of utils.h:44-53)."""
         d = np.asarray(direction, f32)

@@ -1,0 +1,96 @@
+"""CPU checks of the drop-in boundary: the C-ABI library loads without a GPU,
+exports every entry point include/phosphor_hip.h declares, the struct layouts
+match the reference's, and calls fail cleanly (codes, messages) without a
+device.  No kernel is launched here."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import phosphor_amd as P
+from phosphor_amd import abi
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "phosphor_hip.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(phx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_expected_entry_points():
+    names = declared_functions()
+    for must in ("phx_trace_rays", "phx_emit_photons", "phx_hash_photons", "phx_scan_cells",
+                 "phx_scatter_photons", "phx_sort_photons", "phx_gather_flux", "phx_renderer_create",
+                 "phx_renderer_run_pass"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(phx_lib):
+    exported = subprocess.run(["nm", "-D", "--defined-only", str(P.LIB_PATH)], capture_output=True, text=True,
+                              check=True).stdout
+    syms = {line.split()[-1] for line in exported.splitlines() if line.strip()}
+    missing = [n for n in declared_functions() if n not in syms]
+    assert not missing, missing
+    # and the Python binding knows the signature of each
+    assert set(declared_functions()) == set(abi.SIGNATURES), set(declared_functions()) ^ set(abi.SIGNATURES)
+
+
+def test_struct_layouts_match_reference():
+    # sizes of include/common/*.h (camera.h:7-22, material.h:36-59, light.h:14-31, photon_hash.h:11-19)
+    assert abi.CAMERA_DTYPE.itemsize == 112 and abi.CAMERA_DTYPE.fields["defocus_angle"][1] == 96
+    assert abi.MATERIAL_DTYPE.itemsize == 96 and abi.MATERIAL_DTYPE.fields["metallic"][1] == 48
+    assert abi.MATERIAL_DTYPE.fields["diff_index"][1] == 72
+    assert abi.LIGHT_DTYPE.itemsize == 128 and abi.LIGHT_DTYPE.fields["kind"][1] == 112
+    assert abi.HASH_DTYPE.itemsize == 48 and abi.HASH_DTYPE.fields["grid_res"][1] == 32
+    assert abi.UV_BLOCK_DTYPE.itemsize == 192 and abi.TEX_META_DTYPE.itemsize == 16
+
+
+def test_version_and_error_paths_without_device(phx_lib):
+    assert phx_lib.phx_version().startswith(b"phosphor")
+    d = np.zeros(1, abi.HASH_DTYPE)
+    # argument validation happens before any device work
+    assert phx_lib.phx_hash_photons(None, 4, d.ctypes.data, None, None, None) == abi.PHX_ERR_INVALID_ARG
+    assert b"null" in phx_lib.phx_last_error()
+    d["grid_res"] = 300
+    assert phx_lib.phx_hash_photons(None, 0, d.ctypes.data, None, None, None) == abi.PHX_ERR_INVALID_ARG
+    assert phx_lib.phx_scan_cells(None, 0, None, None, None, 0, None) == abi.PHX_ERR_WORKSPACE
+    assert phx_lib.phx_emit_photons_workspace_size(1 << 20, 16) > (1 << 20) * 16 * 24
+    assert phx_lib.phx_sort_photons_workspace_size(1 << 20, 262145) >= 4 * 4 * (1 << 20)
+
+
+def test_renderer_rejects_bad_arguments(phx_lib):
+    from phosphor_amd import PassConfig, scenes
+
+    sc = scenes.cornell_box(8, 8)
+    bad = PassConfig(width=8, height=8, photons_per_round=16, grid_res=300)
+    h = C.c_void_p()
+    args, desc = bad.to_c(), sc.desc()
+    rc = phx_lib.phx_renderer_create(C.byref(args), C.byref(desc), C.byref(h))
+    assert rc == abi.PHX_ERR_INVALID_ARG and not h.value
+    assert b"grid_res" in phx_lib.phx_last_error()
+
+
+def test_bvh_host_builder_matches_oracle(phx_lib, oracle_lib):
+    from phosphor_amd import scenes
+
+    for sc in (scenes.cornell_box(8, 8), scenes.caustic_scene(8, 8, level=3)):
+        n = sc.n_triangles
+        a = np.zeros((2 * n - 1, 8), np.float32)
+        b = np.zeros((2 * n - 1, 8), np.float32)
+        assert phx_lib.phx_build_bvh_median(sc.tri_verts.ctypes.data, n, a.ctypes.data) == 2 * n - 1
+        assert oracle_lib.ref_build_bvh(sc.tri_verts.ctypes.data, n, b.ctypes.data) == 2 * n - 1
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_no_device_reported_here(phx_lib):
+    import os
+
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    assert phx_lib.phx_device_count() == 0

@@ -1,0 +1,104 @@
+"""GPU parity of the kernel-level C-ABI entry points phx_trace_rays,
+phx_emit_photons and phx_gather_flux, called with the reference's argument
+lists (trace_rays.cl:14-40, emit_photons.cl:12-32, gather_flux.cl:5-22) on
+buffers uploaded by the test, against the oracle."""
+import numpy as np
+import pytest
+
+import phosphor_amd as P
+from phosphor_amd import PassConfig, scenes
+
+import oracle
+
+from tests.gpu_util import dptr, empty_dev, from_dev, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+class DeviceScene:
+    """The scene's buffers on the device, laid out as RenderingBuffers binds them."""
+
+    def __init__(self, sc, ref):
+        self.sc = sc
+        self.tv, self.tn, self.tt, self.tuv = (to_dev(a) for a in (sc.tri_verts, sc.tri_normals, sc.tri_tangents,
+                                                                   sc.tri_uvs))
+        self.tmat = to_dev(sc.tri_mat_index)
+        self.nodes = to_dev(ref.bvh_nodes())
+        self.etv, self.etn, self.etuv, self.etmat = (to_dev(a) for a in (sc.etri_verts, sc.etri_normals,
+                                                                         sc.etri_uvs, sc.etri_mat_index))
+        self.mats, self.uvt, self.lights = to_dev(sc.materials), to_dev(sc.uv_transforms), to_dev(sc.lights)
+        lum = []
+        run = np.float32(0)
+        for p in sc.lights["power"]:
+            v = p[0] * np.float32(0.2126) + p[1] * np.float32(0.7152) + p[2] * np.float32(0.0722)
+            run = np.float32(run + max(v, np.float32(0)))
+            lum.append(run)
+        self.lum_host = np.array(lum, np.float32)
+        self.lum = to_dev(self.lum_host)
+        self.total_lum = float(self.lum_host[-1])
+        self.tex_meta = to_dev(np.zeros(4, np.uint32))
+        self.atlas = to_dev(np.zeros(16, np.uint8))
+        c, r = ref.scene_sphere()
+        self.center = c.astype(np.float32)
+        self.radius = r
+        self.n = sc.n_triangles
+
+
+def test_trace_emit_gather_entry_points():
+    sc = scenes.caustic_scene(48, 40, level=3)
+    cfg = PassConfig(width=48, height=40, photons_per_round=6000, max_photons_per_batch=6000, max_photon_bounces=12,
+                     grid_res=60, seed=11, initial_radius=0.015)
+    ref = oracle.RefRenderer(cfg, sc)
+    assert ref.run_pass() == 0  # seed of pass 1 drawn from pcg_seed(11)
+    ds = DeviceScene(sc, ref)
+    L = P.lib()
+    seed = oracle.lib().ref_random_u32(np.array([oracle.lib().ref_pcg_seed(11)], np.uint32).ctypes.data)
+    n_px = cfg.width * cfg.height
+    hp = [empty_dev(16 * n_px) for _ in range(6)]
+    P.check(L.phx_trace_rays(sc.camera.ctypes.data, cfg.width, cfg.height, seed, cfg.direct_samples, dptr(ds.tv),
+                             dptr(ds.tn), dptr(ds.tt), dptr(ds.tuv), dptr(ds.nodes), dptr(ds.tmat), ds.n,
+                             dptr(ds.mats), dptr(ds.uvt), dptr(ds.tex_meta), dptr(ds.atlas), dptr(ds.lights),
+                             len(sc.lights), dptr(ds.lum), ds.total_lum, ds.center.ctypes.data, ds.radius,
+                             dptr(ds.etv), dptr(ds.etn), dptr(ds.etuv), dptr(ds.etmat), None, 0, 0, 0,
+                             *(dptr(b) for b in hp), None), "trace")
+    got = np.stack([from_dev(b, np.float32, (n_px, 4)) for b in hp], axis=1)
+    want = ref.read_hitpoints()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+    # emission in two batches with a global offset, then a truncated buffer
+    for max_photons in (cfg.photons_per_round * cfg.max_photon_bounces, 2500):
+        dpos, dpw, ddr = empty_dev(16 * max_photons), empty_dev(4 * max_photons), empty_dev(4 * max_photons)
+        counter = to_dev(np.zeros(1, np.uint32))
+        for off, cnt in ((0, 3500), (3500, 2500)):
+            wsb = L.phx_emit_photons_workspace_size(cnt, cfg.max_photon_bounces)
+            ws = empty_dev(wsb, None)
+            P.check(L.phx_emit_photons(dptr(ds.lights), len(sc.lights), max_photons, off, cnt,
+                                       cfg.max_photon_bounces, seed, dptr(ds.nodes), dptr(ds.tv), dptr(ds.tn),
+                                       dptr(ds.tt), dptr(ds.tuv), dptr(ds.tmat), ds.n, dptr(ds.etv), dptr(ds.etn),
+                                       dptr(ds.etuv), dptr(ds.etmat), dptr(ds.mats), dptr(ds.uvt),
+                                       dptr(ds.tex_meta), dptr(ds.atlas), dptr(ds.lum), ds.total_lum,
+                                       ds.center.ctypes.data, ds.radius, dptr(dpos), dptr(dpw), dptr(ddr),
+                                       dptr(counter), dptr(ws), wsb, None), "emit")
+        count = int(from_dev(counter, np.uint32, (1,))[0])
+        assert count == ref.last_photon_count
+        n = min(count, max_photons)
+        wpos, wpw, wdr = ref.read_deposits()  # canonical order, untruncated buffer
+        assert np.array_equal(from_dev(dpos, np.uint32, (n, 4)), wpos[:n].view(np.uint32))
+        assert np.array_equal(from_dev(dpw, np.uint32, (n,)), wpw[:n])
+        assert np.array_equal(from_dev(ddr, np.uint32, (n,)), wdr[:n])
+
+    # gather on the oracle's hitpoints and photon map, from the zero state
+    ppos, ppw, pdr, cs, ce = ref.read_photon_map()
+    d = ref.hash_info()
+    flux = np.zeros((n_px, 4), np.float32)
+    flux[:, 3] = np.float32(cfg.initial_radius) ** 2
+    dflux, dw, drad = to_dev(flux), to_dev(np.zeros(n_px, np.float32)), to_dev(np.zeros((n_px, 4), np.float32))
+    dhp = [to_dev(np.ascontiguousarray(want[:, k])) for k in range(6)]
+    dp, dpw2, ddr2, dcs, dce = to_dev(ppos), to_dev(ppw), to_dev(pdr), to_dev(cs), to_dev(ce)
+    P.check(L.phx_gather_flux(*(dptr(b) for b in dhp), n_px, dptr(dp), dptr(dpw2), dptr(ddr2), len(ppw), dptr(dcs),
+                              dptr(dce), d.ctypes.data, cfg.sppm_alpha, dptr(dflux), dptr(dw), dptr(drad), None),
+            "gather")
+    rf, rw, rr = ref.read_state()
+    assert np.array_equal(from_dev(dflux, np.uint32, (n_px, 4)), rf.view(np.uint32))
+    assert np.array_equal(from_dev(dw, np.uint32, (n_px,)), rw.view(np.uint32))
+    assert np.array_equal(from_dev(drad, np.uint32, (n_px, 4)), rr.view(np.uint32))
