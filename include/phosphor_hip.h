@@ -143,6 +143,36 @@ int phx_trace_rays(const PhxCameraParams *camera, uint32_t image_width, uint32_t
                    phx_float4 *hp_base_color, phx_float4 *hp_direct, phx_float4 *hp_emission,
                    void *stream);
 
+/* Device scratch needed by phx_trace_rays_wavefront. */
+size_t phx_trace_rays_wavefront_workspace_size(uint32_t image_width, uint32_t image_height, uint32_t direct_samples);
+
+/* Same arguments and the same outputs, bit for bit, as phx_trace_rays (the
+ * trace_rays.cl:14-40 list), computed in three launches: camera paths that
+ * draw every NEE light fragment and queue its shadow ray, a shadow-ray kernel
+ * (the direct_samples rays of one event are adjacent in the queue), and a
+ * resolve pass that sums the NEE terms in the reference's order.  Extra
+ * arguments: shadow_any_hit = 1 lets shadow rays stop at their first hit,
+ * which is exact only when no material has transmission >= 1e-5 and no BVH
+ * interior node is deeper than 22 (so the 24-entry stack never overflows);
+ * the caller vouches for both.  `workspace` holds
+ * phx_trace_rays_wavefront_workspace_size(width, height, direct_samples) bytes. */
+int phx_trace_rays_wavefront(const PhxCameraParams *camera, uint32_t image_width, uint32_t image_height,
+                             uint32_t seed, uint32_t direct_samples,
+                             const PhxTriangleVertices *tri_verts, const PhxTriangleNormals *tri_normals,
+                             const PhxTriangleTangents *tri_tangents, const PhxTriangleUvs *tri_uvs,
+                             const PhxBvhNode *bvh_nodes, const uint32_t *tri_mat_index, uint32_t n_triangles,
+                             const PhxMaterial *materials, const PhxUvTransformBlock *uv_transforms,
+                             const PhxTextureMeta *tex_meta, const uint8_t *tex_atlas,
+                             const PhxLight *lights, uint32_t n_lights, const float *lum_pref_sum, float total_lum,
+                             const phx_float4 *scene_center, float scene_radius,
+                             const PhxTriangleVertices *etri_verts, const PhxTriangleNormals *etri_normals,
+                             const PhxTriangleUvs *etri_uvs, const uint32_t *etri_mat_index,
+                             const float *envmap_data, uint32_t envmap_width, uint32_t envmap_height,
+                             uint32_t has_envmap, phx_float4 *hp_pos, phx_float4 *hp_normal,
+                             phx_float4 *hp_throughput, phx_float4 *hp_base_color, phx_float4 *hp_direct,
+                             phx_float4 *hp_emission, uint32_t shadow_any_hit, void *workspace,
+                             size_t workspace_bytes, void *stream);
+
 /* Device scratch needed by phx_emit_photons for one batch. */
 size_t phx_emit_photons_workspace_size(uint32_t photons_to_emit, uint32_t max_photon_bounces);
 
@@ -291,6 +321,7 @@ typedef struct PhxRenderArgs {
     uint32_t seed;                  /* --seed */
     float sppm_alpha;               /* --sppm-alpha */
     float initial_radius;           /* 0 → reference rule (cell half-diagonal, gather_flux.cl:51-56) */
+    uint32_t trace_mode;            /* 0 = wavefront trace (default), 1 = one-kernel trace; same results */
 } PhxRenderArgs;
 
 typedef struct PhxRenderer PhxRenderer;

@@ -63,11 +63,12 @@ class PassConfig:
     seed: int = 2137
     sppm_alpha: float = 0.7
     initial_radius: float = 0.0
+    trace_mode: int = 0  # 0 wavefront trace, 1 one-kernel trace (identical results)
 
     def to_c(self) -> RenderArgs:
         return RenderArgs(self.width, self.height, self.photons_per_round, self.max_photons_per_batch,
                           self.max_photon_bounces, self.grid_res, self.direct_samples, self.seed,
-                          self.sppm_alpha, self.initial_radius)
+                          self.sppm_alpha, self.initial_radius, self.trace_mode)
 
 
 class Renderer:

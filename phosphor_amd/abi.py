@@ -61,7 +61,7 @@ class RenderArgs(C.Structure):
         ("width", C.c_uint32), ("height", C.c_uint32), ("photons_per_round", C.c_uint32),
         ("max_photons_per_batch", C.c_uint32), ("max_photon_bounces", C.c_uint32), ("grid_res", C.c_uint32),
         ("direct_samples", C.c_uint32), ("seed", C.c_uint32), ("sppm_alpha", C.c_float),
-        ("initial_radius", C.c_float),
+        ("initial_radius", C.c_float), ("trace_mode", C.c_uint32),
     ]
 
 
@@ -98,6 +98,10 @@ SIGNATURES = {
     "phx_version": (C.c_char_p,),
     "phx_trace_rays": (INT, VP, U32, U32, U32, U32, VP, VP, VP, VP, VP, VP, U32, VP, VP, VP, VP, VP, U32, VP, F32,
                        VP, F32, VP, VP, VP, VP, VP, U32, U32, U32, VP, VP, VP, VP, VP, VP, VP),
+    "phx_trace_rays_wavefront_workspace_size": (SZ, U32, U32, U32),
+    "phx_trace_rays_wavefront": (INT, VP, U32, U32, U32, U32, VP, VP, VP, VP, VP, VP, U32, VP, VP, VP, VP, VP, U32,
+                                 VP, F32, VP, F32, VP, VP, VP, VP, VP, U32, U32, U32, VP, VP, VP, VP, VP, VP, U32,
+                                 VP, SZ, VP),
     "phx_emit_photons_workspace_size": (SZ, U32, U32),
     "phx_emit_photons": (INT, VP, U32, U32, U32, U32, U32, U32, VP, VP, VP, VP, VP, VP, U32, VP, VP, VP, VP, VP, VP,
                          VP, VP, VP, F32, VP, F32, VP, VP, VP, VP, VP, SZ, VP),

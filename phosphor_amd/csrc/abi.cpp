@@ -124,6 +124,54 @@ int phx_trace_rays(const PhxCameraParams *camera, uint32_t image_width, uint32_t
                       "phx_trace_rays");
 }
 
+size_t phx_trace_rays_wavefront_workspace_size(uint32_t image_width, uint32_t image_height, uint32_t direct_samples) {
+    return trace_wavefront_workspace_bytes(image_width * image_height, direct_samples);
+}
+
+int phx_trace_rays_wavefront(const PhxCameraParams *camera, uint32_t image_width, uint32_t image_height,
+                             uint32_t seed, uint32_t direct_samples, const PhxTriangleVertices *tri_verts,
+                             const PhxTriangleNormals *tri_normals, const PhxTriangleTangents *tri_tangents,
+                             const PhxTriangleUvs *tri_uvs, const PhxBvhNode *bvh_nodes,
+                             const uint32_t *tri_mat_index, uint32_t n_triangles, const PhxMaterial *materials,
+                             const PhxUvTransformBlock *uv_transforms, const PhxTextureMeta *tex_meta,
+                             const uint8_t *tex_atlas, const PhxLight *lights, uint32_t n_lights,
+                             const float *lum_pref_sum, float total_lum, const phx_float4 *scene_center,
+                             float scene_radius, const PhxTriangleVertices *etri_verts,
+                             const PhxTriangleNormals *etri_normals, const PhxTriangleUvs *etri_uvs,
+                             const uint32_t *etri_mat_index, const float *envmap_data, uint32_t envmap_width,
+                             uint32_t envmap_height, uint32_t has_envmap, phx_float4 *hp_pos, phx_float4 *hp_normal,
+                             phx_float4 *hp_throughput, phx_float4 *hp_base_color, phx_float4 *hp_direct,
+                             phx_float4 *hp_emission, uint32_t shadow_any_hit, void *workspace,
+                             size_t workspace_bytes, void *stream) {
+    if (!camera || !tri_verts || !tri_normals || !tri_uvs || !bvh_nodes || !tri_mat_index || !materials ||
+        !hp_pos || !hp_normal || !hp_throughput || !hp_base_color || !hp_direct || !hp_emission)
+        return fail(PHX_ERR_INVALID_ARG, "phx_trace_rays_wavefront: null pointer argument");
+    if (n_triangles == 0)
+        return fail(PHX_ERR_INVALID_ARG, "phx_trace_rays_wavefront: empty scene");
+    if (n_lights > 0 && (!lights || !lum_pref_sum))
+        return fail(PHX_ERR_INVALID_ARG, "phx_trace_rays_wavefront: lights without a luminance prefix sum");
+    if (has_envmap && (!envmap_data || envmap_width == 0 || envmap_height == 0))
+        return fail(PHX_ERR_INVALID_ARG, "phx_trace_rays_wavefront: has_envmap without envmap data");
+    if (direct_samples == 0 || direct_samples > 128)
+        return fail(PHX_ERR_INVALID_ARG, "phx_trace_rays_wavefront: direct_samples outside 1..128");
+    uint64_t slots = (uint64_t)image_width * image_height * 5u * direct_samples;
+    if (slots > 0xFFFFFFFFull)
+        return fail(PHX_ERR_TOO_LARGE, "phx_trace_rays_wavefront: shadow queue exceeds 2^32 slots");
+    if (!workspace || workspace_bytes < trace_wavefront_workspace_bytes(image_width * image_height, direct_samples))
+        return fail(PHX_ERR_WORKSPACE, "phx_trace_rays_wavefront: workspace smaller than the size query");
+    SceneArgs sc = scene_args(tri_verts, tri_normals, tri_tangents, tri_uvs, bvh_nodes, tri_mat_index, n_triangles,
+                              materials, uv_transforms, tex_meta, tex_atlas, lights, n_lights, lum_pref_sum, total_lum,
+                              scene_center, scene_radius, etri_verts, etri_normals, etri_uvs, etri_mat_index);
+    sc.envmap = envmap_data;
+    sc.env_w = envmap_width;
+    sc.env_h = envmap_height;
+    sc.has_env = has_envmap ? 1u : 0u;
+    return hip_status(launch_trace_wavefront(*camera, image_width, image_height, seed, direct_samples, sc,
+                                             shadow_any_hit, hp_pos, hp_normal, hp_throughput, hp_base_color,
+                                             hp_direct, hp_emission, workspace, (hipStream_t)stream),
+                      "phx_trace_rays_wavefront");
+}
+
 size_t phx_emit_photons_workspace_size(uint32_t photons_to_emit, uint32_t max_photon_bounces) {
     return emit_workspace_bytes(photons_to_emit, max_photon_bounces);
 }

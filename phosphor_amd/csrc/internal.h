@@ -56,6 +56,13 @@ hipError_t launch_trace(const PhxCameraParams &cam, uint32_t w, uint32_t h, uint
                         const SceneArgs &sc, phx_float4 *hp_pos, phx_float4 *hp_normal, phx_float4 *hp_throughput,
                         phx_float4 *hp_base_color, phx_float4 *hp_direct, phx_float4 *hp_emission, hipStream_t s);
 
+size_t trace_wavefront_workspace_bytes(uint32_t n_pixels, uint32_t direct_samples);
+hipError_t launch_trace_wavefront(const PhxCameraParams &cam, uint32_t w, uint32_t h, uint32_t seed,
+                                  uint32_t direct_samples, const SceneArgs &sc, uint32_t shadow_any_hit,
+                                  phx_float4 *hp_pos, phx_float4 *hp_normal, phx_float4 *hp_throughput,
+                                  phx_float4 *hp_base_color, phx_float4 *hp_direct, phx_float4 *hp_emission,
+                                  void *ws, hipStream_t s);
+
 size_t emit_workspace_bytes(uint32_t photons_to_emit, uint32_t max_bounces);
 hipError_t launch_emit(const SceneArgs &sc, uint32_t max_photons, uint32_t offset, uint32_t photons_to_emit,
                        uint32_t max_bounces, uint32_t seed, phx_float4 *photon_pos, uint32_t *photon_power,

@@ -264,6 +264,406 @@ hipError_t launch_trace(const PhxCameraParams &cam, u32 w, u32 h, u32 seed, u32 
     return hipGetLastError();
 }
 
+/* ----------------------------------------------------------------------
+ * Wavefront form of trace_rays (same outputs, bit for bit).
+ *
+ * The random draws of the NEE loops (light selection, emissive point) never
+ * depend on shadow visibility, and the Russian roulette after a specular NEE
+ * only reads throughput, so a camera-path kernel can draw all light fragments
+ * of an NEE event and keep walking the path.  Shadow rays go to a queue (the
+ * direct_samples rays of one event are contiguous: same origin, coherent
+ * directions) traced by a lean kernel; a resolve kernel then forms
+ * emission and direct exactly in the reference's order of additions:
+ *   emission = ((BLACK + a_0) + s_0) + a_1 + ... (+ envmap on a miss),
+ * a_k = throughput_k·emissive_k, s_k = throughput_k·(Σ_d pre_d·trans_d / n)
+ * for specular events, and direct = (throughput·occlusion)·(Σ / n) at the
+ * first diffuse event (trace_rays.cl:97-147).
+ * ---------------------------------------------------------------------- */
+constexpr int WF_CAM_BLOCK = 128;
+constexpr int WF_SHADOW_BLOCK = 256;
+constexpr u32 NEE_NONE = 0, NEE_DIFFUSE = 1, NEE_SPECULAR = 2;
+
+struct WfState {
+    phx_float4 *rec_a, *rec_thr; /* [pixel · 5 + bounce] */
+    u32 *rec_base;               /* first shadow slot of the NEE event of that bounce */
+    u32 *meta;                   /* per pixel: bits 0-2 records, 3-12 NEE kinds, 15 envmap term */
+    phx_float4 *env;             /* per pixel: throughput · envmap on a miss */
+    f32 *occl;                   /* per pixel: occlusion at the diffuse hit */
+    u32 *counter;                /* shadow slots handed out */
+    phx_float4 *sh_o;            /* origin.xyz, t_max in w */
+    phx_float4 *sh_d;            /* direction */
+    phx_float4 *sh_pre;          /* NEE weight; w = 0 valid, w = -1 no shadow ray (adds BLACK) */
+    phx_float4 *sh_tr;           /* shadow transmittance */
+    u32 cap;
+};
+
+struct WfCamArgs {
+    PhxCameraParams cam;
+    u32 w, h, seed, n;
+    SceneView sc;
+    WfState st;
+    phx_float4 *hp_pos, *hp_normal, *hp_throughput, *hp_base_color;
+};
+
+__global__ __launch_bounds__(WF_CAM_BLOCK) void k_trace_camera(WfCamArgs a) {
+    __shared__ u32 s_stack[BVH_STACK_SIZE * WF_CAM_BLOCK];
+    __shared__ f32 s_ior[IOR_STACK_SIZE * WF_CAM_BLOCK];
+    const SceneView &sc = a.sc;
+    const WfState &S = a.st;
+    u32 n_pixels = a.w * a.h;
+    u32 tid = blockIdx.x * WF_CAM_BLOCK + threadIdx.x;
+    bool live = tid < n_pixels;
+    u32 *stk = s_stack + threadIdx.x;
+    u32 px = live ? tid % a.w : 0, py = live ? tid / a.w : 0;
+    u32 rng = pcg_seed(a.seed + tid);
+    F4 origin = ZERO, dir = ZERO;
+    if (live)
+        make_ray(a.cam, px, py, a.w, a.h, rng, origin, dir);
+    F4 throughput = WHITE;
+    IorStack ior{s_ior + threadIdx.x, (u32)WF_CAM_BLOCK, 0u};
+    init_ior_stack(ior, AIR_IOR);
+    u32 depth = 0, kinds = 0, env_flag = 0;
+    bool diffuse = false;
+    size_t rbase = (size_t)tid * MAX_RAY_BOUNCES;
+    /* the loop runs while any lane of the wave is live, so the wave-wide slot
+       allocation below sees every lane */
+    bool active = live;
+    while (__any(active)) {
+        bool nee = false, spec = false;
+        F4 hit_pos = ZERO, sn = ZERO, bc = ZERO, f0 = ZERO, view = ZERO;
+        f32 rough = 0.0f;
+        Bsdf bs;
+        bs.dir = ZERO;
+        bs.throughput = ZERO;
+        bs.event = BSDF_TRANSMIT;
+        F4 hit_n = ZERO;
+        if (active) {
+            HitRecord rec;
+            bool hit = scene_intersect(sc, origin, dir, EPS, INF, rec, stk, WF_CAM_BLOCK);
+            if (!hit) {
+                if (sc.has_env) {
+                    st(S.env + tid, throughput * sample_envmap(sc, dir));
+                    env_flag = 1;
+                }
+                active = false;
+            } else {
+                SurfaceHit sh = process_hit(sc, rec, origin, dir);
+                const PhxMaterial &mat = sc.materials[sh.mat_index];
+                f32 curr_ior = current_ior(ior);
+                if (curr_ior != AIR_IOR && mat.thickness > EPS && mat.att_dist > EPS)
+                    throughput = throughput * beer_lambert(from(mat.att_color), mat.att_dist, sh.t);
+                Ctx c = evaluate_material(sc, mat, sh.mat_index, rec, sh);
+                st(S.rec_a + rbase + depth, throughput * c.emissive);
+                view = -dir;
+                bs = sample_bsdf(rng, c, c.shading_normal, sh.normal, view, ior, sh.front_face);
+                hit_pos = sh.position;
+                hit_n = sh.normal;
+                sn = c.shading_normal;
+                bc = c.base_color;
+                rough = c.roughness;
+                if (bs.event == BSDF_DIFFUSE) {
+                    nee = true;
+                    diffuse = true;
+                    kinds |= NEE_DIFFUSE << (2 * depth);
+                    st(a.hp_pos + tid, sh.position);
+                    st(a.hp_normal + tid, sh.normal);
+                    st(a.hp_base_color + tid, c.base_color);
+                    st(a.hp_throughput + tid, throughput);
+                    S.occl[tid] = c.occlusion;
+                } else if (bs.event == BSDF_METALLIC || bs.event == BSDF_FRESNEL) {
+                    nee = true;
+                    spec = true;
+                    kinds |= NEE_SPECULAR << (2 * depth);
+                    if (bs.event == BSDF_METALLIC) {
+                        f0 = c.base_color;
+                    } else {
+                        f32 ratio = (curr_ior - c.ior) / (curr_ior + c.ior);
+                        f32 r0 = ratio * ratio;
+                        f0 = mk(r0, r0, r0, 0.0f);
+                    }
+                }
+                if (nee)
+                    st(S.rec_thr + rbase + depth, throughput);
+            }
+        }
+        /* one atomic per wave for the shadow slots of this bounce */
+        unsigned long long want = __ballot(nee);
+        u32 lane = threadIdx.x & 63u;
+        u32 rank = (u32)__popcll(want & ((1ull << lane) - 1ull));
+        u32 wave_base = 0;
+        if (want) {
+            u32 total = (u32)__popcll(want) * a.n;
+            u32 first = (u32)__ffsll((long long)want) - 1u;
+            if (lane == first)
+                wave_base = atomicAdd(S.counter, total);
+            wave_base = __shfl(wave_base, (int)first, 64);
+        }
+        if (nee) {
+            u32 base = wave_base + rank * a.n;
+            S.rec_base[rbase + depth] = base;
+            for (u32 d = 0; d < a.n; d++) {
+                LightFragment f;
+                bool valid = false;
+                F4 pre = BLACK;
+                if (!spec) {
+                    if (sample_light_fragment(rng, sc, hit_pos, sn, f)) {
+                        valid = true;
+                        pre = f.radiance * (bc / PI);
+                    }
+                } else {
+                    f32 ndv = dot(view, sn);
+                    /* direct_lighting_specular: each early return adds BLACK */
+                    if (!(ndv < EPS) && sample_light_fragment(rng, sc, hit_pos, sn, f) && !(f.n_dot_dir < EPS)) {
+                        f32 alpha = rough * rough;
+                        F4 hv = normalize(f.dir + view);
+                        f32 ndh = phx_fmaxf(dot(sn, hv), 0.0f);
+                        f32 ggx = ggx_ndf(ndh, alpha);
+                        F4 fr = fresnel4(f0, view, hv);
+                        f32 sm = smith_g1_ggx(f.n_dot_dir, alpha) * smith_g1_ggx(ndv, alpha);
+                        F4 spv = f.radiance * ggx * fr * sm / (4.0f * f.n_dot_dir * ndv);
+                        if (!(dot(spv, spv) < EPS * EPS)) {
+                            valid = true;
+                            pre = spv;
+                        }
+                    }
+                }
+                size_t slot = (size_t)base + d;
+                if (valid) {
+                    st(S.sh_o + slot, mk(hit_pos.x, hit_pos.y, hit_pos.z, f.dist - EPS));
+                    st(S.sh_d + slot, f.dir);
+                    st(S.sh_pre + slot, mk(pre.x, pre.y, pre.z, 0.0f));
+                } else {
+                    st(S.sh_pre + slot, mk(0.0f, 0.0f, 0.0f, -1.0f));
+                }
+            }
+        }
+        if (diffuse)
+            active = false;
+        if (active) {
+            /* trace_rays.cl:149-160 */
+            throughput = throughput * bs.throughput;
+            bool killed = false;
+            if (depth >= RR_MIN_RAY_DEPTH) {
+                f32 q = phx_fmaxf(throughput.x, phx_fmaxf(throughput.y, throughput.z));
+                q = phx_clampf(q, RR_MIN_Q, RR_MAX_Q);
+                if (random_float(rng) > q)
+                    killed = true;
+                else
+                    throughput = throughput * (1.0f / q);
+            }
+            if (!killed) {
+                F4 side = (dot(bs.dir, hit_n) > 0.0f) ? hit_n : -hit_n;
+                origin = hit_pos + bs.dir * EPS + side * EPS;
+                dir = bs.dir;
+            }
+            depth++;
+            if (killed || depth >= MAX_RAY_BOUNCES)
+                active = false;
+        }
+    }
+    if (!live)
+        return;
+    if (!diffuse) {
+        st(a.hp_pos + tid, ZERO);
+        st(a.hp_normal + tid, ZERO);
+        st(a.hp_base_color + tid, BLACK);
+        st(a.hp_throughput + tid, throughput);
+    }
+    /* records exist for every bounce that hit: 0..depth at a diffuse end,
+       0..depth-1 after a miss (nothing recorded for the missing bounce), a
+       roulette kill or the bounce limit (depth already advanced) */
+    u32 nrec = diffuse ? depth + 1 : depth;
+    S.meta[tid] = nrec | (kinds << 3) | (env_flag << 15);
+}
+
+struct WfShadowArgs {
+    SceneView sc;
+    WfState st;
+    u32 any_hit;
+};
+
+template <bool ANY>
+__device__ __forceinline__ void shadow_one(const WfShadowArgs &a, u32 slot, u32 *stk) {
+    const SceneView &sc = a.sc;
+    const WfState &S = a.st;
+    F4 pre = ld(S.sh_pre + slot);
+    if (pre.w != 0.0f)
+        return;
+    F4 ot = ld(S.sh_o + slot);
+    F4 o = mk(ot.x, ot.y, ot.z, 0.0f);
+    F4 d = ld(S.sh_d + slot);
+    f32 t_min = EPS, t_max = ot.w;
+    F4 trans = WHITE;
+    /* light_sampling.h:302-333 */
+    for (u32 depth = 0; depth < MAX_RAY_BOUNCES; depth++) {
+        HitRecord rec;
+        if (!traverse<ANY>(sc, o, d, t_min, t_max, rec, stk, WF_SHADOW_BLOCK))
+            break;
+        const PhxMaterial &mat = sc.materials[rec.mat_index];
+        if (ANY || mat.transmission < EPS) {
+            trans = BLACK;
+            break;
+        }
+        F4 bcol = from(mat.base_color);
+        if (mat.diff_index != NO_TEXTURE) {
+            SurfaceHit s2 = process_hit(sc, rec, o, d);
+            bcol = bcol * srgb_to_linear(sample_texture_uv(sc, rec.mat_index, mat.diff_index, s2.uvx, s2.uvy, 0));
+        }
+        if (mat.att_dist > EPS)
+            trans = trans * beer_lambert(from(mat.att_color), mat.att_dist, rec.t - t_min);
+        trans = trans * (bcol * mat.transmission);
+        if (dot(trans, trans) < EPS) {
+            trans = BLACK;
+            break;
+        }
+        t_min = rec.t + EPS;
+    }
+    st(S.sh_tr + slot, trans);
+}
+
+__global__ __launch_bounds__(WF_SHADOW_BLOCK) void k_trace_shadow(WfShadowArgs a) {
+    __shared__ u32 s_stack[BVH_STACK_SIZE * WF_SHADOW_BLOCK];
+    u32 *stk = s_stack + threadIdx.x;
+    u32 count = *a.st.counter;
+    for (u32 slot = blockIdx.x * WF_SHADOW_BLOCK + threadIdx.x; slot < count; slot += gridDim.x * WF_SHADOW_BLOCK) {
+        if (a.any_hit)
+            shadow_one<true>(a, slot, stk);
+        else
+            shadow_one<false>(a, slot, stk);
+    }
+}
+
+struct WfResolveArgs {
+    WfState st;
+    u32 n_pixels, n;
+    phx_float4 *hp_direct, *hp_emission;
+};
+
+__global__ __launch_bounds__(256) void k_trace_resolve(WfResolveArgs a) {
+    u32 tid = blockIdx.x * 256 + threadIdx.x;
+    if (tid >= a.n_pixels)
+        return;
+    const WfState &S = a.st;
+    u32 meta = S.meta[tid];
+    u32 nrec = meta & 7u;
+    size_t rbase = (size_t)tid * MAX_RAY_BOUNCES;
+    F4 e = BLACK;
+    F4 direct = BLACK;
+    for (u32 k = 0; k < nrec; k++) {
+        e = e + ld(S.rec_a + rbase + k);
+        u32 kind = (meta >> (3 + 2 * k)) & 3u;
+        if (kind == NEE_NONE)
+            continue;
+        size_t base = S.rec_base[rbase + k];
+        F4 sum = BLACK;
+        for (u32 d = 0; d < a.n; d++) {
+            F4 pre = ld(S.sh_pre + base + d);
+            if (pre.w != 0.0f) {
+                sum = sum + BLACK;
+            } else {
+                sum = sum + pre * ld(S.sh_tr + base + d);
+            }
+        }
+        F4 thr = ld(S.rec_thr + rbase + k);
+        if (kind == NEE_SPECULAR) {
+            e = e + thr * (sum / (f32)a.n);
+        } else {
+            e = mk(e.x, e.y, e.z, 1.0f);
+            direct = thr * S.occl[tid] * (sum / (f32)a.n);
+        }
+    }
+    if ((meta >> 15) & 1u)
+        e = e + ld(S.env + tid);
+    st(a.hp_emission + tid, e);
+    st(a.hp_direct + tid, direct);
+}
+
+static WfState wf_layout(void *ws, u32 n_pixels, u32 n) {
+    size_t rec = (size_t)n_pixels * MAX_RAY_BOUNCES;
+    size_t cap = rec * n;
+    char *p = (char *)ws;
+    WfState S;
+    S.counter = (u32 *)p;
+    p += 256;
+    S.rec_a = (phx_float4 *)p;
+    p += align_up(rec * sizeof(phx_float4), 256);
+    S.rec_thr = (phx_float4 *)p;
+    p += align_up(rec * sizeof(phx_float4), 256);
+    S.rec_base = (u32 *)p;
+    p += align_up(rec * sizeof(u32), 256);
+    S.meta = (u32 *)p;
+    p += align_up((size_t)n_pixels * sizeof(u32), 256);
+    S.env = (phx_float4 *)p;
+    p += align_up((size_t)n_pixels * sizeof(phx_float4), 256);
+    S.occl = (f32 *)p;
+    p += align_up((size_t)n_pixels * sizeof(f32), 256);
+    S.sh_o = (phx_float4 *)p;
+    p += align_up(cap * sizeof(phx_float4), 256);
+    S.sh_d = (phx_float4 *)p;
+    p += align_up(cap * sizeof(phx_float4), 256);
+    S.sh_pre = (phx_float4 *)p;
+    p += align_up(cap * sizeof(phx_float4), 256);
+    S.sh_tr = (phx_float4 *)p;
+    p += align_up(cap * sizeof(phx_float4), 256);
+    S.cap = (u32)(cap > 0xFFFFFFFFull ? 0xFFFFFFFFull : cap);
+    return S;
+}
+
+size_t trace_wavefront_workspace_bytes(u32 n_pixels, u32 n) {
+    size_t rec = (size_t)n_pixels * MAX_RAY_BOUNCES;
+    size_t cap = rec * n;
+    return 256 + 2 * align_up(rec * sizeof(phx_float4), 256) + align_up(rec * sizeof(u32), 256) +
+           align_up((size_t)n_pixels * sizeof(u32), 256) + align_up((size_t)n_pixels * sizeof(phx_float4), 256) +
+           align_up((size_t)n_pixels * sizeof(f32), 256) + 4 * align_up(cap * sizeof(phx_float4), 256);
+}
+
+hipError_t launch_trace_wavefront(const PhxCameraParams &cam, u32 w, u32 h, u32 seed, u32 direct_samples,
+                                  const SceneArgs &sc, u32 shadow_any_hit, phx_float4 *hp_pos,
+                                  phx_float4 *hp_normal, phx_float4 *hp_throughput, phx_float4 *hp_base_color,
+                                  phx_float4 *hp_direct, phx_float4 *hp_emission, void *ws, hipStream_t s) {
+    u32 n = w * h;
+    if (n == 0)
+        return hipSuccess;
+    WfState S = wf_layout(ws, n, direct_samples);
+    hipError_t e = launch_fill_u32(S.counter, 0u, 1, s);
+    if (e != hipSuccess)
+        return e;
+    WfCamArgs ca;
+    ca.cam = cam;
+    ca.w = w;
+    ca.h = h;
+    ca.seed = seed;
+    ca.n = direct_samples;
+    ca.sc = sc;
+    ca.st = S;
+    ca.hp_pos = hp_pos;
+    ca.hp_normal = hp_normal;
+    ca.hp_throughput = hp_throughput;
+    ca.hp_base_color = hp_base_color;
+    hipLaunchKernelGGL(k_trace_camera, dim3((n + WF_CAM_BLOCK - 1) / WF_CAM_BLOCK), dim3(WF_CAM_BLOCK), 0, s, ca);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    WfShadowArgs sa;
+    sa.sc = sc;
+    sa.st = S;
+    sa.any_hit = shadow_any_hit;
+    size_t expect = (size_t)n * direct_samples;
+    u32 blocks = (u32)((expect + WF_SHADOW_BLOCK - 1) / WF_SHADOW_BLOCK);
+    if (blocks > 16384)
+        blocks = 16384;
+    hipLaunchKernelGGL(k_trace_shadow, dim3(blocks), dim3(WF_SHADOW_BLOCK), 0, s, sa);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    WfResolveArgs ra;
+    ra.st = S;
+    ra.n_pixels = n;
+    ra.n = direct_samples;
+    ra.hp_direct = hp_direct;
+    ra.hp_emission = hp_emission;
+    hipLaunchKernelGGL(k_trace_resolve, dim3((n + 255) / 256), dim3(256), 0, s, ra);
+    return hipGetLastError();
+}
+
 /* ---------------------------------------------------------------------- */
 
 struct EmitArgs {

@@ -106,17 +106,19 @@ PHX_DEV u32 random_u32(u32 &state) {
 }
 PHX_DEV f32 random_float(u32 &state) { return (f32)random_u32(state) / U32_MAX_F; }
 
-/* utils.h:59-71 */
-PHX_DEV void make_tbn(F4 n, F4 &t, F4 &b) {
+/* utils.h:59-71 (returned by value so the frame stays in registers) */
+struct TBN {
+    F4 t, b;
+};
+PHX_DEV TBN make_tbn(F4 n) {
     n.w = 0.0f;
-    F4 tmp = (phx_fabsf(n.x) > 0.1f) ? mk(0.0f, 1.0f, 0.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f, 0.0f);
-    t = cross(n, tmp);
-    if (length(t) < EPS) {
-        tmp = mk(0.0f, 0.0f, 1.0f, 0.0f);
-        t = cross(n, tmp);
-    }
+    bool big_x = phx_fabsf(n.x) > 0.1f;
+    /* cross(n, e_y) or cross(n, e_x), written out (same products and order) */
+    F4 t = big_x ? cross(n, mk(0.0f, 1.0f, 0.0f, 0.0f)) : cross(n, mk(1.0f, 0.0f, 0.0f, 0.0f));
+    if (length(t) < EPS)
+        t = cross(n, mk(0.0f, 0.0f, 1.0f, 0.0f));
     t = normalize(t);
-    b = cross(n, t);
+    return TBN{t, cross(n, t)};
 }
 /* random.h:40-46 */
 PHX_DEV F4 random_unit_vector(u32 &rng) {
@@ -138,9 +140,8 @@ PHX_DEV F4 random_in_unit_hemisphere(u32 &rng, F4 n) {
     f32 s, c;
     phx_sincosf(phi, &s, &c);
     F4 hl = mk(st_ * c, st_ * s, ct, 0.0f);
-    F4 t, b;
-    make_tbn(n, t, b);
-    return normalize(hl.x * t + hl.y * b + hl.z * n);
+    TBN f = make_tbn(n);
+    return normalize(hl.x * f.t + hl.y * f.b + hl.z * n);
 }
 
 /* camera.h:28-47 */
@@ -279,49 +280,62 @@ PHX_DEV bool slab(f32 mn, f32 mx, f32 o, f32 inv, f32 &t_min, f32 &t_max) {
     return t_max > t_min;
 }
 
-/* hit.h:135-172.  Depth-first, left child first, no near/far ordering; a push
- * that would overflow the 24-entry stack drops both children, as the
- * reference does.  `stk` points at this lane's column of an LDS array laid
- * out [depth][lanes]; `stride` is the number of lanes. */
-PHX_DEV bool scene_intersect(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
-                             u32 stride) {
+/* hit.h:135-172.  Depth-first, left child first, no near/far ordering.
+ * The reference pushes the right child, pushes the left child and pops the
+ * left child at once; here the left child stays in a register and only the
+ * right child is pushed, which visits nodes in the same order.  With q the
+ * index of the top entry of this stack, the reference's stack pointer after
+ * a pop equals q, so its overflow rule (drop both children when p + 2 >= 24)
+ * is applied as q + 2 >= 24.  `stk` points at this lane's column of an LDS
+ * array laid out [depth][lanes]; `stride` is the number of lanes.
+ * ANY: stop at the first accepted triangle (used only where any hit decides
+ * the result, see the shadow kernel). */
+template <bool ANY>
+PHX_DEV bool traverse(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
+                      u32 stride) {
     f32 ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
-    int p = 0;
-    stk[0] = 0;
+    int q = -1;
+    u32 index = 0;
     bool found = false;
     f32 closest = t_max;
-    while (p >= 0) {
-        u32 index = stk[p * stride];
-        p--;
+    for (;;) {
         const PhxBvhNode *node = sc.nodes + index;
         F4 nmin = ld(&node->min);
         F4 nmax = ld(&node->max);
         f32 lo = t_min, hi = closest;
-        if (!slab(nmin.x, nmax.x, o.x, ix, lo, hi))
-            continue;
-        if (!slab(nmin.y, nmax.y, o.y, iy, lo, hi))
-            continue;
-        if (!slab(nmin.z, nmax.z, o.z, iz, lo, hi))
-            continue;
-        u32 tri = phx_as_uint(nmin.w);
-        if (tri != 0xFFFFFFFFu) {
-            HitRecord cand;
-            if (is_triangle_hit(sc, tri, o, d, t_min, closest, cand)) {
-                closest = cand.t;
-                rec = cand;
-                found = true;
+        bool box = slab(nmin.x, nmax.x, o.x, ix, lo, hi) && slab(nmin.y, nmax.y, o.y, iy, lo, hi) &&
+                   slab(nmin.z, nmax.z, o.z, iz, lo, hi);
+        if (box) {
+            u32 tri = phx_as_uint(nmin.w);
+            if (tri != 0xFFFFFFFFu) {
+                HitRecord cand;
+                if (is_triangle_hit(sc, tri, o, d, t_min, closest, cand)) {
+                    closest = cand.t;
+                    rec = cand;
+                    found = true;
+                    if (ANY)
+                        break;
+                }
+            } else if (q + 2 < BVH_STACK_SIZE) {
+                stk[(q + 1) * stride] = phx_as_uint(nmax.w);
+                q++;
+                index = index + 1;
+                continue;
             }
-            continue;
         }
-        if (p + 2 >= BVH_STACK_SIZE)
-            continue;
-        stk[(p + 1) * stride] = phx_as_uint(nmax.w);
-        stk[(p + 2) * stride] = index + 1;
-        p += 2;
+        if (q < 0)
+            break;
+        index = stk[q * stride];
+        q--;
     }
     if (found)
         rec.mat_index = sc.tmat[rec.tri_index];
     return found;
+}
+
+PHX_DEV bool scene_intersect(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
+                             u32 stride) {
+    return traverse<false>(sc, o, d, t_min, t_max, rec, stk, stride);
 }
 
 struct SurfaceHit {
@@ -464,17 +478,16 @@ PHX_DEV F4 apply_normal_map(F4 ms, F4 gn, F4 t, F4 b) {
 /* trace_rays.cl:83-93 / emit_photons.cl:66-76: tangent frame at the hit.  The
  * frame only feeds the normal map; it is computed only when one is bound
  * (the result of evaluate_material is the same either way). */
-PHX_DEV void hit_tbn(const SceneView &sc, const HitRecord &rec, const SurfaceHit &sh, F4 &t, F4 &b) {
+PHX_DEV TBN hit_tbn(const SceneView &sc, const HitRecord &rec, const SurfaceHit &sh) {
     f32 bw = 1.0f - rec.u - rec.v;
     const PhxTriangleTangents *tt = sc.tt + sh.tri_index;
     F4 tr = bw * ld(&tt->t0) + rec.u * ld(&tt->t1) + rec.v * ld(&tt->t2);
     F4 br = (length(tr) > EPS) ? cross(sh.normal, tr) : ZERO;
     if (length(br) > EPS) {
-        b = normalize(br);
-        t = cross(b, sh.normal);
-    } else {
-        make_tbn(sh.normal, t, b);
+        F4 b = normalize(br);
+        return TBN{cross(b, sh.normal), b};
     }
+    return make_tbn(sh.normal);
 }
 
 /* shading.h:74-123 */
@@ -505,9 +518,9 @@ PHX_DEV Ctx evaluate_material(const SceneView &sc, const PhxMaterial &mat, u32 m
     c.ior = mat.ior;
     c.shading_normal = sh.normal;
     if (mat.norm_index != NO_TEXTURE) {
-        F4 t, b;
-        hit_tbn(sc, rec, sh, t, b);
-        c.shading_normal = apply_normal_map(sample_texture_uv(sc, mi, mat.norm_index, ux, uy, 2), sh.normal, t, b);
+        TBN f = hit_tbn(sc, rec, sh);
+        c.shading_normal =
+            apply_normal_map(sample_texture_uv(sc, mi, mat.norm_index, ux, uy, 2), sh.normal, f.t, f.b);
     }
     return c;
 }
@@ -535,8 +548,8 @@ PHX_DEV F4 safe_reflect(F4 d, F4 n, F4 g) {
 /* bsdfs.h:27-72 (Heitz 2018 VNDF sampling with ≤ 8 rejection attempts) */
 PHX_DEV F4 ggx_sample_vndf(u32 &rng, F4 normal, F4 geom_normal, F4 view, f32 roughness) {
     f32 alpha = roughness * roughness;
-    F4 t, b;
-    make_tbn(normal, t, b);
+    TBN f = make_tbn(normal);
+    F4 t = f.t, b = f.b;
     F4 vl = normalize(mk(dot(view, t), dot(view, b), dot(view, normal), 0.0f));
     F4 vh = normalize(mk(alpha * vl.x, alpha * vl.y, vl.z, 0.0f));
     f32 len_sq = vh.x * vh.x + vh.y * vh.y;
@@ -776,10 +789,9 @@ PHX_DEV void sample_light(u32 &rng, const SceneView &sc, F4 &origin, F4 &dir, F4
         f32 sn, cs;
         phx_sincosf(phi, &sn, &cs);
         F4 ldl = mk(st_ * cs, st_ * sn, ct, 0.0f);
-        F4 t, b;
         F4 ldir = normalize(from(l.direction));
-        make_tbn(ldir, t, b);
-        dir = normalize(ldl.x * t + ldl.y * b + ldl.z * ldir);
+        TBN f = make_tbn(ldir);
+        dir = normalize(ldl.x * f.t + ldl.y * f.b + ldl.z * ldir);
         f32 fo = spotlight_falloff(ct, ci, co);
         f32 sa = 2.0f * PI * (1.0f - co);
         power = from(l.power) * fo * sa;

@@ -146,7 +146,8 @@ struct PhxRenderer {
     uint32_t photon_buf_size = 0;
     /* Renderer's own unsorted photon buffers (renderer.hpp:48-51) */
     DevBuf d_photon_pos, d_photon_power, d_photon_dir, d_batch_size;
-    DevBuf d_emit_ws, d_scan_ws, d_sort_ws;
+    DevBuf d_emit_ws, d_scan_ws, d_sort_ws, d_trace_ws;
+    uint32_t shadow_any_hit = 0;
     uint32_t rng = 0;
     uint32_t rounds = 0;
     uint64_t total_photons_emitted = 0;
@@ -209,6 +210,26 @@ int PhxRenderer::set_scene(const PhxSceneDesc &sc) {
             return fail(PHX_ERR_INVALID_ARG, "set_scene: BVH build failed");
     }
     const PhxBvhNode &root = nodes[0];
+    /* any-hit shadow rays are exact when every material is opaque to shadow
+       rays (light_sampling.h:316) and the traversal stack never overflows
+       (no interior node deeper than 22, hit.h:163) */
+    {
+        bool transmissive = false;
+        for (uint32_t i = 0; i < sc.n_materials; i++)
+            transmissive = transmissive || !(sc.materials[i].transmission < 1e-5f);
+        uint32_t max_depth = 0;
+        std::vector<std::pair<uint32_t, uint32_t>> todo{{0u, 0u}};
+        while (!todo.empty()) {
+            auto [i, dpt] = todo.back();
+            todo.pop_back();
+            if (phx_as_uint(nodes[i].min.w) != 0xFFFFFFFFu)
+                continue;
+            max_depth = std::max(max_depth, dpt);
+            todo.push_back({i + 1, dpt + 1});
+            todo.push_back({phx_as_uint(nodes[i].max.w), dpt + 1});
+        }
+        shadow_any_hit = (!transmissive && max_depth <= 22) ? 1u : 0u;
+    }
 
     GeometryBuffers &g = buffers.geometry_buf;
     g.n_triangles = n;
@@ -324,6 +345,9 @@ int PhxRenderer::set_scene(const PhxSceneDesc &sc) {
     PHX_TRY(d_emit_ws.alloc(emit_workspace_bytes(per_batch, a.max_photon_bounces)), "alloc emit workspace");
     PHX_TRY(d_scan_ws.alloc(scan_workspace_bytes(bucket_count)), "alloc scan workspace");
     PHX_TRY(d_sort_ws.alloc(sort_workspace_bytes(photon_buf_size, bucket_count)), "alloc sort workspace");
+    if (a.trace_mode == 0)
+        PHX_TRY(d_trace_ws.alloc(trace_wavefront_workspace_bytes(im.n_pixels, a.direct_samples)),
+                "alloc trace workspace");
 
     rng = host_pcg_seed(a.seed);
     PHX_TRY(hipStreamSynchronize(s), "set_scene");
@@ -333,6 +357,13 @@ int PhxRenderer::set_scene(const PhxSceneDesc &sc) {
 /* renderer.cpp:90-95 */
 int PhxRenderer::trace_rays(uint32_t seed) {
     const ImageBuffers &i = buffers.image_buf;
+    if (args.trace_mode == 0)
+        return hip_status(launch_trace_wavefront(i.camera, i.image_width, i.image_height, seed, args.direct_samples,
+                                                 buffers.scene_args(), shadow_any_hit, i.d_hp_pos.as<phx_float4>(),
+                                                 i.d_hp_normal.as<phx_float4>(), i.d_hp_throughput.as<phx_float4>(),
+                                                 i.d_hp_base_color.as<phx_float4>(), i.d_hp_direct.as<phx_float4>(),
+                                                 i.d_hp_emission.as<phx_float4>(), d_trace_ws.p, stream),
+                          "trace_rays");
     return hip_status(launch_trace(i.camera, i.image_width, i.image_height, seed, args.direct_samples,
                                    buffers.scene_args(), i.d_hp_pos.as<phx_float4>(), i.d_hp_normal.as<phx_float4>(),
                                    i.d_hp_throughput.as<phx_float4>(), i.d_hp_base_color.as<phx_float4>(),
@@ -436,6 +467,8 @@ int phx_renderer_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, Ph
         return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: grid_res outside 1..255 (cmd_args.hpp:51)");
     if (args->max_photon_bounces == 0 || args->max_photon_bounces > 32)
         return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: max_bounces outside 1..32 (cmd_args.hpp:49)");
+    if (args->trace_mode > 1)
+        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: trace_mode must be 0 or 1");
     if (args->direct_samples == 0 || args->direct_samples > 128)
         return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: direct_samples outside 1..128 (cmd_args.hpp:59)");
     int dev_count = 0;

@@ -44,6 +44,35 @@ class DeviceScene:
         self.n = sc.n_triangles
 
 
+def _check_wavefront(ds, sc, cfg, seed, want, any_hit):
+    L = P.lib()
+    n_px = cfg.width * cfg.height
+    hp = [empty_dev(16 * n_px, 0x7F) for _ in range(6)]
+    wsb = L.phx_trace_rays_wavefront_workspace_size(cfg.width, cfg.height, cfg.direct_samples)
+    ws = empty_dev(wsb, 0x33)
+    for _ in range(2):  # the workspace is reused as is
+        P.check(L.phx_trace_rays_wavefront(sc.camera.ctypes.data, cfg.width, cfg.height, seed, cfg.direct_samples,
+                                           dptr(ds.tv), dptr(ds.tn), dptr(ds.tt), dptr(ds.tuv), dptr(ds.nodes),
+                                           dptr(ds.tmat), ds.n, dptr(ds.mats), dptr(ds.uvt), dptr(ds.tex_meta),
+                                           dptr(ds.atlas), dptr(ds.lights), len(sc.lights), dptr(ds.lum),
+                                           ds.total_lum, ds.center.ctypes.data, ds.radius, dptr(ds.etv),
+                                           dptr(ds.etn), dptr(ds.etuv), dptr(ds.etmat), None, 0, 0, 0,
+                                           *(dptr(b) for b in hp), any_hit, dptr(ws), wsb, None), "wavefront")
+        got = np.stack([from_dev(b, np.float32, (n_px, 4)) for b in hp], axis=1)
+        bad = np.nonzero((got.view(np.uint32) != want.view(np.uint32)).any(axis=(1, 2)))[0]
+        assert bad.size == 0, f"{bad.size} pixels differ, first {bad[:5]}"
+
+
+def test_wavefront_trace_any_hit_on_opaque_scene():
+    sc = scenes.cornell_box(64, 48)
+    cfg = PassConfig(width=64, height=48, photons_per_round=512, max_photon_bounces=4, grid_res=20, seed=4)
+    ref = oracle.RefRenderer(cfg, sc)
+    assert ref.run_pass() == 0
+    ds = DeviceScene(sc, ref)
+    seed = oracle.lib().ref_random_u32(np.array([oracle.lib().ref_pcg_seed(4)], np.uint32).ctypes.data)
+    _check_wavefront(ds, sc, cfg, seed, ref.read_hitpoints(), any_hit=1)
+
+
 def test_trace_emit_gather_entry_points():
     sc = scenes.caustic_scene(48, 40, level=3)
     cfg = PassConfig(width=48, height=40, photons_per_round=6000, max_photons_per_batch=6000, max_photon_bounces=12,
@@ -64,6 +93,7 @@ def test_trace_emit_gather_entry_points():
     got = np.stack([from_dev(b, np.float32, (n_px, 4)) for b in hp], axis=1)
     want = ref.read_hitpoints()
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    _check_wavefront(ds, sc, cfg, seed, want, any_hit=0)
 
     # emission in two batches with a global offset, then a truncated buffer
     for max_photons in (cfg.photons_per_round * cfg.max_photon_bounces, 2500):
