@@ -279,7 +279,7 @@ int phx_gather_flux(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const
         return fail(PHX_ERR_INVALID_ARG, "phx_gather_flux: grid_res outside 1..255");
     return hip_status(launch_gather(hp_pos, hp_normal, hp_throughput, hp_base_color, hp_direct, hp_emission, n_pixels,
                                     photon_pos, photon_power, photon_dir, cell_start, cell_end, *data, sppm_alpha,
-                                    flux, photon_weight, radiance, (hipStream_t)stream),
+                                    flux, photon_weight, radiance, 0u, (hipStream_t)stream),
                       "phx_gather_flux");
 }
 

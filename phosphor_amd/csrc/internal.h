@@ -88,7 +88,8 @@ hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, 
                          const phx_float4 *hp_base_color, const phx_float4 *hp_direct, const phx_float4 *hp_emission,
                          uint32_t n_pixels, const phx_float4 *ppos, const uint32_t *ppow, const uint32_t *pdir,
                          const uint32_t *cell_start, const uint32_t *cell_end, const PhxPhotonHashData &d,
-                         float alpha, phx_float4 *flux, float *photon_weight, phx_float4 *radiance, hipStream_t s);
+                         float alpha, phx_float4 *flux, float *photon_weight, phx_float4 *radiance, uint32_t width,
+                         hipStream_t s);
 
 /* error bookkeeping of the C ABI */
 void set_error(const std::string &msg);

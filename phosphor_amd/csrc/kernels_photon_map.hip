@@ -437,8 +437,25 @@ struct GatherArgs {
     phx_float4 *radiance;
 };
 
-__global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
-    u32 tid = blockIdx.x * 256 + threadIdx.x;
+/* One wave per 8×8 pixel tile: neighbouring hitpoints query the same grid
+ * cells, so the lanes of a wave read the same photon records (one memory
+ * transaction serves many lanes) and run loops of similar length; tiles are
+ * assigned XCD by XCD so each L2 holds the photons of one image region.  Per
+ * pixel the arithmetic and the order of the sums are those of
+ * photon_hash.h:65-100. */
+__global__ __launch_bounds__(256) void k_gather(GatherArgs a, u32 width, u32 tiles_x) {
+    u32 tid;
+    if (tiles_x == 0) { /* no image shape known: consecutive pixels */
+        tid = blockIdx.x * 256 + threadIdx.x;
+    } else {
+        u32 wave = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+        u32 lane = threadIdx.x & 63u;
+        u32 tx = wave % tiles_x, ty = wave / tiles_x;
+        u32 x = tx * 8 + (lane & 7u), y = ty * 8 + (lane >> 3);
+        if (x >= width)
+            return;
+        tid = y * width + x;
+    }
     if (tid >= a.n_pixels)
         return;
     F4 ge = ld(a.hp_emission + tid);
@@ -461,21 +478,20 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
         f32 hx = d.cell_sizes.x * 0.5f, hy = d.cell_sizes.y * 0.5f, hz = d.cell_sizes.z * 0.5f;
         r_sq = hx * hx + hy * hy + hz * hz;
     }
+    F4 pos = ld(a.hp_pos + tid), sn = ld(a.hp_normal + tid);
     F4 nf = BLACK;
     f32 found = 0.0f;
-    F4 pos = ld(a.hp_pos + tid), sn = ld(a.hp_normal + tid);
     f32 two_r_sq = 2.0f * r_sq;
     for (u32 i = 0; i < 27; i++) {
         i32 cx = (i32)(i % 3) - 1, cy = (i32)((i / 3) % 3) - 1, cz = (i32)((i / 9) % 3) - 1;
-        F4 p2 = pos;
-        p2.x += (f32)cx * d.cell_sizes.x;
-        p2.y += (f32)cy * d.cell_sizes.y;
-        p2.z += (f32)cz * d.cell_sizes.z;
-        u32 h = get_hash(p2.x, p2.y, p2.z, d);
+        f32 qx = pos.x + (f32)cx * d.cell_sizes.x;
+        f32 qy = pos.y + (f32)cy * d.cell_sizes.y;
+        f32 qz = pos.z + (f32)cz * d.cell_sizes.z;
+        u32 h = get_hash(qx, qy, qz, d);
         if (h == 0)
             continue;
-        u32 s0 = a.cell_start[h], e0 = a.cell_end[h];
-        for (u32 k = s0; k < e0; k++) {
+        u32 e0 = a.cell_end[h];
+        for (u32 k = a.cell_start[h]; k < e0; k++) {
             F4 dd = pos - ld(a.ppos + k);
             f32 dsq = dot(dd, dd);
             if (dsq >= r_sq)
@@ -504,12 +520,19 @@ hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, 
                          const phx_float4 *hp_base_color, const phx_float4 *hp_direct, const phx_float4 *hp_emission,
                          u32 n_pixels, const phx_float4 *ppos, const u32 *ppow, const u32 *pdir, const u32 *cell_start,
                          const u32 *cell_end, const PhxPhotonHashData &d, f32 alpha, phx_float4 *flux,
-                         f32 *photon_weight, phx_float4 *radiance, hipStream_t s) {
+                         f32 *photon_weight, phx_float4 *radiance, u32 width, hipStream_t s) {
     if (n_pixels == 0)
         return hipSuccess;
     GatherArgs a{hp_pos, hp_normal, hp_throughput, hp_base_color, hp_direct, hp_emission, n_pixels, ppos, ppow,
                  pdir,   cell_start, cell_end,   d,             alpha,     flux,      photon_weight, radiance};
-    hipLaunchKernelGGL(k_gather, dim3((n_pixels + 255) / 256), dim3(256), 0, s, a);
+    if (width == 0) { /* image shape unknown (C ABI call): flat pixel range */
+        hipLaunchKernelGGL(k_gather, dim3((n_pixels + 255) / 256), dim3(256), 0, s, a, 0u, 0u);
+        return hipGetLastError();
+    }
+    u32 h = (n_pixels + width - 1) / width;
+    u32 tiles_x = (width + 7) / 8, tiles_y = (h + 7) / 8;
+    u32 waves = tiles_x * tiles_y;
+    hipLaunchKernelGGL(k_gather, dim3((waves + 3) / 4), dim3(256), 0, s, a, width, tiles_x);
     return hipGetLastError();
 }
 

@@ -9,18 +9,21 @@
  * one LDS stack.  The sequence of random draws, the accumulation order of
  * the NEE sums and every expression are those of the reference.
  *
- * emit_photons (kernels/emit_photons.cl:12-114): one lane per photon; the
- * k-th deposit of photon i goes to a depth-major staging slot [k][i] and the
- * per-photon deposit count is stored; a second kernel compacts the staging
- * area in (photon, bounce) order after an exclusive scan of the counts, which
- * makes the deposit order canonical instead of atomic_inc arrival order.
+ * trace_rays also has a wavefront form (camera paths, queued shadow rays,
+ * in-order resolve) with the same outputs; the pass driver uses it.
+ *
+ * emit_photons (kernels/emit_photons.cl:12-114): one launch per bounce over
+ * a queue of live photons; the k-th deposit of photon i goes to a depth-major
+ * staging slot [k][i] and the per-photon deposit count is stored; a final
+ * kernel compacts the staging area in (photon, bounce) order after an
+ * exclusive scan of the counts, which makes the deposit order canonical
+ * instead of atomic_inc arrival order.
  */
 #include "phx_device.h"
 
 namespace phx {
 
 constexpr int TRACE_BLOCK = 128;
-constexpr int EMIT_BLOCK = 128;
 
 struct TraceArgs {
     PhxCameraParams cam;
@@ -664,72 +667,6 @@ hipError_t launch_trace_wavefront(const PhxCameraParams &cam, u32 w, u32 h, u32 
     return hipGetLastError();
 }
 
-/* ---------------------------------------------------------------------- */
-
-struct EmitArgs {
-    SceneView sc;
-    u32 offset, photons_to_emit, max_bounces, seed;
-    phx_float4 *st_pos; /* [max_bounces][photons_to_emit] */
-    u32 *st_power, *st_dir, *count;
-};
-
-__global__ __launch_bounds__(EMIT_BLOCK) void k_emit_paths(EmitArgs a) {
-    __shared__ u32 s_stack[BVH_STACK_SIZE * EMIT_BLOCK];
-    __shared__ f32 s_ior[IOR_STACK_SIZE * EMIT_BLOCK];
-    const SceneView &sc = a.sc;
-    u32 li = blockIdx.x * EMIT_BLOCK + threadIdx.x;
-    if (li >= a.photons_to_emit)
-        return;
-    u32 *stk = s_stack + threadIdx.x;
-    u32 tid = li + a.offset;
-    u32 rng = pcg_seed(a.seed + tid);
-    F4 origin, dir, power;
-    sample_light(rng, sc, origin, dir, power);
-    IorStack ior{s_ior + threadIdx.x, (u32)EMIT_BLOCK, 0u};
-    init_ior_stack(ior, AIR_IOR);
-    F4 throughput = mk(1.0f, 1.0f, 1.0f, 0.0f);
-    u32 n = 0;
-    size_t stride = a.photons_to_emit;
-    for (u32 depth = 0; depth < a.max_bounces; depth++) {
-        HitRecord rec;
-        if (!scene_intersect(sc, origin, dir, EPS, INF, rec, stk, EMIT_BLOCK))
-            break;
-        SurfaceHit sh = process_hit(sc, rec, origin, dir);
-        const PhxMaterial &mat = sc.materials[sh.mat_index];
-        f32 curr_ior = current_ior(ior);
-        if (curr_ior != AIR_IOR && mat.thickness > EPS && mat.att_dist > EPS) {
-            F4 vol = beer_lambert(from(mat.att_color), mat.att_dist, sh.t);
-            power = power * vol;
-            throughput = throughput * vol;
-        }
-        Ctx c = evaluate_material(sc, mat, sh.mat_index, rec, sh);
-        if (depth >= RR_MIN_PHOTON_DEPTH) {
-            f32 q = phx_fmaxf(throughput.x, phx_fmaxf(throughput.y, throughput.z));
-            q = phx_clampf(q, RR_MIN_Q, RR_MAX_Q);
-            if (random_float(rng) > q)
-                break;
-            f32 iq = 1.0f / q;
-            throughput = throughput * iq;
-            power = power * iq;
-        }
-        F4 view = -dir;
-        Bsdf bs = sample_bsdf(rng, c, c.shading_normal, sh.normal, view, ior, sh.front_face);
-        if (bs.event == BSDF_DIFFUSE && dot(dir, sh.normal) < 0.0f) {
-            size_t slot = (size_t)n * stride + li;
-            st(a.st_pos + slot, sh.position);
-            a.st_power[slot] = encode_rgbe(power);
-            a.st_dir[slot] = encode_oct(dir);
-            n++;
-        }
-        power = power * bs.throughput;
-        throughput = throughput * bs.throughput;
-        F4 side = (dot(bs.dir, sh.normal) > 0.0f) ? sh.normal : -sh.normal;
-        origin = sh.position + bs.dir * EPS + side * EPS;
-        dir = bs.dir;
-    }
-    a.count[li] = n;
-}
-
 /* Moves photon li's deposits to base + offset[li] + k (base = the counter
  * value before this batch), dropping those at or beyond max_photons. */
 __global__ __launch_bounds__(256) void k_emit_compact(const phx_float4 *__restrict__ st_pos,
@@ -755,16 +692,209 @@ __global__ __launch_bounds__(256) void k_emit_compact(const phx_float4 *__restri
 
 __global__ void k_counter_add(u32 *counter, const u32 *total) { *counter += *total; }
 
+/* ----------------------------------------------------------------------
+ * Wavefront form of emit_photons (same deposits, bit for bit).  Bounce k of
+ * every live photon runs in one launch over a queue of live photons, so the
+ * lanes of a wave stay busy however the path lengths differ; a photon's state
+ * (origin, direction, power, throughput, PCG state, IOR stack, deposit count)
+ * lives in HBM between bounces.  Queue order does not matter: every photon
+ * writes only its own state and its own deposit slots [k][photon].
+ * ---------------------------------------------------------------------- */
+constexpr int EMIT_WF_BLOCK = 128;
+constexpr u32 EMIT_WF_BOUNCES = 3; /* bounce-synchronous launches before the finishing launch */
+
+struct EmitWfState {
+    phx_float4 *origin, *dir, *power, *thr;
+    u32 *rng, *ior_cur, *count;
+    f32 *ior; /* [depth][photon] */
+    u32 *queue[2];
+    u32 *qcount; /* [bounce] live photons entering bounce k */
+};
+
+struct EmitWfArgs {
+    SceneView sc;
+    u32 offset, photons, max_bounces, seed, depth;
+    EmitWfState st;
+    phx_float4 *st_pos;
+    u32 *st_power, *st_dir;
+};
+
+__global__ __launch_bounds__(256) void k_emit_wf_start(EmitWfArgs a) {
+    u32 li = blockIdx.x * 256 + threadIdx.x;
+    if (li >= a.photons)
+        return;
+    const EmitWfState &S = a.st;
+    u32 rng = pcg_seed(a.seed + li + a.offset);
+    F4 origin, dir, power;
+    sample_light(rng, a.sc, origin, dir, power);
+    st(S.origin + li, origin);
+    st(S.dir + li, dir);
+    st(S.power + li, power);
+    st(S.thr + li, mk(1.0f, 1.0f, 1.0f, 0.0f));
+    S.rng[li] = rng;
+    S.ior_cur[li] = 0;
+    S.ior[li] = AIR_IOR;
+    S.count[li] = 0;
+}
+
+/* One bounce of emit_photons.cl:49-113 on a photon's state; returns false
+ * when the path ends (escape, roulette or the bounce limit). */
+__device__ __forceinline__ bool emit_step(const EmitWfArgs &a, u32 li, u32 depth, F4 &origin, F4 &dir, F4 &power,
+                                          F4 &throughput, u32 &rng, IorStack &ior, u32 &n, u32 *stk) {
+    const SceneView &sc = a.sc;
+    HitRecord rec;
+    if (!scene_intersect(sc, origin, dir, EPS, INF, rec, stk, EMIT_WF_BLOCK))
+        return false;
+    SurfaceHit sh = process_hit(sc, rec, origin, dir);
+    const PhxMaterial &mat = sc.materials[sh.mat_index];
+    f32 curr_ior = current_ior(ior);
+    if (curr_ior != AIR_IOR && mat.thickness > EPS && mat.att_dist > EPS) {
+        F4 vol = beer_lambert(from(mat.att_color), mat.att_dist, sh.t);
+        power = power * vol;
+        throughput = throughput * vol;
+    }
+    Ctx c = evaluate_material(sc, mat, sh.mat_index, rec, sh);
+    if (depth >= RR_MIN_PHOTON_DEPTH) {
+        f32 q = phx_fmaxf(throughput.x, phx_fmaxf(throughput.y, throughput.z));
+        q = phx_clampf(q, RR_MIN_Q, RR_MAX_Q);
+        if (random_float(rng) > q)
+            return false;
+        f32 iq = 1.0f / q;
+        throughput = throughput * iq;
+        power = power * iq;
+    }
+    F4 view = -dir;
+    Bsdf bs = sample_bsdf(rng, c, c.shading_normal, sh.normal, view, ior, sh.front_face);
+    if (bs.event == BSDF_DIFFUSE && dot(dir, sh.normal) < 0.0f) {
+        size_t slot = (size_t)n * a.photons + li;
+        st(a.st_pos + slot, sh.position);
+        a.st_power[slot] = encode_rgbe(power);
+        a.st_dir[slot] = encode_oct(dir);
+        n++;
+    }
+    power = power * bs.throughput;
+    throughput = throughput * bs.throughput;
+    F4 side = (dot(bs.dir, sh.normal) > 0.0f) ? sh.normal : -sh.normal;
+    origin = sh.position + bs.dir * EPS + side * EPS;
+    dir = bs.dir;
+    return depth + 1 < a.max_bounces;
+}
+
+__global__ __launch_bounds__(EMIT_WF_BLOCK) void k_emit_wf_bounce(EmitWfArgs a) {
+    __shared__ u32 s_stack[BVH_STACK_SIZE * EMIT_WF_BLOCK];
+    u32 *stk = s_stack + threadIdx.x;
+    const EmitWfState &S = a.st;
+    u32 depth = a.depth;
+    u32 n_live = depth == 0 ? a.photons : S.qcount[depth];
+    const u32 *qin = S.queue[depth & 1];
+    u32 *qout = S.queue[(depth + 1) & 1];
+    u32 stride = gridDim.x * EMIT_WF_BLOCK;
+    u32 rounds = (n_live + stride - 1) / stride; /* same trip count for every lane of the grid */
+    u32 lane = threadIdx.x & 63u;
+    for (u32 r = 0; r < rounds; r++) {
+        if (r * stride + blockIdx.x * EMIT_WF_BLOCK >= n_live)
+            break; /* the whole block is past the live photons */
+        u32 qi = r * stride + blockIdx.x * EMIT_WF_BLOCK + threadIdx.x;
+        bool keep = false;
+        u32 li = 0;
+        if (qi < n_live) {
+            li = depth == 0 ? qi : qin[qi];
+            F4 origin = ld(S.origin + li), dir = ld(S.dir + li);
+            F4 power = ld(S.power + li), throughput = ld(S.thr + li);
+            u32 rng = S.rng[li];
+            IorStack ior{S.ior + li, a.photons, S.ior_cur[li]};
+            u32 n = S.count[li], n0 = n;
+            keep = emit_step(a, li, depth, origin, dir, power, throughput, rng, ior, n, stk);
+            if (n != n0)
+                S.count[li] = n;
+            if (keep) {
+                st(S.origin + li, origin);
+                st(S.dir + li, dir);
+                st(S.power + li, power);
+                st(S.thr + li, throughput);
+                S.rng[li] = rng;
+                S.ior_cur[li] = ior.cur;
+            }
+        }
+        /* append survivors to the next queue, one atomic per wave */
+        unsigned long long m = __ballot(keep);
+        if (m) {
+            u32 first = (u32)__ffsll((long long)m) - 1u;
+            u32 base = 0;
+            if (lane == first)
+                base = atomicAdd(&S.qcount[depth + 1], (u32)__popcll(m));
+            base = __shfl(base, (int)first, 64);
+            if (keep)
+                qout[base + (u32)__popcll(m & ((1ull << lane) - 1ull))] = li;
+        }
+    }
+}
+
+/* Runs every photon still live at bounce a.depth to the end of its path in
+ * one launch, so the long tail of the path-length distribution costs one
+ * launch instead of one per bounce. */
+__global__ __launch_bounds__(EMIT_WF_BLOCK) void k_emit_wf_finish(EmitWfArgs a) {
+    __shared__ u32 s_stack[BVH_STACK_SIZE * EMIT_WF_BLOCK];
+    u32 *stk = s_stack + threadIdx.x;
+    const EmitWfState &S = a.st;
+    u32 n_live = a.depth == 0 ? a.photons : S.qcount[a.depth];
+    const u32 *qin = S.queue[a.depth & 1];
+    for (u32 qi = blockIdx.x * EMIT_WF_BLOCK + threadIdx.x; qi < n_live; qi += gridDim.x * EMIT_WF_BLOCK) {
+        u32 li = a.depth == 0 ? qi : qin[qi];
+        F4 origin = ld(S.origin + li), dir = ld(S.dir + li);
+        F4 power = ld(S.power + li), throughput = ld(S.thr + li);
+        u32 rng = S.rng[li];
+        IorStack ior{S.ior + li, a.photons, S.ior_cur[li]};
+        u32 n = S.count[li], n0 = n;
+        for (u32 depth = a.depth; depth < a.max_bounces; depth++)
+            if (!emit_step(a, li, depth, origin, dir, power, throughput, rng, ior, n, stk))
+                break;
+        if (n != n0)
+            S.count[li] = n;
+    }
+}
+
+static EmitWfState emit_wf_layout(char *&p, u32 photons, u32 max_bounces) {
+    EmitWfState S;
+    size_t n = photons;
+    S.origin = (phx_float4 *)p;
+    p += align_up(n * 16, 256);
+    S.dir = (phx_float4 *)p;
+    p += align_up(n * 16, 256);
+    S.power = (phx_float4 *)p;
+    p += align_up(n * 16, 256);
+    S.thr = (phx_float4 *)p;
+    p += align_up(n * 16, 256);
+    S.rng = (u32 *)p;
+    p += align_up(n * 4, 256);
+    S.ior_cur = (u32 *)p;
+    p += align_up(n * 4, 256);
+    S.count = (u32 *)p;
+    p += align_up(n * 4, 256);
+    S.ior = (f32 *)p;
+    p += align_up(n * IOR_STACK_SIZE * 4, 256);
+    S.queue[0] = (u32 *)p;
+    p += align_up(n * 4, 256);
+    S.queue[1] = (u32 *)p;
+    p += align_up(n * 4, 256);
+    S.qcount = (u32 *)p;
+    p += align_up(((size_t)max_bounces + 2) * 4, 256);
+    return S;
+}
+
 size_t emit_workspace_bytes(u32 photons, u32 max_bounces) {
     size_t slots = (size_t)photons * max_bounces;
+    size_t n = photons;
     size_t b = 0;
     b += align_up(slots * sizeof(phx_float4), 256); /* staging pos */
     b += align_up(slots * sizeof(u32), 256);        /* staging power */
     b += align_up(slots * sizeof(u32), 256);        /* staging dir */
-    b += align_up((size_t)photons * sizeof(u32), 256); /* counts */
-    b += align_up((size_t)photons * sizeof(u32), 256); /* offsets */
-    b += 256;                                          /* total */
+    b += align_up(n * sizeof(u32), 256);            /* offsets */
+    b += 256;                                       /* total */
     b += align_up(scan_workspace_bytes(photons), 256);
+    /* wavefront state */
+    b += 4 * align_up(n * 16, 256) + 5 * align_up(n * 4, 256) + align_up(n * IOR_STACK_SIZE * 4, 256) +
+         align_up(((size_t)max_bounces + 2) * 4, 256);
     return b;
 }
 
@@ -781,33 +911,56 @@ hipError_t launch_emit(const SceneArgs &sc, u32 max_photons, u32 offset, u32 pho
     p += align_up(slots * sizeof(u32), 256);
     u32 *st_dir = (u32 *)p;
     p += align_up(slots * sizeof(u32), 256);
-    u32 *count = (u32 *)p;
-    p += align_up((size_t)photons * sizeof(u32), 256);
     u32 *offs = (u32 *)p;
     p += align_up((size_t)photons * sizeof(u32), 256);
     u32 *total = (u32 *)p;
     p += 256;
     void *scan_ws = p;
+    p += align_up(scan_workspace_bytes(photons), 256);
+    EmitWfState S = emit_wf_layout(p, photons, max_bounces);
 
-    EmitArgs a;
+    hipError_t e = launch_fill_u32(S.qcount, 0u, (size_t)max_bounces + 2, s);
+    if (e != hipSuccess)
+        return e;
+    EmitWfArgs a;
     a.sc = sc;
     a.offset = offset;
-    a.photons_to_emit = photons;
+    a.photons = photons;
     a.max_bounces = max_bounces;
     a.seed = seed;
+    a.depth = 0;
+    a.st = S;
     a.st_pos = st_pos;
     a.st_power = st_power;
     a.st_dir = st_dir;
-    a.count = count;
-    hipLaunchKernelGGL(k_emit_paths, dim3((photons + EMIT_BLOCK - 1) / EMIT_BLOCK), dim3(EMIT_BLOCK), 0, s, a);
-    hipError_t e = hipGetLastError();
+    hipLaunchKernelGGL(k_emit_wf_start, dim3((photons + 255) / 256), dim3(256), 0, s, a);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    /* enough blocks to fill the GPU; bounces with few live photons retire
+       the surplus blocks at once */
+    u32 blocks = (photons + EMIT_WF_BLOCK - 1) / EMIT_WF_BLOCK;
+    if (blocks > 4096)
+        blocks = 4096;
+    /* bounce-synchronous launches while most photons are alive, then one
+       launch for the rest of every path */
+    u32 wave_bounces = max_bounces < EMIT_WF_BOUNCES ? max_bounces : EMIT_WF_BOUNCES;
+    for (u32 depth = 0; depth < wave_bounces; depth++) {
+        a.depth = depth;
+        hipLaunchKernelGGL(k_emit_wf_bounce, dim3(blocks), dim3(EMIT_WF_BLOCK), 0, s, a);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+    }
+    if (wave_bounces < max_bounces) {
+        a.depth = wave_bounces;
+        hipLaunchKernelGGL(k_emit_wf_finish, dim3(blocks), dim3(EMIT_WF_BLOCK), 0, s, a);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+    }
+    e = launch_scan(S.count, photons, offs, nullptr, total, scan_ws, s);
     if (e != hipSuccess)
         return e;
-    e = launch_scan(count, photons, offs, nullptr, total, scan_ws, s);
-    if (e != hipSuccess)
-        return e;
-    hipLaunchKernelGGL(k_emit_compact, dim3((photons + 255) / 256), dim3(256), 0, s, st_pos, st_power, st_dir, count,
-                       offs, photons, n_photons, max_photons, photon_pos, photon_power, photon_dir);
+    hipLaunchKernelGGL(k_emit_compact, dim3((photons + 255) / 256), dim3(256), 0, s, st_pos, st_power, st_dir,
+                       S.count, offs, photons, n_photons, max_photons, photon_pos, photon_power, photon_dir);
     e = hipGetLastError();
     if (e != hipSuccess)
         return e;

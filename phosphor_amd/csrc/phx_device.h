@@ -91,6 +91,17 @@ constexpr F4 ZERO = {0.0f, 0.0f, 0.0f, 0.0f};
 constexpr F4 BLACK = {0.0f, 0.0f, 0.0f, 0.0f};
 constexpr F4 WHITE = {1.0f, 1.0f, 1.0f, 0.0f};
 
+/* Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
+ * with its own L2.  Mapping dispatch order b to logical block
+ * (b mod 8)·⌈n/8⌉-style chunks gives every XCD one contiguous range of
+ * logical blocks, so blocks that touch neighbouring data share an L2.  The
+ * map is a bijection on [0, n). */
+PHX_DEV u32 xcd_block(u32 b, u32 n) {
+    u32 q = n / 8u, r = n % 8u;
+    u32 x = b % 8u, k = b / 8u;
+    return x < r ? x * (q + 1u) + k : r * (q + 1u) + (x - r) * q + k;
+}
+
 /* ---------------- random.h:16-62 (PCG32, RXS-M-XS output) ---------------- */
 PHX_DEV u32 pcg_seed(u32 seed) {
     u32 s = 0u;

@@ -425,7 +425,7 @@ int PhxRenderer::gather_flux() {
                                     p.d_photon_dir.as<uint32_t>(), p.d_cell_start.as<uint32_t>(),
                                     p.d_cell_end.as<uint32_t>(), photon_hash_info, args.sppm_alpha,
                                     i.d_flux.as<phx_float4>(), i.d_photon_weight.as<float>(),
-                                    i.d_radiance.as<phx_float4>(), stream),
+                                    i.d_radiance.as<phx_float4>(), i.image_width, stream),
                       "gather_flux");
 }
 
