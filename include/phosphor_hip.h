@@ -322,7 +322,11 @@ typedef struct PhxRenderArgs {
     float sppm_alpha;               /* --sppm-alpha */
     float initial_radius;           /* 0 → reference rule (cell half-diagonal, gather_flux.cl:51-56) */
     uint32_t trace_mode;            /* 0 = wavefront trace (default), 1 = one-kernel trace; same results */
+    uint32_t flags;                 /* PHX_PASS_* */
 } PhxRenderArgs;
+
+/* Capture the pass into a hipGraph on the first run_pass and replay it. */
+#define PHX_PASS_GRAPH 1u
 
 typedef struct PhxRenderer PhxRenderer;
 
@@ -330,13 +334,23 @@ typedef struct PhxRenderer PhxRenderer;
 int phx_renderer_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, PhxRenderer **out);
 void phx_renderer_destroy(PhxRenderer *r);
 /* One SPPM round of Renderer::run (renderer.cpp:64-78): round_seed from the
- * host PCG, then trace → emit → hash/scan/sort → gather.  Asynchronous; the
- * per-pass deposit count is read back once (renderer.cpp:113-118). */
+ * PCG seeded with args.seed (drawn on the device), then trace → emit →
+ * hash/scan/sort → gather.  Asynchronous: the pass never waits for the host
+ * (the deposit count of renderer.cpp:113-118 stays on the device). */
 int phx_renderer_run_pass(PhxRenderer *r);
-/* Block until all enqueued passes have finished. */
+/* Block until all enqueued passes have finished; PHX_ERR_NO_PHOTONS if a
+ * pass since the last call deposited nothing (renderer.cpp:117-118). */
 int phx_renderer_synchronize(PhxRenderer *r);
-/* Stage timings of the last pass in milliseconds: trace, emit, hash+scan+sort, gather. */
+/* Stage timings of the last pass in milliseconds: trace, emit, hash+scan+sort,
+ * gather (zeros for graph replays, which have no stage events). */
 int phx_renderer_last_stage_ms(PhxRenderer *r, float out_ms[4]);
+/* Duration of the last pass in milliseconds (both modes). */
+int phx_renderer_last_pass_ms(PhxRenderer *r, float *out_ms);
+/* Checkpoint / resume of the progressive state (flux, photon_weight,
+ * radiance, round count, seed generator).  No counterpart in the reference. */
+size_t phx_renderer_checkpoint_bytes(PhxRenderer *r);
+int phx_renderer_save_checkpoint(PhxRenderer *r, void *buf, size_t bytes);
+int phx_renderer_load_checkpoint(PhxRenderer *r, const void *buf, size_t bytes);
 /* Number of deposited photons in the last pass (before the clamp). */
 uint32_t phx_renderer_last_photon_count(PhxRenderer *r);
 uint32_t phx_renderer_rounds_done(PhxRenderer *r);

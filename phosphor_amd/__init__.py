@@ -64,11 +64,13 @@ class PassConfig:
     sppm_alpha: float = 0.7
     initial_radius: float = 0.0
     trace_mode: int = 0  # 0 wavefront trace, 1 one-kernel trace (identical results)
+    graph: bool = False  # capture the pass into a hipGraph once and replay it
 
     def to_c(self) -> RenderArgs:
         return RenderArgs(self.width, self.height, self.photons_per_round, self.max_photons_per_batch,
                           self.max_photon_bounces, self.grid_res, self.direct_samples, self.seed,
-                          self.sppm_alpha, self.initial_radius, self.trace_mode)
+                          self.sppm_alpha, self.initial_radius, self.trace_mode,
+                          abi.PHX_PASS_GRAPH if self.graph else 0)
 
 
 class Renderer:
@@ -117,6 +119,28 @@ class Renderer:
         out = np.zeros(4, np.float32)
         check(lib().phx_renderer_last_stage_ms(self._h, ptr(out)), "stage_ms")
         return out
+
+    def pass_ms(self) -> float:
+        out = np.zeros(1, np.float32)
+        check(lib().phx_renderer_last_pass_ms(self._h, ptr(out)), "pass_ms")
+        return float(out[0])
+
+    def checkpoint(self) -> bytes:
+        """Progressive state (flux, N, radiance, round count, seed generator)."""
+        n = lib().phx_renderer_checkpoint_bytes(self._h)
+        buf = np.zeros(n, np.uint8)
+        check(lib().phx_renderer_save_checkpoint(self._h, ptr(buf), n), "checkpoint")
+        return buf.tobytes()
+
+    def restore(self, data: bytes) -> None:
+        buf = np.frombuffer(data, np.uint8).copy()
+        check(lib().phx_renderer_load_checkpoint(self._h, ptr(buf), buf.size), "restore")
+
+    def save(self, path) -> None:
+        Path(path).write_bytes(self.checkpoint())
+
+    def load(self, path) -> None:
+        self.restore(Path(path).read_bytes())
 
     @property
     def last_photon_count(self) -> int:

@@ -61,8 +61,11 @@ class RenderArgs(C.Structure):
         ("width", C.c_uint32), ("height", C.c_uint32), ("photons_per_round", C.c_uint32),
         ("max_photons_per_batch", C.c_uint32), ("max_photon_bounces", C.c_uint32), ("grid_res", C.c_uint32),
         ("direct_samples", C.c_uint32), ("seed", C.c_uint32), ("sppm_alpha", C.c_float),
-        ("initial_radius", C.c_float), ("trace_mode", C.c_uint32),
+        ("initial_radius", C.c_float), ("trace_mode", C.c_uint32), ("flags", C.c_uint32),
     ]
+
+
+PHX_PASS_GRAPH = 1
 
 
 class SceneDesc(C.Structure):
@@ -126,6 +129,10 @@ SIGNATURES = {
     "phx_renderer_run_pass": (INT, VP),
     "phx_renderer_synchronize": (INT, VP),
     "phx_renderer_last_stage_ms": (INT, VP, VP),
+    "phx_renderer_last_pass_ms": (INT, VP, VP),
+    "phx_renderer_checkpoint_bytes": (C.c_size_t, VP),
+    "phx_renderer_save_checkpoint": (INT, VP, VP, C.c_size_t),
+    "phx_renderer_load_checkpoint": (INT, VP, VP, C.c_size_t),
     "phx_renderer_last_photon_count": (U32, VP),
     "phx_renderer_rounds_done": (U32, VP),
     "phx_renderer_read_state": (INT, VP, VP, VP, VP),

@@ -46,27 +46,35 @@ struct SceneArgs {
 constexpr uint32_t SCAN_TILE = 4096; /* 256 threads × 16 elements */
 size_t scan_workspace_bytes(uint32_t n);
 /* out = exclusive scan of in; out2 (optional) receives a copy; total
- * (optional, device) receives the sum.  `ws` holds scan_workspace_bytes(n). */
+ * (optional, device) receives the sum.  `ws` holds scan_workspace_bytes(n).
+ * With d_n (device) the length is min(*d_n, n). */
 hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *total, void *ws,
-                       hipStream_t s);
+                       hipStream_t s, const uint32_t *d_n = nullptr);
 
 hipError_t launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
+/* seed[i] = random_u32(state[i]) for i < n (PCG, random.h:29-34) */
+hipError_t launch_next_round_seed(uint32_t *state, uint32_t *seed, uint32_t n, hipStream_t s);
+/* info[0] = *count; info[1] counts passes with a zero count */
+hipError_t launch_note_count(const uint32_t *count, uint32_t *info, hipStream_t s);
 
 hipError_t launch_trace(const PhxCameraParams &cam, uint32_t w, uint32_t h, uint32_t seed, uint32_t direct_samples,
                         const SceneArgs &sc, phx_float4 *hp_pos, phx_float4 *hp_normal, phx_float4 *hp_throughput,
-                        phx_float4 *hp_base_color, phx_float4 *hp_direct, phx_float4 *hp_emission, hipStream_t s);
+                        phx_float4 *hp_base_color, phx_float4 *hp_direct, phx_float4 *hp_emission, hipStream_t s,
+                        const uint32_t *d_seed = nullptr);
 
-size_t trace_wavefront_workspace_bytes(uint32_t n_pixels, uint32_t direct_samples);
+/* with_scene: room for a scene index per shadow slot (multi-scene queues) */
+size_t trace_wavefront_workspace_bytes(uint32_t n_pixels, uint32_t direct_samples, bool with_scene = false);
 hipError_t launch_trace_wavefront(const PhxCameraParams &cam, uint32_t w, uint32_t h, uint32_t seed,
                                   uint32_t direct_samples, const SceneArgs &sc, uint32_t shadow_any_hit,
                                   phx_float4 *hp_pos, phx_float4 *hp_normal, phx_float4 *hp_throughput,
                                   phx_float4 *hp_base_color, phx_float4 *hp_direct, phx_float4 *hp_emission,
-                                  void *ws, hipStream_t s);
+                                  void *ws, hipStream_t s, const uint32_t *d_seed = nullptr);
 
 size_t emit_workspace_bytes(uint32_t photons_to_emit, uint32_t max_bounces);
 hipError_t launch_emit(const SceneArgs &sc, uint32_t max_photons, uint32_t offset, uint32_t photons_to_emit,
                        uint32_t max_bounces, uint32_t seed, phx_float4 *photon_pos, uint32_t *photon_power,
-                       uint32_t *photon_dir, uint32_t *n_photons, void *ws, hipStream_t s);
+                       uint32_t *photon_dir, uint32_t *n_photons, void *ws, hipStream_t s,
+                       const uint32_t *d_seed = nullptr);
 
 hipError_t launch_hash(const phx_float4 *pos, uint32_t n, const PhxPhotonHashData &d, uint32_t *hashes,
                        uint32_t *counts, hipStream_t s);
@@ -82,7 +90,7 @@ size_t sort_workspace_bytes(uint32_t n, uint32_t n_cells);
 hipError_t launch_sort(const phx_float4 *pos_in, const uint32_t *pow_in, const uint32_t *dir_in,
                        const uint32_t *hashes, const uint32_t *cell_start, uint32_t *cell_end, uint32_t n,
                        uint32_t n_cells, phx_float4 *pos_out, uint32_t *pow_out, uint32_t *dir_out, void *ws,
-                       hipStream_t s);
+                       hipStream_t s, const uint32_t *d_n = nullptr);
 
 hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const phx_float4 *hp_throughput,
                          const phx_float4 *hp_base_color, const phx_float4 *hp_direct, const phx_float4 *hp_emission,

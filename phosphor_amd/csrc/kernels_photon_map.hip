@@ -40,6 +40,34 @@ hipError_t launch_fill_u32(u32 *p, u32 v, size_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
+/* Round seeds of a pass (renderer.cpp:65, random_u32 of the host PCG), one
+ * per scene, drawn on the device so a pass needs no host input. */
+__global__ void k_next_round_seed(u32 *state, u32 *seed, u32 n) {
+    u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        seed[i] = random_u32(state[i]);
+}
+
+hipError_t launch_next_round_seed(u32 *state, u32 *seed, u32 n, hipStream_t s) {
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_next_round_seed, dim3((n + 63) / 64), dim3(64), 0, s, state, seed, n);
+    return hipGetLastError();
+}
+
+/* info[0] = the pass's deposit count, info[1] += 1 when it is zero */
+__global__ void k_note_count(const u32 *count, u32 *info) {
+    u32 c = *count;
+    info[0] = c;
+    if (c == 0)
+        info[1] += 1u;
+}
+
+hipError_t launch_note_count(const u32 *count, u32 *info, hipStream_t s) {
+    hipLaunchKernelGGL(k_note_count, dim3(1), dim3(1), 0, s, count, info);
+    return hipGetLastError();
+}
+
 /* ---------------- scan ---------------- */
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 16;
@@ -65,17 +93,28 @@ __device__ __forceinline__ u32 wave_sum(u32 x) {
     return x;
 }
 
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan(const u32 *__restrict__ in, u32 n, u32 *__restrict__ out,
-                                                      u32 *__restrict__ out2, u32 *__restrict__ total,
-                                                      unsigned long long *status, u32 *ticket, u32 n_tiles) {
+/* With d_n set, the length is min(*d_n, n) and the grid is sized for n:
+ * tiles past the device length take a ticket and leave (no tile waits on a
+ * later ticket). */
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan(const u32 *__restrict__ in, u32 n, const u32 *d_n,
+                                                      u32 *__restrict__ out, u32 *__restrict__ out2,
+                                                      u32 *__restrict__ total, unsigned long long *status,
+                                                      u32 *ticket) {
     __shared__ u32 s[SCAN_TILE + SCAN_TILE / 16];
     __shared__ u32 s_wave[SCAN_BLOCK / 64];
     __shared__ u32 s_tile, s_prefix;
     u32 t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    if (d_n) {
+        u32 c = *d_n;
+        n = c < n ? c : n;
+    }
+    u32 n_tiles = (u32)(((size_t)n + SCAN_TILE - 1) / SCAN_TILE);
     if (t == 0)
         s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     u32 tile = s_tile;
+    if (tile >= n_tiles)
+        return;
     u32 base = tile * SCAN_TILE;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; k++) {
@@ -177,7 +216,8 @@ size_t scan_workspace_bytes(u32 n) {
     return 256 + align_up(tiles * sizeof(unsigned long long), 256);
 }
 
-hipError_t launch_scan(const u32 *in, u32 n, u32 *out, u32 *out2, u32 *total, void *ws, hipStream_t s) {
+hipError_t launch_scan(const u32 *in, u32 n, u32 *out, u32 *out2, u32 *total, void *ws, hipStream_t s,
+                       const u32 *d_n) {
     u32 n_tiles = (u32)(((size_t)n + SCAN_TILE - 1) / SCAN_TILE);
     u32 *ticket = (u32 *)ws;
     unsigned long long *status = (unsigned long long *)((char *)ws + 256);
@@ -188,8 +228,7 @@ hipError_t launch_scan(const u32 *in, u32 n, u32 *out, u32 *out2, u32 *total, vo
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || n == 0)
         return e;
-    hipLaunchKernelGGL(k_scan, dim3(n_tiles), dim3(SCAN_BLOCK), 0, s, in, n, out, out2, total, status, ticket,
-                       n_tiles);
+    hipLaunchKernelGGL(k_scan, dim3(n_tiles), dim3(SCAN_BLOCK), 0, s, in, n, d_n, out, out2, total, status, ticket);
     return hipGetLastError();
 }
 
@@ -264,92 +303,119 @@ constexpr int SORT_ITEMS = 16;
 constexpr u32 SORT_TILE = SORT_BLOCK * SORT_ITEMS; /* 4096 */
 constexpr int SORT_WAVES = SORT_BLOCK / 64;
 
-__global__ __launch_bounds__(SORT_BLOCK) void k_radix_hist(const u32 *__restrict__ keys, u32 n, u32 shift,
-                                                           u32 n_tiles, u32 *__restrict__ hist) {
+/* Item count of a sort: n, or min(*d_n, n) when the count lives on the device
+ * (the grid is then sized for n and the kernels loop over the tiles that
+ * exist). */
+__device__ __forceinline__ u32 sort_count(u32 n, const u32 *d_n) {
+    if (!d_n)
+        return n;
+    u32 c = *d_n;
+    return c < n ? c : n;
+}
+
+__global__ __launch_bounds__(SORT_BLOCK) void k_radix_hist(const u32 *__restrict__ keys, u32 n, const u32 *d_n,
+                                                           u32 shift, u32 *__restrict__ hist) {
     __shared__ u32 h[RADIX];
-    u32 t = threadIdx.x, tile = blockIdx.x;
-    h[t] = 0u;
-    __syncthreads();
+    u32 t = threadIdx.x;
+    n = sort_count(n, d_n);
+    u32 n_tiles = (u32)(((size_t)n + SORT_TILE - 1) / SORT_TILE);
+    for (u32 tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        h[t] = 0u;
+        __syncthreads();
 #pragma unroll
-    for (int k = 0; k < SORT_ITEMS; k++) {
-        u32 e = tile * SORT_TILE + (u32)k * SORT_BLOCK + t;
-        if (e < n)
-            atomicAdd(&h[(keys[e] >> shift) & (RADIX - 1)], 1u);
+        for (int k = 0; k < SORT_ITEMS; k++) {
+            u32 e = tile * SORT_TILE + (u32)k * SORT_BLOCK + t;
+            if (e < n)
+                atomicAdd(&h[(keys[e] >> shift) & (RADIX - 1)], 1u);
+        }
+        __syncthreads();
+        hist[(size_t)t * n_tiles + tile] = h[t];
+        __syncthreads();
     }
-    __syncthreads();
-    hist[(size_t)t * n_tiles + tile] = h[t];
 }
 
 /* Element order inside a tile is (wave, item, lane): element
  * tile·4096 + wave·1024 + item·64 + lane, so ranks computed item by item
  * within a wave, then offset by the counts of earlier waves, are stable. */
 __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(
-    const u32 *__restrict__ keys_in, const u32 *__restrict__ vals_in, u32 n, u32 shift, u32 n_tiles,
+    const u32 *__restrict__ keys_in, const u32 *__restrict__ vals_in, u32 n, const u32 *d_n, u32 shift,
     const u32 *__restrict__ offs, u32 *__restrict__ keys_out, u32 *__restrict__ vals_out, int final_pass,
     const phx_float4 *__restrict__ pos_in, const u32 *__restrict__ pow_in, const u32 *__restrict__ dir_in,
     phx_float4 *__restrict__ pos_out, u32 *__restrict__ pow_out, u32 *__restrict__ dir_out) {
     __shared__ u32 cnt[SORT_WAVES][RADIX];
-    u32 t = threadIdx.x, lane = t & 63u, w = t >> 6, tile = blockIdx.x;
-    for (u32 i = t; i < SORT_WAVES * RADIX; i += SORT_BLOCK)
-        (&cnt[0][0])[i] = 0u;
-    __syncthreads();
+    u32 t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    n = sort_count(n, d_n);
+    u32 n_tiles = (u32)(((size_t)n + SORT_TILE - 1) / SORT_TILE);
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
-    u32 key[SORT_ITEMS], val[SORT_ITEMS], rank[SORT_ITEMS];
-    u32 ebase = tile * SORT_TILE + w * (SORT_ITEMS * 64);
+    for (u32 tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        for (u32 i = t; i < SORT_WAVES * RADIX; i += SORT_BLOCK)
+            (&cnt[0][0])[i] = 0u;
+        __syncthreads();
+        u32 key[SORT_ITEMS], val[SORT_ITEMS], rank[SORT_ITEMS];
+        u32 ebase = tile * SORT_TILE + w * (SORT_ITEMS * 64);
 #pragma unroll
-    for (int k = 0; k < SORT_ITEMS; k++) {
-        u32 e = ebase + (u32)k * 64 + lane;
-        bool valid = e < n;
-        key[k] = valid ? keys_in[e] : 0u;
-        val[k] = valid ? (vals_in ? vals_in[e] : e) : 0u;
-        u32 d = (key[k] >> shift) & (RADIX - 1);
-        unsigned long long peers = __ballot(valid);
+        for (int k = 0; k < SORT_ITEMS; k++) {
+            u32 e = ebase + (u32)k * 64 + lane;
+            bool valid = e < n;
+            key[k] = valid ? keys_in[e] : 0u;
+            val[k] = valid ? (vals_in ? vals_in[e] : e) : 0u;
+            u32 d = (key[k] >> shift) & (RADIX - 1);
+            unsigned long long peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < RADIX_BITS; b++) {
-            bool bit = (d >> b) & 1u;
-            unsigned long long m = __ballot(bit);
-            peers &= bit ? m : ~m;
+            for (int b = 0; b < RADIX_BITS; b++) {
+                bool bit = (d >> b) & 1u;
+                unsigned long long m = __ballot(bit);
+                peers &= bit ? m : ~m;
+            }
+            u32 below = (u32)__popcll(peers & lt_mask);
+            u32 c = cnt[w][d];
+            rank[k] = c + below;
+            if (valid && below == 0)
+                cnt[w][d] = c + (u32)__popcll(peers);
         }
-        u32 below = (u32)__popcll(peers & lt_mask);
-        u32 c = cnt[w][d];
-        rank[k] = c + below;
-        if (valid && below == 0)
-            cnt[w][d] = c + (u32)__popcll(peers);
-    }
-    __syncthreads();
-    {
-        /* cnt[w][d] ← global offset of digit d for this tile + counts of earlier waves */
-        u32 d = t;
-        u32 run = offs[(size_t)d * n_tiles + tile];
+        __syncthreads();
+        {
+            /* cnt[w][d] ← global offset of digit d for this tile + counts of earlier waves */
+            u32 d = t;
+            u32 run = offs[(size_t)d * n_tiles + tile];
 #pragma unroll
-        for (int ww = 0; ww < SORT_WAVES; ww++) {
-            u32 c = cnt[ww][d];
-            cnt[ww][d] = run;
-            run += c;
+            for (int ww = 0; ww < SORT_WAVES; ww++) {
+                u32 c = cnt[ww][d];
+                cnt[ww][d] = run;
+                run += c;
+            }
         }
-    }
-    __syncthreads();
+        __syncthreads();
 #pragma unroll
-    for (int k = 0; k < SORT_ITEMS; k++) {
-        u32 e = ebase + (u32)k * 64 + lane;
-        if (e >= n)
-            continue;
-        u32 d = (key[k] >> shift) & (RADIX - 1);
-        u32 dst = cnt[w][d] + rank[k];
-        if (final_pass) {
-            u32 src = val[k];
-            pos_out[dst] = pos_in[src];
-            pow_out[dst] = pow_in[src];
-            dir_out[dst] = dir_in[src];
-        } else {
-            keys_out[dst] = key[k];
-            vals_out[dst] = val[k];
+        for (int k = 0; k < SORT_ITEMS; k++) {
+            u32 e = ebase + (u32)k * 64 + lane;
+            if (e >= n)
+                continue;
+            u32 d = (key[k] >> shift) & (RADIX - 1);
+            u32 dst = cnt[w][d] + rank[k];
+            if (final_pass) {
+                u32 src = val[k];
+                pos_out[dst] = pos_in[src];
+                pow_out[dst] = pow_in[src];
+                dir_out[dst] = dir_in[src];
+            } else {
+                keys_out[dst] = key[k];
+                vals_out[dst] = val[k];
+            }
         }
+        __syncthreads();
     }
 }
 
+/* 256 digits × n_tiles histogram entries, the length of the offset scan */
+__global__ void k_sort_hist_len(u32 n, const u32 *d_n, u32 *len) {
+    n = sort_count(n, d_n);
+    *len = (u32)(((size_t)n + SORT_TILE - 1) / SORT_TILE) * RADIX;
+}
+
 __global__ __launch_bounds__(256) void k_cell_end(const u32 *__restrict__ cell_start, u32 *__restrict__ cell_end,
-                                                  u32 n_cells, u32 n) {
+                                                  u32 n_cells, u32 n, const u32 *d_n) {
+    n = sort_count(n, d_n);
     for (u32 h = blockIdx.x * 256 + threadIdx.x; h < n_cells; h += gridDim.x * 256)
         cell_end[h] = (h + 1 < n_cells) ? cell_start[h + 1] : n;
 }
@@ -365,20 +431,29 @@ static u32 key_bits(u32 n_cells) {
 size_t sort_workspace_bytes(u32 n, u32 n_cells) {
     (void)n_cells;
     size_t tiles = ((size_t)n + SORT_TILE - 1) / SORT_TILE;
-    size_t b = 0;
+    size_t b = 256;                                           /* device scan length */
     b += 4 * align_up((size_t)n * sizeof(u32), 256);        /* keys/vals ping-pong */
     b += 2 * align_up(tiles * RADIX * sizeof(u32), 256);     /* hist, offs */
     b += align_up(scan_workspace_bytes((u32)(tiles * RADIX)), 256);
     return b;
 }
 
+/* Persistent grid for a device-side count: enough blocks to fill the GPU,
+ * each looping over tiles. */
+constexpr u32 SORT_DEV_BLOCKS = 2048;
+
 hipError_t launch_sort(const phx_float4 *pos_in, const u32 *pow_in, const u32 *dir_in, const u32 *hashes,
                        const u32 *cell_start, u32 *cell_end, u32 n, u32 n_cells, phx_float4 *pos_out, u32 *pow_out,
-                       u32 *dir_out, void *ws, hipStream_t s) {
+                       u32 *dir_out, void *ws, hipStream_t s, const u32 *d_n) {
     hipError_t e;
     if (n > 0) {
         u32 n_tiles = (u32)(((size_t)n + SORT_TILE - 1) / SORT_TILE);
+        u32 grid = n_tiles;
+        if (d_n && grid > SORT_DEV_BLOCKS)
+            grid = SORT_DEV_BLOCKS;
         char *p = (char *)ws;
+        u32 *hist_len = (u32 *)p;
+        p += 256;
         u32 *kbuf[2], *vbuf[2];
         kbuf[0] = (u32 *)p;
         p += align_up((size_t)n * sizeof(u32), 256);
@@ -393,20 +468,26 @@ hipError_t launch_sort(const phx_float4 *pos_in, const u32 *pow_in, const u32 *d
         u32 *offs = (u32 *)p;
         p += align_up((size_t)n_tiles * RADIX * sizeof(u32), 256);
         void *scan_ws = p;
+        if (d_n) {
+            hipLaunchKernelGGL(k_sort_hist_len, dim3(1), dim3(1), 0, s, n, d_n, hist_len);
+            if ((e = hipGetLastError()) != hipSuccess)
+                return e;
+        }
         u32 passes = (key_bits(n_cells) + RADIX_BITS - 1) / RADIX_BITS;
         const u32 *kin = hashes;
         const u32 *vin = nullptr;
         for (u32 pass = 0; pass < passes; pass++) {
             u32 shift = pass * RADIX_BITS;
             bool last = pass + 1 == passes;
-            hipLaunchKernelGGL(k_radix_hist, dim3(n_tiles), dim3(SORT_BLOCK), 0, s, kin, n, shift, n_tiles, hist);
+            hipLaunchKernelGGL(k_radix_hist, dim3(grid), dim3(SORT_BLOCK), 0, s, kin, n, d_n, shift, hist);
             if ((e = hipGetLastError()) != hipSuccess)
                 return e;
-            if ((e = launch_scan(hist, n_tiles * RADIX, offs, nullptr, nullptr, scan_ws, s)) != hipSuccess)
+            if ((e = launch_scan(hist, n_tiles * RADIX, offs, nullptr, nullptr, scan_ws, s,
+                                 d_n ? hist_len : nullptr)) != hipSuccess)
                 return e;
             u32 *kout = kbuf[pass & 1], *vout = vbuf[pass & 1];
-            hipLaunchKernelGGL(k_radix_scatter, dim3(n_tiles), dim3(SORT_BLOCK), 0, s, kin, vin, n, shift, n_tiles,
-                               offs, kout, vout, last ? 1 : 0, pos_in, pow_in, dir_in, pos_out, pow_out, dir_out);
+            hipLaunchKernelGGL(k_radix_scatter, dim3(grid), dim3(SORT_BLOCK), 0, s, kin, vin, n, d_n, shift, offs,
+                               kout, vout, last ? 1 : 0, pos_in, pow_in, dir_in, pos_out, pow_out, dir_out);
             if ((e = hipGetLastError()) != hipSuccess)
                 return e;
             kin = kout;
@@ -417,7 +498,7 @@ hipError_t launch_sort(const phx_float4 *pos_in, const u32 *pow_in, const u32 *d
         u32 blocks = (n_cells + 255) / 256;
         if (blocks > 4096)
             blocks = 4096;
-        hipLaunchKernelGGL(k_cell_end, dim3(blocks), dim3(256), 0, s, cell_start, cell_end, n_cells, n);
+        hipLaunchKernelGGL(k_cell_end, dim3(blocks), dim3(256), 0, s, cell_start, cell_end, n_cells, n, d_n);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
     }

@@ -9,9 +9,14 @@
  * library on one HIP stream.
  *
  * Differences from the reference, all internal: the scan is one kernel, the
- * scatter is the stable sort (canonical photon order), and the deposit count
- * is read back once per pass exactly where the reference reads it
- * (renderer.cpp:113-118).
+ * scatter is the stable sort (canonical photon order), and a pass never waits
+ * for the host.  The round seed is drawn on the device from a PCG state kept
+ * there, and the deposit count that the reference reads back between emit and
+ * hash (renderer.cpp:113-118) stays on the device: hash, sort and cell-end
+ * kernels read it.  The count is copied to pinned host memory at the end of
+ * the pass; a pass without deposits ("no photon hit", renderer.cpp:117-118)
+ * is reported by the next synchronisation.  With PHX_PASS_GRAPH the pass is
+ * captured once into a hipGraph and replayed.
  */
 #include <algorithm>
 #include <cmath>
@@ -140,6 +145,10 @@ struct PhxRenderer {
     PhxRenderArgs args{};
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {};
+    hipEvent_t ev_info = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    uint32_t *h_info = nullptr; /* pinned: last deposit count, passes without deposits */
     RenderingBuffers buffers;
     PhxPhotonHashData photon_hash_info{};
     uint32_t bucket_count = 0;
@@ -147,27 +156,42 @@ struct PhxRenderer {
     /* Renderer's own unsorted photon buffers (renderer.hpp:48-51) */
     DevBuf d_photon_pos, d_photon_power, d_photon_dir, d_batch_size;
     DevBuf d_emit_ws, d_scan_ws, d_sort_ws, d_trace_ws;
+    DevBuf d_rng;  /* [0] PCG state of the round seeds, [1] the current round seed */
+    DevBuf d_info; /* [0] last deposit count, [1] passes without deposits */
     uint32_t shadow_any_hit = 0;
-    uint32_t rng = 0;
     uint32_t rounds = 0;
     uint64_t total_photons_emitted = 0;
-    uint32_t last_count = 0;
+    uint32_t zero_reported = 0;
     bool timed = false;
+    bool info_pending = false;
 
     ~PhxRenderer() {
         if (stream)
             (void)hipStreamSynchronize(stream);
+        if (graph_exec)
+            (void)hipGraphExecDestroy(graph_exec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
         for (hipEvent_t &e : ev)
             if (e)
                 (void)hipEventDestroy(e);
+        if (ev_info)
+            (void)hipEventDestroy(ev_info);
+        if (h_info)
+            (void)hipHostFree(h_info);
         if (stream)
             (void)hipStreamDestroy(stream);
     }
 
+    uint32_t *seed_ptr() const { return d_rng.as<uint32_t>() + 1; }
     int set_scene(const PhxSceneDesc &scene);
     int run_pass();
-    int trace_rays(uint32_t seed);
-    int emit_photons(uint32_t seed);
+    int enqueue_pass(bool events);
+    int capture_pass();
+    int collect();
+    int check_passes();
+    int trace_rays();
+    int emit_photons();
     int hash_and_sort_photons();
     int gather_flux();
 };
@@ -178,12 +202,6 @@ static uint32_t host_pcg_seed(uint32_t seed) { /* random.h:16-24 */
     s += seed;
     s = s * 747796405u + 2891336453u;
     return s;
-}
-
-static uint32_t host_random_u32(uint32_t &state) { /* random.h:29-34 */
-    uint32_t old = state;
-    state = state * 747796405u + 2891336453u;
-    return ((old >> ((old >> 28u) + 4u)) ^ old) * 277803737u;
 }
 
 #define PHX_TRY(expr, what)                                                                                            \
@@ -349,67 +367,68 @@ int PhxRenderer::set_scene(const PhxSceneDesc &sc) {
         PHX_TRY(d_trace_ws.alloc(trace_wavefront_workspace_bytes(im.n_pixels, a.direct_samples)),
                 "alloc trace workspace");
 
-    rng = host_pcg_seed(a.seed);
+    /* renderer.cpp:54: the round-seed generator, kept on the device */
+    uint32_t rng0[2] = {host_pcg_seed(a.seed), 0u};
+    PHX_TRY(d_rng.alloc(sizeof(rng0)), "alloc seeds");
+    PHX_TRY(hipMemcpyAsync(d_rng.p, rng0, sizeof(rng0), hipMemcpyHostToDevice, s), "upload seeds");
+    PHX_TRY(d_info.alloc(2 * sizeof(uint32_t)), "alloc pass info");
+    PHX_TRY(hipMemsetAsync(d_info.p, 0, d_info.bytes, s), "clear pass info");
     PHX_TRY(hipStreamSynchronize(s), "set_scene");
     return PHX_OK;
 }
 
 /* renderer.cpp:90-95 */
-int PhxRenderer::trace_rays(uint32_t seed) {
+int PhxRenderer::trace_rays() {
     const ImageBuffers &i = buffers.image_buf;
     if (args.trace_mode == 0)
-        return hip_status(launch_trace_wavefront(i.camera, i.image_width, i.image_height, seed, args.direct_samples,
+        return hip_status(launch_trace_wavefront(i.camera, i.image_width, i.image_height, 0u, args.direct_samples,
                                                  buffers.scene_args(), shadow_any_hit, i.d_hp_pos.as<phx_float4>(),
                                                  i.d_hp_normal.as<phx_float4>(), i.d_hp_throughput.as<phx_float4>(),
                                                  i.d_hp_base_color.as<phx_float4>(), i.d_hp_direct.as<phx_float4>(),
-                                                 i.d_hp_emission.as<phx_float4>(), d_trace_ws.p, stream),
+                                                 i.d_hp_emission.as<phx_float4>(), d_trace_ws.p, stream, seed_ptr()),
                           "trace_rays");
-    return hip_status(launch_trace(i.camera, i.image_width, i.image_height, seed, args.direct_samples,
+    return hip_status(launch_trace(i.camera, i.image_width, i.image_height, 0u, args.direct_samples,
                                    buffers.scene_args(), i.d_hp_pos.as<phx_float4>(), i.d_hp_normal.as<phx_float4>(),
                                    i.d_hp_throughput.as<phx_float4>(), i.d_hp_base_color.as<phx_float4>(),
-                                   i.d_hp_direct.as<phx_float4>(), i.d_hp_emission.as<phx_float4>(), stream),
+                                   i.d_hp_direct.as<phx_float4>(), i.d_hp_emission.as<phx_float4>(), stream,
+                                   seed_ptr()),
                       "trace_rays");
 }
 
-/* renderer.cpp:97-121 */
-int PhxRenderer::emit_photons(uint32_t seed) {
+/* renderer.cpp:97-121; the count stays on the device (clamped to the buffer
+ * by its readers, as renderer.cpp:115 clamps it) */
+int PhxRenderer::emit_photons() {
     PHX_TRY(launch_fill_u32(d_batch_size.as<uint32_t>(), 0u, 1, stream), "emit_photons: reset counter");
     uint32_t ppr = args.photons_per_round;
     uint32_t per_batch = std::min(ppr, args.max_photons_per_batch ? args.max_photons_per_batch : 1u);
     SceneArgs sc = buffers.scene_args();
     for (uint32_t off = 0; off < ppr; off += per_batch) {
         uint32_t to_emit = std::min(per_batch, ppr - off);
-        PHX_TRY(launch_emit(sc, photon_buf_size, off, to_emit, args.max_photon_bounces, seed,
+        PHX_TRY(launch_emit(sc, photon_buf_size, off, to_emit, args.max_photon_bounces, 0u,
                             d_photon_pos.as<phx_float4>(), d_photon_power.as<uint32_t>(), d_photon_dir.as<uint32_t>(),
-                            d_batch_size.as<uint32_t>(), d_emit_ws.p, stream),
+                            d_batch_size.as<uint32_t>(), d_emit_ws.p, stream, seed_ptr()),
                 "emit_photons");
     }
-    uint32_t count = 0;
-    PHX_TRY(hipMemcpyAsync(&count, d_batch_size.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream), "emit readback");
-    PHX_TRY(hipStreamSynchronize(stream), "emit readback");
-    last_count = count;
-    uint32_t n = std::min(count, photon_buf_size);
-    if (n == 0)
-        return fail(PHX_ERR_NO_PHOTONS, "no photon hit"); /* renderer.cpp:117-118 */
-    buffers.photon_buf.n_photons = n;
+    PHX_TRY(launch_note_count(d_batch_size.as<uint32_t>(), d_info.as<uint32_t>(), stream), "emit_photons: count");
     return PHX_OK;
 }
 
-/* renderer.cpp:123-153 */
+/* renderer.cpp:123-153, sized by the device-side count */
 int PhxRenderer::hash_and_sort_photons() {
     PhotonBuffers &p = buffers.photon_buf;
-    uint32_t n = p.n_photons;
+    const uint32_t *d_n = d_batch_size.as<uint32_t>();
+    uint32_t n_max = photon_buf_size;
     PHX_TRY(launch_fill_u32(p.d_cell_counts.as<uint32_t>(), 0u, bucket_count, stream), "clear cell counts");
-    PHX_TRY(launch_hash(d_photon_pos.as<phx_float4>(), n, photon_hash_info, p.d_photon_hashes.as<uint32_t>(),
-                        p.d_cell_counts.as<uint32_t>(), stream),
+    PHX_TRY(launch_hash_dev(d_photon_pos.as<phx_float4>(), d_n, n_max, photon_hash_info,
+                            p.d_photon_hashes.as<uint32_t>(), p.d_cell_counts.as<uint32_t>(), stream),
             "hash_photons");
     PHX_TRY(launch_scan(p.d_cell_counts.as<uint32_t>(), bucket_count, p.d_cell_start.as<uint32_t>(),
                         p.d_cell_end.as<uint32_t>(), nullptr, d_scan_ws.p, stream),
             "scan_cells");
     PHX_TRY(launch_sort(d_photon_pos.as<phx_float4>(), d_photon_power.as<uint32_t>(), d_photon_dir.as<uint32_t>(),
                         p.d_photon_hashes.as<uint32_t>(), p.d_cell_start.as<uint32_t>(),
-                        p.d_cell_end.as<uint32_t>(), n, bucket_count, p.d_photon_pos.as<phx_float4>(),
-                        p.d_photon_power.as<uint32_t>(), p.d_photon_dir.as<uint32_t>(), d_sort_ws.p, stream),
+                        p.d_cell_end.as<uint32_t>(), n_max, bucket_count, p.d_photon_pos.as<phx_float4>(),
+                        p.d_photon_power.as<uint32_t>(), p.d_photon_dir.as<uint32_t>(), d_sort_ws.p, stream, d_n),
             "sort_photons");
     return PHX_OK;
 }
@@ -429,26 +448,84 @@ int PhxRenderer::gather_flux() {
                       "gather_flux");
 }
 
-/* renderer.cpp:64-78, one round */
-int PhxRenderer::run_pass() {
-    uint32_t round_seed = host_random_u32(rng);
+/* renderer.cpp:64-78, one round, enqueued without host synchronisation;
+ * stage events are recorded between the stages when `events` is set */
+int PhxRenderer::enqueue_pass(bool events) {
     int rc;
-    PHX_TRY(hipEventRecord(ev[0], stream), "event");
-    if ((rc = trace_rays(round_seed)) != PHX_OK)
+    PHX_TRY(launch_next_round_seed(d_rng.as<uint32_t>(), seed_ptr(), 1, stream), "round seed");
+    if ((rc = trace_rays()) != PHX_OK)
         return rc;
-    PHX_TRY(hipEventRecord(ev[1], stream), "event");
-    if ((rc = emit_photons(round_seed)) != PHX_OK)
+    if (events)
+        PHX_TRY(hipEventRecord(ev[1], stream), "event");
+    if ((rc = emit_photons()) != PHX_OK)
         return rc;
-    PHX_TRY(hipEventRecord(ev[2], stream), "event");
+    if (events)
+        PHX_TRY(hipEventRecord(ev[2], stream), "event");
     if ((rc = hash_and_sort_photons()) != PHX_OK)
         return rc;
-    PHX_TRY(hipEventRecord(ev[3], stream), "event");
-    if ((rc = gather_flux()) != PHX_OK)
+    if (events)
+        PHX_TRY(hipEventRecord(ev[3], stream), "event");
+    return gather_flux();
+}
+
+/* records one pass into a graph (kernel nodes only: counters and flags are
+ * cleared by fill kernels, never by memset nodes) */
+int PhxRenderer::capture_pass() {
+    PHX_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "graph capture");
+    int rc = enqueue_pass(false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(stream, &g);
+    if (rc != PHX_OK) {
+        if (g)
+            (void)hipGraphDestroy(g);
         return rc;
+    }
+    if (e != hipSuccess)
+        return hip_status(e, "graph capture");
+    graph = g;
+    return hip_status(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0), "graph instantiate");
+}
+
+int PhxRenderer::run_pass() {
+    int rc;
+    bool use_graph = (args.flags & PHX_PASS_GRAPH) != 0;
+    if (use_graph && !graph_exec && (rc = capture_pass()) != PHX_OK)
+        return rc;
+    PHX_TRY(hipEventRecord(ev[0], stream), "event");
+    if (use_graph) {
+        PHX_TRY(hipGraphLaunch(graph_exec, stream), "graph launch");
+    } else if ((rc = enqueue_pass(true)) != PHX_OK) {
+        return rc;
+    }
     PHX_TRY(hipEventRecord(ev[4], stream), "event");
-    timed = true;
+    PHX_TRY(hipMemcpyAsync(h_info, d_info.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream), "pass info");
+    PHX_TRY(hipEventRecord(ev_info, stream), "event");
+    info_pending = true;
+    timed = !use_graph;
     rounds++;
     total_photons_emitted += args.photons_per_round;
+    return PHX_OK;
+}
+
+/* waits for the last pass's deposit count */
+int PhxRenderer::collect() {
+    if (!info_pending)
+        return PHX_OK;
+    PHX_TRY(hipEventSynchronize(ev_info), "pass info");
+    info_pending = false;
+    buffers.photon_buf.n_photons = std::min(h_info[0], photon_buf_size);
+    return PHX_OK;
+}
+
+/* reports, once, passes that deposited nothing (renderer.cpp:117-118) */
+int PhxRenderer::check_passes() {
+    int rc = collect();
+    if (rc != PHX_OK)
+        return rc;
+    if (h_info[1] > zero_reported) {
+        zero_reported = h_info[1];
+        return fail(PHX_ERR_NO_PHOTONS, "no photon hit");
+    }
     return PHX_OK;
 }
 
@@ -469,6 +546,8 @@ int phx_renderer_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, Ph
         return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: max_bounces outside 1..32 (cmd_args.hpp:49)");
     if (args->trace_mode > 1)
         return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: trace_mode must be 0 or 1");
+    if (args->flags & ~(uint32_t)PHX_PASS_GRAPH)
+        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: unknown flags");
     if (args->direct_samples == 0 || args->direct_samples > 128)
         return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: direct_samples outside 1..128 (cmd_args.hpp:59)");
     int dev_count = 0;
@@ -482,6 +561,11 @@ int phx_renderer_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, Ph
     for (hipEvent_t &ev : r->ev)
         if ((e = hipEventCreate(&ev)) != hipSuccess)
             return hip_status(e, "phx_renderer_create: event");
+    if ((e = hipEventCreateWithFlags(&r->ev_info, hipEventDisableTiming)) != hipSuccess)
+        return hip_status(e, "phx_renderer_create: event");
+    if ((e = hipHostMalloc((void **)&r->h_info, 2 * sizeof(uint32_t), hipHostMallocDefault)) != hipSuccess)
+        return hip_status(e, "phx_renderer_create: pinned info");
+    r->h_info[0] = r->h_info[1] = 0;
     int rc = r->set_scene(*scene);
     if (rc != PHX_OK)
         return rc;
@@ -500,7 +584,18 @@ int phx_renderer_run_pass(PhxRenderer *r) {
 int phx_renderer_synchronize(PhxRenderer *r) {
     if (!r)
         return fail(PHX_ERR_INVALID_ARG, "null renderer");
-    return hip_status(hipStreamSynchronize(r->stream), "phx_renderer_synchronize");
+    PHX_TRY(hipStreamSynchronize(r->stream), "phx_renderer_synchronize");
+    return r->check_passes();
+}
+
+int phx_renderer_last_pass_ms(PhxRenderer *r, float *out_ms) {
+    if (!r || !out_ms)
+        return fail(PHX_ERR_INVALID_ARG, "null argument");
+    *out_ms = 0.0f;
+    if (r->rounds == 0)
+        return PHX_OK;
+    PHX_TRY(hipEventSynchronize(r->ev[4]), "pass timing");
+    return hip_status(hipEventElapsedTime(out_ms, r->ev[0], r->ev[4]), "pass timing");
 }
 
 int phx_renderer_last_stage_ms(PhxRenderer *r, float out_ms[4]) {
@@ -520,7 +615,12 @@ int phx_renderer_last_stage_ms(PhxRenderer *r, float out_ms[4]) {
     return PHX_OK;
 }
 
-uint32_t phx_renderer_last_photon_count(PhxRenderer *r) { return r ? r->last_count : 0u; }
+uint32_t phx_renderer_last_photon_count(PhxRenderer *r) {
+    if (!r)
+        return 0u;
+    (void)r->collect();
+    return r->rounds ? r->h_info[0] : 0u;
+}
 uint32_t phx_renderer_rounds_done(PhxRenderer *r) { return r ? r->rounds : 0u; }
 
 int phx_renderer_read_state(PhxRenderer *r, phx_float4 *flux, float *photon_weight, phx_float4 *radiance) {
@@ -562,6 +662,7 @@ int phx_renderer_read_photon_map(PhxRenderer *r, phx_float4 *pos, uint32_t *powe
                                  uint32_t *cell_start, uint32_t *cell_end) {
     if (!r)
         return fail(PHX_ERR_INVALID_ARG, "null renderer");
+    (void)r->collect();
     const PhotonBuffers &p = r->buffers.photon_buf;
     size_t n = p.n_photons, nb = r->bucket_count;
     hipStream_t s = r->stream;
@@ -593,6 +694,76 @@ int phx_renderer_device_buffers(PhxRenderer *r, void **ptrs, uint32_t n_ptrs) {
     uint32_t total = (uint32_t)(sizeof(all) / sizeof(all[0]));
     for (uint32_t k = 0; k < n_ptrs && k < total; k++)
         ptrs[k] = all[k];
+    return PHX_OK;
+}
+
+/* Checkpoint of the progressive state: header, flux, photon_weight,
+ * radiance.  Resuming from it continues the same sequence of round seeds. */
+struct PhxCheckpointHeader {
+    uint32_t magic, version, width, height;
+    uint32_t rounds, rng_state, photons_per_round, reserved;
+    uint64_t total_photons_emitted;
+};
+static const uint32_t PHX_CKPT_MAGIC = 0x43584850u; /* "PHXC" */
+
+size_t phx_renderer_checkpoint_bytes(PhxRenderer *r) {
+    if (!r)
+        return 0;
+    size_t np = r->buffers.image_buf.n_pixels;
+    return sizeof(PhxCheckpointHeader) + np * (2 * sizeof(phx_float4) + sizeof(float));
+}
+
+int phx_renderer_save_checkpoint(PhxRenderer *r, void *buf, size_t bytes) {
+    if (!r || !buf)
+        return fail(PHX_ERR_INVALID_ARG, "null argument");
+    if (bytes < phx_renderer_checkpoint_bytes(r))
+        return fail(PHX_ERR_WORKSPACE, "phx_renderer_save_checkpoint: buffer too small");
+    PHX_TRY(hipStreamSynchronize(r->stream), "checkpoint");
+    const ImageBuffers &i = r->buffers.image_buf;
+    size_t np = i.n_pixels;
+    PhxCheckpointHeader h{};
+    h.magic = PHX_CKPT_MAGIC;
+    h.version = 1;
+    h.width = i.image_width;
+    h.height = i.image_height;
+    h.rounds = r->rounds;
+    h.photons_per_round = r->args.photons_per_round;
+    h.total_photons_emitted = r->total_photons_emitted;
+    PHX_TRY(hipMemcpy(&h.rng_state, r->d_rng.p, sizeof(uint32_t), hipMemcpyDeviceToHost), "checkpoint");
+    char *p = (char *)buf;
+    std::memcpy(p, &h, sizeof(h));
+    p += sizeof(h);
+    PHX_TRY(hipMemcpy(p, i.d_flux.p, np * sizeof(phx_float4), hipMemcpyDeviceToHost), "checkpoint");
+    p += np * sizeof(phx_float4);
+    PHX_TRY(hipMemcpy(p, i.d_photon_weight.p, np * sizeof(float), hipMemcpyDeviceToHost), "checkpoint");
+    p += np * sizeof(float);
+    PHX_TRY(hipMemcpy(p, i.d_radiance.p, np * sizeof(phx_float4), hipMemcpyDeviceToHost), "checkpoint");
+    return PHX_OK;
+}
+
+int phx_renderer_load_checkpoint(PhxRenderer *r, const void *buf, size_t bytes) {
+    if (!r || !buf)
+        return fail(PHX_ERR_INVALID_ARG, "null argument");
+    if (bytes < sizeof(PhxCheckpointHeader))
+        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_load_checkpoint: truncated checkpoint");
+    PhxCheckpointHeader h;
+    std::memcpy(&h, buf, sizeof(h));
+    const ImageBuffers &i = r->buffers.image_buf;
+    size_t np = i.n_pixels;
+    if (h.magic != PHX_CKPT_MAGIC || h.version != 1)
+        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_load_checkpoint: not a checkpoint");
+    if (h.width != i.image_width || h.height != i.image_height || bytes < phx_renderer_checkpoint_bytes(r))
+        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_load_checkpoint: image size differs");
+    PHX_TRY(hipStreamSynchronize(r->stream), "checkpoint");
+    const char *p = (const char *)buf + sizeof(h);
+    PHX_TRY(hipMemcpy(i.d_flux.p, p, np * sizeof(phx_float4), hipMemcpyHostToDevice), "checkpoint");
+    p += np * sizeof(phx_float4);
+    PHX_TRY(hipMemcpy(i.d_photon_weight.p, p, np * sizeof(float), hipMemcpyHostToDevice), "checkpoint");
+    p += np * sizeof(float);
+    PHX_TRY(hipMemcpy(i.d_radiance.p, p, np * sizeof(phx_float4), hipMemcpyHostToDevice), "checkpoint");
+    PHX_TRY(hipMemcpy(r->d_rng.p, &h.rng_state, sizeof(uint32_t), hipMemcpyHostToDevice), "checkpoint");
+    r->rounds = h.rounds;
+    r->total_photons_emitted = h.total_photons_emitted;
     return PHX_OK;
 }
 

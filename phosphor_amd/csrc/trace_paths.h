@@ -1,0 +1,445 @@
+/*
+ * trace_paths.h — per-item bodies of the wavefront trace and the wavefront
+ * photon emission, shared by the one-scene kernels (kernels_trace.hip) and
+ * the multi-scene kernels (kernels_batch.hip), so both run the same code on
+ * every pixel and photon.
+ *
+ * Wavefront trace (same outputs as trace_rays.cl:14-169, bit for bit): the
+ * random draws of the NEE loops (light selection, emissive point) never
+ * depend on shadow visibility, and the Russian roulette after a specular NEE
+ * only reads throughput, so a camera path draws all light fragments of an NEE
+ * event and keeps walking.  Shadow rays go to a queue (the direct_samples
+ * rays of one event are contiguous: same origin, coherent directions) traced
+ * by a lean kernel; the resolve step then forms emission and direct exactly in
+ * the reference's order of additions:
+ *   emission = ((BLACK + a_0) + s_0) + a_1 + ... (+ envmap on a miss),
+ * a_k = throughput_k·emissive_k, s_k = throughput_k·(Σ_d pre_d·trans_d / n)
+ * for specular events, and direct = (throughput·occlusion)·(Σ / n) at the
+ * first diffuse event (trace_rays.cl:97-147).
+ *
+ * Wavefront emission (same deposits as emit_photons.cl:12-114): a photon's
+ * state (origin, direction, power, throughput, PCG state, IOR stack, deposit
+ * count) lives in HBM between bounces; its k-th deposit goes to staging slot
+ * [k][photon].
+ */
+#ifndef PHX_TRACE_PATHS_H
+#define PHX_TRACE_PATHS_H
+
+#include "phx_device.h"
+
+namespace phx {
+
+constexpr int WF_CAM_BLOCK = 128;
+constexpr int WF_SHADOW_BLOCK = 256;
+constexpr int EMIT_WF_BLOCK = 128;
+constexpr u32 EMIT_WF_BOUNCES = 3; /* bounce-synchronous launches before the finishing launch */
+constexpr u32 NEE_NONE = 0, NEE_DIFFUSE = 1, NEE_SPECULAR = 2;
+
+struct WfState {
+    phx_float4 *rec_a, *rec_thr; /* [pixel · 5 + bounce] */
+    u32 *rec_base;               /* first shadow slot of the NEE event of that bounce */
+    u32 *meta;                   /* per pixel: bits 0-2 records, 3-12 NEE kinds, 15 envmap term */
+    phx_float4 *env;             /* per pixel: throughput · envmap on a miss */
+    f32 *occl;                   /* per pixel: occlusion at the diffuse hit */
+    u32 *counter;                /* shadow slots handed out */
+    phx_float4 *sh_o;            /* origin.xyz, t_max in w */
+    phx_float4 *sh_d;            /* direction */
+    phx_float4 *sh_pre;          /* NEE weight; w = 0 valid, w = -1 no shadow ray (adds BLACK) */
+    phx_float4 *sh_tr;           /* shadow transmittance */
+    u32 *sh_scene;               /* scene of each slot (multi-scene queues only, else null) */
+    u32 cap;
+};
+
+/* hitpoint outputs written by the camera path */
+struct CamOut {
+    phx_float4 *hp_pos, *hp_normal, *hp_throughput, *hp_base_color;
+};
+
+/* One camera path of trace_rays.cl up to its first diffuse hit, with every
+ * NEE event's fragments drawn and its shadow rays queued.  `pix` is the pixel
+ * within its image (PCG stream and raster position), `out` its index in the
+ * hitpoint arrays, `wf` its index in the wavefront state.  Every lane of the
+ * wave must call this together (the queue slots are allocated wave-wide);
+ * lanes with live = false only take part in the allocation. */
+PHX_DEV void camera_path(const SceneView &sc, const PhxCameraParams &cam, u32 w, u32 h, u32 seed, u32 n, bool live,
+                         u32 pix, u32 out, u32 wf, u32 tag, const WfState &S, const CamOut &o, u32 *stk, f32 *s_ior,
+                         u32 block) {
+    u32 px = live ? pix % w : 0, py = live ? pix / w : 0;
+    u32 rng = pcg_seed(seed + pix);
+    F4 origin = ZERO, dir = ZERO;
+    if (live)
+        make_ray(cam, px, py, w, h, rng, origin, dir);
+    F4 throughput = WHITE;
+    IorStack ior{s_ior, block, 0u};
+    init_ior_stack(ior, AIR_IOR);
+    u32 depth = 0, kinds = 0, env_flag = 0;
+    bool diffuse = false;
+    size_t rbase = (size_t)wf * MAX_RAY_BOUNCES;
+    /* the loop runs while any lane of the wave is live, so the wave-wide slot
+       allocation below sees every lane */
+    bool active = live;
+    while (__any(active)) {
+        bool nee = false, spec = false;
+        F4 hit_pos = ZERO, sn = ZERO, bc = ZERO, f0 = ZERO, view = ZERO;
+        f32 rough = 0.0f;
+        Bsdf bs;
+        bs.dir = ZERO;
+        bs.throughput = ZERO;
+        bs.event = BSDF_TRANSMIT;
+        F4 hit_n = ZERO;
+        if (active) {
+            HitRecord rec;
+            bool hit = scene_intersect(sc, origin, dir, EPS, INF, rec, stk, block);
+            if (!hit) {
+                if (sc.has_env) {
+                    st(S.env + wf, throughput * sample_envmap(sc, dir));
+                    env_flag = 1;
+                }
+                active = false;
+            } else {
+                SurfaceHit sh = process_hit(sc, rec, origin, dir);
+                const PhxMaterial &mat = sc.materials[sh.mat_index];
+                f32 curr_ior = current_ior(ior);
+                if (curr_ior != AIR_IOR && mat.thickness > EPS && mat.att_dist > EPS)
+                    throughput = throughput * beer_lambert(from(mat.att_color), mat.att_dist, sh.t);
+                Ctx c = evaluate_material(sc, mat, sh.mat_index, rec, sh);
+                st(S.rec_a + rbase + depth, throughput * c.emissive);
+                view = -dir;
+                bs = sample_bsdf(rng, c, c.shading_normal, sh.normal, view, ior, sh.front_face);
+                hit_pos = sh.position;
+                hit_n = sh.normal;
+                sn = c.shading_normal;
+                bc = c.base_color;
+                rough = c.roughness;
+                if (bs.event == BSDF_DIFFUSE) {
+                    nee = true;
+                    diffuse = true;
+                    kinds |= NEE_DIFFUSE << (2 * depth);
+                    st(o.hp_pos + out, sh.position);
+                    st(o.hp_normal + out, sh.normal);
+                    st(o.hp_base_color + out, c.base_color);
+                    st(o.hp_throughput + out, throughput);
+                    S.occl[wf] = c.occlusion;
+                } else if (bs.event == BSDF_METALLIC || bs.event == BSDF_FRESNEL) {
+                    nee = true;
+                    spec = true;
+                    kinds |= NEE_SPECULAR << (2 * depth);
+                    if (bs.event == BSDF_METALLIC) {
+                        f0 = c.base_color;
+                    } else {
+                        f32 ratio = (curr_ior - c.ior) / (curr_ior + c.ior);
+                        f32 r0 = ratio * ratio;
+                        f0 = mk(r0, r0, r0, 0.0f);
+                    }
+                }
+                if (nee)
+                    st(S.rec_thr + rbase + depth, throughput);
+            }
+        }
+        /* one atomic per wave for the shadow slots of this bounce */
+        unsigned long long want = __ballot(nee);
+        u32 lane = threadIdx.x & 63u;
+        u32 rank = (u32)__popcll(want & ((1ull << lane) - 1ull));
+        u32 wave_base = 0;
+        if (want) {
+            u32 total = (u32)__popcll(want) * n;
+            u32 first = (u32)__ffsll((long long)want) - 1u;
+            if (lane == first)
+                wave_base = atomicAdd(S.counter, total);
+            wave_base = __shfl(wave_base, (int)first, 64);
+        }
+        if (nee) {
+            u32 base = wave_base + rank * n;
+            S.rec_base[rbase + depth] = base;
+            for (u32 d = 0; d < n; d++) {
+                LightFragment f;
+                bool valid = false;
+                F4 pre = BLACK;
+                if (!spec) {
+                    if (sample_light_fragment(rng, sc, hit_pos, sn, f)) {
+                        valid = true;
+                        pre = f.radiance * (bc / PI);
+                    }
+                } else {
+                    f32 ndv = dot(view, sn);
+                    /* direct_lighting_specular: each early return adds BLACK */
+                    if (!(ndv < EPS) && sample_light_fragment(rng, sc, hit_pos, sn, f) && !(f.n_dot_dir < EPS)) {
+                        f32 alpha = rough * rough;
+                        F4 hv = normalize(f.dir + view);
+                        f32 ndh = phx_fmaxf(dot(sn, hv), 0.0f);
+                        f32 ggx = ggx_ndf(ndh, alpha);
+                        F4 fr = fresnel4(f0, view, hv);
+                        f32 sm = smith_g1_ggx(f.n_dot_dir, alpha) * smith_g1_ggx(ndv, alpha);
+                        F4 spv = f.radiance * ggx * fr * sm / (4.0f * f.n_dot_dir * ndv);
+                        if (!(dot(spv, spv) < EPS * EPS)) {
+                            valid = true;
+                            pre = spv;
+                        }
+                    }
+                }
+                size_t slot = (size_t)base + d;
+                if (valid) {
+                    st(S.sh_o + slot, mk(hit_pos.x, hit_pos.y, hit_pos.z, f.dist - EPS));
+                    st(S.sh_d + slot, f.dir);
+                    st(S.sh_pre + slot, mk(pre.x, pre.y, pre.z, 0.0f));
+                    if (S.sh_scene)
+                        S.sh_scene[slot] = tag;
+                } else {
+                    st(S.sh_pre + slot, mk(0.0f, 0.0f, 0.0f, -1.0f));
+                }
+            }
+        }
+        if (diffuse)
+            active = false;
+        if (active) {
+            /* trace_rays.cl:149-160 */
+            throughput = throughput * bs.throughput;
+            bool killed = false;
+            if (depth >= RR_MIN_RAY_DEPTH) {
+                f32 q = phx_fmaxf(throughput.x, phx_fmaxf(throughput.y, throughput.z));
+                q = phx_clampf(q, RR_MIN_Q, RR_MAX_Q);
+                if (random_float(rng) > q)
+                    killed = true;
+                else
+                    throughput = throughput * (1.0f / q);
+            }
+            if (!killed) {
+                F4 side = (dot(bs.dir, hit_n) > 0.0f) ? hit_n : -hit_n;
+                origin = hit_pos + bs.dir * EPS + side * EPS;
+                dir = bs.dir;
+            }
+            depth++;
+            if (killed || depth >= MAX_RAY_BOUNCES)
+                active = false;
+        }
+    }
+    if (!live)
+        return;
+    if (!diffuse) {
+        st(o.hp_pos + out, ZERO);
+        st(o.hp_normal + out, ZERO);
+        st(o.hp_base_color + out, BLACK);
+        st(o.hp_throughput + out, throughput);
+    }
+    /* records exist for every bounce that hit: 0..depth at a diffuse end,
+       0..depth-1 after a miss (nothing recorded for the missing bounce), a
+       roulette kill or the bounce limit (depth already advanced) */
+    u32 nrec = diffuse ? depth + 1 : depth;
+    S.meta[wf] = nrec | (kinds << 3) | (env_flag << 15);
+}
+
+/* shadow_transmittance (light_sampling.h:302-333) of one queued shadow ray;
+ * ANY = any-hit traversal (exact when no material transmits and the stack
+ * never overflows). */
+template <bool ANY> PHX_DEV void shadow_one(const SceneView &sc, const WfState &S, u32 slot, u32 *stk) {
+    F4 pre = ld(S.sh_pre + slot);
+    if (pre.w != 0.0f)
+        return;
+    F4 ot = ld(S.sh_o + slot);
+    F4 o = mk(ot.x, ot.y, ot.z, 0.0f);
+    F4 d = ld(S.sh_d + slot);
+    f32 t_min = EPS, t_max = ot.w;
+    F4 trans = WHITE;
+    for (u32 depth = 0; depth < MAX_RAY_BOUNCES; depth++) {
+        HitRecord rec;
+        if (!traverse<ANY>(sc, o, d, t_min, t_max, rec, stk, WF_SHADOW_BLOCK))
+            break;
+        const PhxMaterial &mat = sc.materials[rec.mat_index];
+        if (ANY || mat.transmission < EPS) {
+            trans = BLACK;
+            break;
+        }
+        F4 bcol = from(mat.base_color);
+        if (mat.diff_index != NO_TEXTURE) {
+            SurfaceHit s2 = process_hit(sc, rec, o, d);
+            bcol = bcol * srgb_to_linear(sample_texture_uv(sc, rec.mat_index, mat.diff_index, s2.uvx, s2.uvy, 0));
+        }
+        if (mat.att_dist > EPS)
+            trans = trans * beer_lambert(from(mat.att_color), mat.att_dist, rec.t - t_min);
+        trans = trans * (bcol * mat.transmission);
+        if (dot(trans, trans) < EPS) {
+            trans = BLACK;
+            break;
+        }
+        t_min = rec.t + EPS;
+    }
+    st(S.sh_tr + slot, trans);
+}
+
+/* emission and direct of one pixel from its records and shadow results */
+PHX_DEV void resolve_pixel(const WfState &S, u32 n, u32 wf, u32 out, phx_float4 *hp_direct, phx_float4 *hp_emission) {
+    u32 meta = S.meta[wf];
+    u32 nrec = meta & 7u;
+    size_t rbase = (size_t)wf * MAX_RAY_BOUNCES;
+    F4 e = BLACK;
+    F4 direct = BLACK;
+    for (u32 k = 0; k < nrec; k++) {
+        e = e + ld(S.rec_a + rbase + k);
+        u32 kind = (meta >> (3 + 2 * k)) & 3u;
+        if (kind == NEE_NONE)
+            continue;
+        size_t base = S.rec_base[rbase + k];
+        F4 sum = BLACK;
+        for (u32 d = 0; d < n; d++) {
+            F4 pre = ld(S.sh_pre + base + d);
+            if (pre.w != 0.0f) {
+                sum = sum + BLACK;
+            } else {
+                sum = sum + pre * ld(S.sh_tr + base + d);
+            }
+        }
+        F4 thr = ld(S.rec_thr + rbase + k);
+        if (kind == NEE_SPECULAR) {
+            e = e + thr * (sum / (f32)n);
+        } else {
+            e = mk(e.x, e.y, e.z, 1.0f);
+            direct = thr * S.occl[wf] * (sum / (f32)n);
+        }
+    }
+    if ((meta >> 15) & 1u)
+        e = e + ld(S.env + wf);
+    st(hp_emission + out, e);
+    st(hp_direct + out, direct);
+}
+
+/* ---------------- photon emission ---------------- */
+struct EmitWfState {
+    phx_float4 *origin, *dir, *power, *thr;
+    u32 *rng, *ior_cur, *count;
+    f32 *ior; /* [depth][photon] */
+    u32 *queue[2];
+    u32 *qcount; /* [bounce] live photons entering bounce k */
+};
+
+/* deposit staging, [k][photon] with `stride` photons per row */
+struct EmitStaging {
+    phx_float4 *pos;
+    u32 *power, *dir;
+    u32 stride;
+};
+
+/* emit_photons.cl:37-47 for photon `index` of its scene, state slot li */
+PHX_DEV void emit_start_one(const SceneView &sc, u32 seed, u32 index, u32 li, const EmitWfState &S, u32 stride) {
+    u32 rng = pcg_seed(seed + index);
+    F4 origin, dir, power;
+    sample_light(rng, sc, origin, dir, power);
+    st(S.origin + li, origin);
+    st(S.dir + li, dir);
+    st(S.power + li, power);
+    st(S.thr + li, mk(1.0f, 1.0f, 1.0f, 0.0f));
+    S.rng[li] = rng;
+    S.ior_cur[li] = 0;
+    S.ior[li] = AIR_IOR;
+    S.count[li] = 0;
+    (void)stride;
+}
+
+/* One bounce of emit_photons.cl:49-113 on a photon's state; returns false
+ * when the path ends (escape, roulette or the bounce limit). */
+PHX_DEV bool emit_step(const SceneView &sc, const EmitStaging &g, u32 li, u32 depth, u32 max_bounces, F4 &origin,
+                       F4 &dir, F4 &power, F4 &throughput, u32 &rng, IorStack &ior, u32 &n, u32 *stk) {
+    HitRecord rec;
+    if (!scene_intersect(sc, origin, dir, EPS, INF, rec, stk, EMIT_WF_BLOCK))
+        return false;
+    SurfaceHit sh = process_hit(sc, rec, origin, dir);
+    const PhxMaterial &mat = sc.materials[sh.mat_index];
+    f32 curr_ior = current_ior(ior);
+    if (curr_ior != AIR_IOR && mat.thickness > EPS && mat.att_dist > EPS) {
+        F4 vol = beer_lambert(from(mat.att_color), mat.att_dist, sh.t);
+        power = power * vol;
+        throughput = throughput * vol;
+    }
+    Ctx c = evaluate_material(sc, mat, sh.mat_index, rec, sh);
+    if (depth >= RR_MIN_PHOTON_DEPTH) {
+        f32 q = phx_fmaxf(throughput.x, phx_fmaxf(throughput.y, throughput.z));
+        q = phx_clampf(q, RR_MIN_Q, RR_MAX_Q);
+        if (random_float(rng) > q)
+            return false;
+        f32 iq = 1.0f / q;
+        throughput = throughput * iq;
+        power = power * iq;
+    }
+    F4 view = -dir;
+    Bsdf bs = sample_bsdf(rng, c, c.shading_normal, sh.normal, view, ior, sh.front_face);
+    if (bs.event == BSDF_DIFFUSE && dot(dir, sh.normal) < 0.0f) {
+        size_t slot = (size_t)n * g.stride + li;
+        st(g.pos + slot, sh.position);
+        g.power[slot] = encode_rgbe(power);
+        g.dir[slot] = encode_oct(dir);
+        n++;
+    }
+    power = power * bs.throughput;
+    throughput = throughput * bs.throughput;
+    F4 side = (dot(bs.dir, sh.normal) > 0.0f) ? sh.normal : -sh.normal;
+    origin = sh.position + bs.dir * EPS + side * EPS;
+    dir = bs.dir;
+    return depth + 1 < max_bounces;
+}
+
+/* Loads photon li's state, runs bounces depth .. (depth + 1 if one_bounce,
+ * else to the end of its path), stores what changed; returns whether the
+ * photon is still live after a single bounce. */
+PHX_DEV bool emit_advance(const SceneView &sc, const EmitWfState &S, const EmitStaging &g, u32 li, u32 depth,
+                          u32 max_bounces, bool one_bounce, u32 *stk) {
+    F4 origin = ld(S.origin + li), dir = ld(S.dir + li);
+    F4 power = ld(S.power + li), throughput = ld(S.thr + li);
+    u32 rng = S.rng[li];
+    IorStack ior{S.ior + li, g.stride, S.ior_cur[li]};
+    u32 n = S.count[li], n0 = n;
+    bool keep;
+    if (one_bounce) {
+        keep = emit_step(sc, g, li, depth, max_bounces, origin, dir, power, throughput, rng, ior, n, stk);
+        if (keep) {
+            st(S.origin + li, origin);
+            st(S.dir + li, dir);
+            st(S.power + li, power);
+            st(S.thr + li, throughput);
+            S.rng[li] = rng;
+            S.ior_cur[li] = ior.cur;
+        }
+    } else {
+        for (u32 d = depth; d < max_bounces; d++)
+            if (!emit_step(sc, g, li, d, max_bounces, origin, dir, power, throughput, rng, ior, n, stk))
+                break;
+        keep = false;
+    }
+    if (n != n0)
+        S.count[li] = n;
+    return keep;
+}
+
+/* appends li to a queue when keep, one atomic per wave; every lane of the
+ * wave must call it */
+PHX_DEV void wave_append(bool keep, u32 li, u32 *qcount, u32 *queue) {
+    u32 lane = threadIdx.x & 63u;
+    unsigned long long m = __ballot(keep);
+    if (m) {
+        u32 first = (u32)__ffsll((long long)m) - 1u;
+        u32 base = 0;
+        if (lane == first)
+            base = atomicAdd(qcount, (u32)__popcll(m));
+        base = __shfl(base, (int)first, 64);
+        if (keep)
+            queue[base + (u32)__popcll(m & ((1ull << lane) - 1ull))] = li;
+    }
+}
+
+/* ---------------- workspace layouts (kernels_trace.hip) ---------------- */
+WfState wf_layout(void *ws, u32 n_pixels, u32 direct_samples, bool with_scene);
+
+struct EmitLayout {
+    EmitStaging g;
+    u32 *offs, *total;
+    void *scan_ws;
+    EmitWfState st;
+};
+EmitWfState emit_wf_layout(char *&p, u32 photons, u32 max_bounces);
+EmitLayout emit_layout(void *ws, u32 photons, u32 max_bounces);
+/* scan of the per-photon deposit counts, compaction to *n_photons + offset,
+ * counter advance */
+hipError_t launch_emit_compact(const EmitLayout &L, u32 photons, u32 max_photons, phx_float4 *photon_pos,
+                               u32 *photon_power, u32 *photon_dir, u32 *n_photons, hipStream_t s);
+
+} // namespace phx
+
+#endif /* PHX_TRACE_PATHS_H */

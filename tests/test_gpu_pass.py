@@ -77,3 +77,62 @@ def test_reference_radius_rule_and_determinism():
     assert np.array_equal(_bits(fa), _bits(fr)) and np.array_equal(_bits(wa), _bits(wr))
     hit = wa > 0
     assert hit.any()
+
+
+def test_graph_replay_bit_exact():
+    """PHX_PASS_GRAPH: the pass is captured once and replayed; every replay,
+    with host work and synchronisations between replays, equals the oracle."""
+    scene = scenes.cornell_box(80, 56)
+    cfg = PassConfig(width=80, height=56, photons_per_round=1 << 13, max_photons_per_batch=3000,
+                     max_photon_bounces=8, grid_res=43, seed=5, initial_radius=0.02, graph=True)
+    gpu = Renderer(cfg, scene)
+    ref = oracle.RefRenderer(cfg, scene)
+    for p in range(4):
+        gpu.run_pass()
+        gpu.synchronize()
+        assert gpu.pass_ms() > 0.0
+        assert ref.run_pass() == 0
+        _compare_pass(gpu, ref, f"graph pass {p}")
+    gpu.close()
+
+
+def test_checkpoint_resume_bit_exact():
+    scene = scenes.caustic_scene(48, 40, level=2)
+    cfg = PassConfig(width=48, height=40, photons_per_round=1 << 12, max_photons_per_batch=1 << 12,
+                     max_photon_bounces=12, grid_res=60, seed=9, initial_radius=0.015)
+    a = Renderer(cfg, scene)
+    a.run(2)
+    ckpt = a.checkpoint()
+    a.run(2)
+    b = Renderer(cfg, scene)
+    b.restore(ckpt)
+    assert b.rounds_done == 2
+    b.run(2)
+    for x, y in zip(a.read_state(), b.read_state()):
+        assert np.array_equal(_bits(x), _bits(y))
+    assert np.array_equal(_bits(a.estimate()), _bits(b.estimate()))
+    other = Renderer(PassConfig(width=40, height=40, photons_per_round=1 << 10, max_photon_bounces=4, grid_res=20),
+                     scene)
+    with pytest.raises(Exception):
+        other.restore(ckpt)
+    for r in (a, b, other):
+        r.close()
+
+
+def test_pass_without_deposits_is_reported():
+    """All-mirror scene: no diffuse event, no deposit; renderer.cpp:117-118 is
+    reported by the synchronisation after the pass."""
+    from phosphor_amd import PhosphorError, abi
+
+    scene = scenes.cornell_box(32, 32)
+    scene.materials["metallic"] = 1.0
+    cfg = PassConfig(width=32, height=32, photons_per_round=1 << 10, max_photons_per_batch=1 << 10,
+                     max_photon_bounces=4, grid_res=20, seed=3)
+    r = Renderer(cfg, scene)
+    r.run_pass()
+    with pytest.raises(PhosphorError) as ei:
+        r.synchronize()
+    assert ei.value.code == abi.PHX_ERR_NO_PHOTONS
+    assert r.last_photon_count == 0
+    r.synchronize()  # reported once
+    r.close()
