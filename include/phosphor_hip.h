@@ -367,6 +367,41 @@ int phx_renderer_device_buffers(PhxRenderer *r, void **ptrs, uint32_t n_ptrs);
  * L = radiance/rounds + flux.xyz/(π r² photons_emitted). */
 int phx_renderer_estimate(PhxRenderer *r, float *rgb_out);
 
+/* ---- Multi-scene passes ----
+ * No counterpart in the reference (one Renderer renders one scene): a batch
+ * renders n scenes, each with its own PhxRenderArgs (max_photons_per_batch and
+ * trace_mode are not used; emission and trace run in groups of whole scenes),
+ * in one sequence of launches per pass.  Each scene's hitpoints, photon map
+ * and SPPM state are bit-identical to a phx_renderer_* render of that scene
+ * with the same arguments. */
+typedef struct PhxBatch PhxBatch;
+typedef struct PhxBatchOptions {
+    uint32_t trace_group_pixels; /* pixels traced per group (0 → 2 Mi) */
+    uint32_t emit_group_photons; /* photons emitted per group (0 → 8 Mi) */
+    uint32_t flags;              /* PHX_PASS_GRAPH */
+    uint32_t reserved;
+} PhxBatchOptions;
+
+int phx_batch_create(const PhxRenderArgs *args, const PhxSceneDesc *scenes, uint32_t n_scenes,
+                     const PhxBatchOptions *options, PhxBatch **out);
+void phx_batch_destroy(PhxBatch *b);
+/* One pass over every scene (asynchronous, no host synchronisation). */
+int phx_batch_run_pass(PhxBatch *b);
+/* Block until enqueued passes finish; PHX_ERR_NO_PHOTONS names a scene whose
+ * pass deposited nothing. */
+int phx_batch_synchronize(PhxBatch *b);
+int phx_batch_last_pass_ms(PhxBatch *b, float *out_ms);
+uint32_t phx_batch_rounds_done(PhxBatch *b);
+/* Deposits of each scene in the last pass. */
+int phx_batch_photon_counts(PhxBatch *b, uint32_t *counts);
+int phx_batch_read_state(PhxBatch *b, uint32_t scene, phx_float4 *flux, float *photon_weight,
+                         phx_float4 *radiance);
+int phx_batch_read_hitpoints(PhxBatch *b, uint32_t scene, phx_float4 *hp6);
+/* The scene's sorted photon map with cell ranges relative to its first record. */
+int phx_batch_read_photon_map(PhxBatch *b, uint32_t scene, phx_float4 *pos, uint32_t *power, uint32_t *dir,
+                              uint32_t *cell_start, uint32_t *cell_end);
+int phx_batch_estimate(PhxBatch *b, uint32_t scene, float *rgb_out);
+
 #ifdef __cplusplus
 }
 #endif

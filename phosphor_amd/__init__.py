@@ -185,6 +185,93 @@ class Renderer:
         return [int(v or 0) for v in out]
 
 
+class Batch:
+    """Many scenes rendered together (phx_batch_*): one pass over every scene,
+    each scene's results bit-identical to a ``Renderer`` with its config."""
+
+    def __init__(self, configs: list[PassConfig], scenes: list, trace_group_pixels: int = 0,
+                 emit_group_photons: int = 0, graph: bool = True):
+        if len(configs) != len(scenes) or not scenes:
+            raise ValueError("one PassConfig per scene, at least one scene")
+        self.configs = list(configs)
+        self.scenes = list(scenes)
+        n = len(scenes)
+        self._args = (RenderArgs * n)(*(c.to_c() for c in configs))
+        self._descs = [sc.desc() for sc in scenes]  # keeps the host arrays referenced
+        self._desc_arr = (abi.SceneDesc * n)(*self._descs)
+        opt = abi.BatchOptions(trace_group_pixels, emit_group_photons, abi.PHX_PASS_GRAPH if graph else 0, 0)
+        h = C.c_void_p()
+        check(lib().phx_batch_create(C.cast(self._args, C.c_void_p), C.cast(self._desc_arr, C.c_void_p), n,
+                                     C.byref(opt), C.byref(h)), "batch_create")
+        self._h = h
+        self.n_scenes = n
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().phx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run_pass(self) -> None:
+        check(lib().phx_batch_run_pass(self._h), "batch_run_pass")
+
+    def run(self, rounds: int) -> None:
+        for _ in range(rounds):
+            self.run_pass()
+        self.synchronize()
+
+    def synchronize(self) -> None:
+        check(lib().phx_batch_synchronize(self._h), "batch_synchronize")
+
+    def pass_ms(self) -> float:
+        out = np.zeros(1, np.float32)
+        check(lib().phx_batch_last_pass_ms(self._h, ptr(out)), "batch_pass_ms")
+        return float(out[0])
+
+    @property
+    def rounds_done(self) -> int:
+        return int(lib().phx_batch_rounds_done(self._h))
+
+    def photon_counts(self) -> np.ndarray:
+        out = np.zeros(self.n_scenes, np.uint32)
+        check(lib().phx_batch_photon_counts(self._h, ptr(out)), "batch_photon_counts")
+        return out
+
+    def _npx(self, s: int) -> int:
+        return self.configs[s].width * self.configs[s].height
+
+    def read_state(self, s: int):
+        n = self._npx(s)
+        flux, weight, rad = np.zeros((n, 4), np.float32), np.zeros(n, np.float32), np.zeros((n, 4), np.float32)
+        check(lib().phx_batch_read_state(self._h, s, ptr(flux), ptr(weight), ptr(rad)), "batch_read_state")
+        return flux, weight, rad
+
+    def read_hitpoints(self, s: int) -> np.ndarray:
+        hp = np.zeros((self._npx(s), 6, 4), np.float32)
+        check(lib().phx_batch_read_hitpoints(self._h, s, ptr(hp)), "batch_read_hitpoints")
+        return hp
+
+    def read_photon_map(self, s: int):
+        n = int(self.photon_counts()[s])
+        nb = self.configs[s].grid_res ** 3 + 1
+        pos, power, dirs = np.zeros((n, 4), np.float32), np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+        cs, ce = np.zeros(nb, np.uint32), np.zeros(nb, np.uint32)
+        check(lib().phx_batch_read_photon_map(self._h, s, ptr(pos), ptr(power), ptr(dirs), ptr(cs), ptr(ce)),
+              "batch_read_photon_map")
+        return pos, power, dirs, cs, ce
+
+    def estimate(self, s: int) -> np.ndarray:
+        c = self.configs[s]
+        rgb = np.zeros((c.width * c.height, 3), np.float32)
+        check(lib().phx_batch_estimate(self._h, s, ptr(rgb)), "batch_estimate")
+        return rgb.reshape(c.height, c.width, 3)
+
+
 def config_for(name: str, **overrides) -> tuple[PassConfig, object, int]:
     """PassConfig, scene and round count of a named workload (phosphor_amd.scenes.CONFIGS)."""
     from . import scenes
@@ -204,4 +291,4 @@ def config_for(name: str, **overrides) -> tuple[PassConfig, object, int]:
     return cfg, scene, spec["rounds"]
 
 
-__all__ = ["Renderer", "PassConfig", "PhosphorError", "lib", "check", "config_for", "abi"]
+__all__ = ["Renderer", "Batch", "PassConfig", "PhosphorError", "lib", "check", "config_for", "abi"]

@@ -68,6 +68,13 @@ class RenderArgs(C.Structure):
 PHX_PASS_GRAPH = 1
 
 
+class BatchOptions(C.Structure):
+    """PhxBatchOptions — group sizes of a multi-scene pass."""
+
+    _fields_ = [("trace_group_pixels", C.c_uint32), ("emit_group_photons", C.c_uint32), ("flags", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
 class SceneDesc(C.Structure):
     """PhxSceneDesc — host pointers of one scene (SceneData after the SoA split)."""
 
@@ -140,6 +147,17 @@ SIGNATURES = {
     "phx_renderer_read_photon_map": (INT, VP, VP, VP, VP, VP, VP),
     "phx_renderer_device_buffers": (INT, VP, VP, U32),
     "phx_renderer_estimate": (INT, VP, VP),
+    "phx_batch_create": (INT, VP, VP, U32, VP, C.POINTER(VP)),
+    "phx_batch_destroy": (None, VP),
+    "phx_batch_run_pass": (INT, VP),
+    "phx_batch_synchronize": (INT, VP),
+    "phx_batch_last_pass_ms": (INT, VP, VP),
+    "phx_batch_rounds_done": (U32, VP),
+    "phx_batch_photon_counts": (INT, VP, VP),
+    "phx_batch_read_state": (INT, VP, U32, VP, VP, VP),
+    "phx_batch_read_hitpoints": (INT, VP, U32, VP),
+    "phx_batch_read_photon_map": (INT, VP, U32, VP, VP, VP, VP, VP),
+    "phx_batch_estimate": (INT, VP, U32, VP),
 }
 
 

@@ -10,6 +10,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/phosphor_hip.h"
 
@@ -99,12 +100,75 @@ hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, 
                          float alpha, phx_float4 *flux, float *photon_weight, phx_float4 *radiance, uint32_t width,
                          hipStream_t s);
 
+/* ---- multi-scene passes ---- */
+/* One scene of a batch: its device scene, camera, grid and its ranges in the
+ * batch-wide arrays (dense pixels, 128-pixel chunks, 8×8 gather tiles,
+ * photons, grid buckets). */
+struct BatchScene {
+    SceneArgs sc;
+    PhxCameraParams cam;
+    PhxPhotonHashData hash;
+    uint32_t width, height, n_pixels, px_base;
+    uint32_t chunk_base, tile_base, tiles_x, tiles_y;
+    uint32_t photons, ph_base, max_bounces, direct_samples;
+    uint32_t bucket_base, n_buckets, any_hit, pad0;
+    float alpha;
+    uint32_t pad1[3];
+};
+
+/* scenes [s0, s1) of a trace or emission group and their dense range */
+struct BatchGroup {
+    uint32_t s0, s1, first, count, chunk0, chunks;
+};
+
+/* Everything one batched pass launches over (device pointers, static sizes). */
+struct BatchPlan {
+    uint32_t n_scenes = 0;
+    const BatchScene *scenes = nullptr; /* device table [n_scenes] */
+    const uint32_t *chunk_scene = nullptr, *tile_scene = nullptr;
+    const uint32_t *ph_base = nullptr; /* [n_scenes + 1] */
+    uint32_t *dep_base = nullptr;      /* [n_scenes + 1], per pass */
+    uint32_t *rng = nullptr, *round_seed = nullptr, *counts = nullptr, *zero_passes = nullptr;
+    uint32_t *tickets = nullptr; /* one per trace group */
+    uint32_t *dep_counter = nullptr;
+    phx_float4 *hp[6] = {};
+    phx_float4 *flux = nullptr, *radiance = nullptr;
+    float *photon_weight = nullptr;
+    phx_float4 *dep_pos = nullptr, *map_pos = nullptr;
+    uint32_t *dep_power = nullptr, *dep_dir = nullptr, *map_power = nullptr, *map_dir = nullptr;
+    uint32_t *hashes = nullptr, *cell_counts = nullptr, *cell_start = nullptr, *cell_end = nullptr;
+    uint32_t dep_cap = 0, total_buckets = 0, total_tiles = 0;
+    void *trace_ws = nullptr, *emit_ws = nullptr, *scan_ws = nullptr, *sort_ws = nullptr;
+    uint32_t trace_max_pixels = 0, max_direct_samples = 0, emit_max_photons = 0, max_bounces = 0;
+    uint32_t cam_blocks = 0;
+    std::vector<BatchGroup> trace_groups, emit_groups;
+};
+
+size_t batch_trace_workspace_bytes(uint32_t max_pixels, uint32_t max_direct_samples);
+size_t batch_emit_workspace_bytes(uint32_t max_photons, uint32_t max_bounces);
+/* persistent camera-kernel blocks that fit the device at once */
+uint32_t batch_camera_blocks();
+/* one pass over every scene of the plan (kernel launches only) */
+hipError_t launch_batch_pass(const BatchPlan &P, hipStream_t s);
+
 /* error bookkeeping of the C ABI */
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
 int hip_status(hipError_t e, const char *what);
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+/* pcg_seed of random.h:16-24 on the host (the round-seed generator) */
+inline uint32_t host_pcg_seed(uint32_t seed) {
+    uint32_t s = 0u;
+    s = s * 747796405u + 2891336453u;
+    s += seed;
+    s = s * 747796405u + 2891336453u;
+    return s;
+}
+
+/* PhxRenderArgs checks of cmd_args.hpp shared by the drivers */
+int check_render_args(const PhxRenderArgs &a, const char *who);
 
 } // namespace phx
 

@@ -16,9 +16,10 @@
  *   in LDS, so equal keys keep their input order; the last pass moves the
  *   24-byte photon records.  Result: the scatter of scatter_photons.cl with
  *   records inside a cell in input order.
- * gather: gather_flux.cl + photon_hash.h:56-103, one lane per pixel.
+ * gather: gather_flux.cl + photon_hash.h:56-103, one lane per pixel (body in
+ *   pass_bodies.h).
  */
-#include "phx_device.h"
+#include "pass_bodies.h"
 
 namespace phx {
 
@@ -506,18 +507,6 @@ hipError_t launch_sort(const phx_float4 *pos_in, const u32 *pow_in, const u32 *d
 }
 
 /* ---------------- gather (gather_flux.cl + photon_hash.h:56-103) ---------------- */
-struct GatherArgs {
-    const phx_float4 *hp_pos, *hp_normal, *hp_throughput, *hp_base_color, *hp_direct, *hp_emission;
-    u32 n_pixels;
-    const phx_float4 *ppos;
-    const u32 *ppow, *pdir, *cell_start, *cell_end;
-    PhxPhotonHashData d;
-    f32 alpha;
-    phx_float4 *flux;
-    f32 *photon_weight;
-    phx_float4 *radiance;
-};
-
 /* One wave per 8×8 pixel tile: neighbouring hitpoints query the same grid
  * cells, so the lanes of a wave read the same photon records (one memory
  * transaction serves many lanes) and run loops of similar length; tiles are
@@ -539,62 +528,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a, u32 width, u32 til
     }
     if (tid >= a.n_pixels)
         return;
-    F4 ge = ld(a.hp_emission + tid);
-    F4 rsf = ld(a.radiance + tid);
-    F4 r = mk(ge.x, ge.y, ge.z, 0.0f);
-    if (ge.w == 0.0f) {
-        st(a.radiance + tid, rsf + r);
-        return;
-    }
-    F4 dr = ld(a.hp_direct + tid);
-    r.x += dr.x;
-    r.y += dr.y;
-    r.z += dr.z;
-    st(a.radiance + tid, rsf + r);
-    F4 fsf = ld(a.flux + tid);
-    f32 wsf = a.photon_weight[tid];
-    f32 r_sq = fsf.w;
-    const PhxPhotonHashData &d = a.d;
-    if (r_sq < EPS) {
-        f32 hx = d.cell_sizes.x * 0.5f, hy = d.cell_sizes.y * 0.5f, hz = d.cell_sizes.z * 0.5f;
-        r_sq = hx * hx + hy * hy + hz * hz;
-    }
-    F4 pos = ld(a.hp_pos + tid), sn = ld(a.hp_normal + tid);
-    F4 nf = BLACK;
-    f32 found = 0.0f;
-    f32 two_r_sq = 2.0f * r_sq;
-    for (u32 i = 0; i < 27; i++) {
-        i32 cx = (i32)(i % 3) - 1, cy = (i32)((i / 3) % 3) - 1, cz = (i32)((i / 9) % 3) - 1;
-        f32 qx = pos.x + (f32)cx * d.cell_sizes.x;
-        f32 qy = pos.y + (f32)cy * d.cell_sizes.y;
-        f32 qz = pos.z + (f32)cz * d.cell_sizes.z;
-        u32 h = get_hash(qx, qy, qz, d);
-        if (h == 0)
-            continue;
-        u32 e0 = a.cell_end[h];
-        for (u32 k = a.cell_start[h]; k < e0; k++) {
-            F4 dd = pos - ld(a.ppos + k);
-            f32 dsq = dot(dd, dd);
-            if (dsq >= r_sq)
-                continue;
-            F4 pd = decode_oct(a.pdir[k]);
-            if (dot(pd, sn) > 0.0f)
-                continue;
-            f32 wgt = GAUSS_ALPHA * (1.0f - (1.0f - phx_expf(-GAUSS_BETA * dsq / two_r_sq)) / GAUSS_DENOM);
-            nf = nf + wgt * decode_rgbe(a.ppow[k]);
-            found += wgt;
-        }
-    }
-    if (found > 0.0f) {
-        f32 npw = wsf + a.alpha * found;
-        f32 ratio = npw / (wsf + found);
-        f32 nr = r_sq * ratio;
-        F4 albedo = ld(a.hp_base_color + tid);
-        F4 contrib = ld(a.hp_throughput + tid) * (albedo / PI) * nf;
-        F4 tmp = (fsf + contrib) * ratio;
-        st(a.flux + tid, mk(tmp.x, tmp.y, tmp.z, nr));
-        a.photon_weight[tid] = npw;
-    }
+    gather_pixel(a, tid, a.d, a.alpha, 0u);
 }
 
 hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const phx_float4 *hp_throughput,

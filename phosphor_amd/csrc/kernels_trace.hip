@@ -10,11 +10,11 @@
  * the NEE sums and every expression are those of the reference.
  *
  * trace_rays also has a wavefront form (camera paths, queued shadow rays,
- * in-order resolve; bodies in trace_paths.h) with the same outputs; the pass
+ * in-order resolve; bodies in pass_bodies.h) with the same outputs; the pass
  * driver uses it.
  *
  * emit_photons (kernels/emit_photons.cl:12-114): launches over a queue of
- * live photons (bodies in trace_paths.h); the k-th deposit of photon i goes
+ * live photons (bodies in pass_bodies.h); the k-th deposit of photon i goes
  * to a depth-major staging slot [k][i] and the per-photon deposit count is
  * stored; a final kernel compacts the staging area in (photon, bounce) order
  * after an exclusive scan of the counts, which makes the deposit order
@@ -23,7 +23,7 @@
  * Seeds: every launcher takes the round seed by value or, for passes that
  * run without host synchronisation (graph replay), a device pointer to it.
  */
-#include "trace_paths.h"
+#include "pass_bodies.h"
 
 namespace phx {
 
@@ -274,7 +274,7 @@ hipError_t launch_trace(const PhxCameraParams &cam, u32 w, u32 h, u32 seed, u32 
     return hipGetLastError();
 }
 
-/* ---------------- wavefront trace (bodies in trace_paths.h) ---------------- */
+/* ---------------- wavefront trace (bodies in pass_bodies.h) ---------------- */
 struct WfCamArgs {
     PhxCameraParams cam;
     u32 w, h, seed, n;

@@ -1,8 +1,8 @@
 /*
- * trace_paths.h — per-item bodies of the wavefront trace and the wavefront
- * photon emission, shared by the one-scene kernels (kernels_trace.hip) and
- * the multi-scene kernels (kernels_batch.hip), so both run the same code on
- * every pixel and photon.
+ * pass_bodies.h — per-item bodies of the wavefront trace, the wavefront
+ * photon emission and the flux gather, shared by the one-scene kernels
+ * (kernels_trace.hip, kernels_photon_map.hip) and the multi-scene kernels
+ * (kernels_batch.hip), so both run the same code on every pixel and photon.
  *
  * Wavefront trace (same outputs as trace_rays.cl:14-169, bit for bit): the
  * random draws of the NEE loops (light selection, emissive point) never
@@ -22,8 +22,8 @@
  * count) lives in HBM between bounces; its k-th deposit goes to staging slot
  * [k][photon].
  */
-#ifndef PHX_TRACE_PATHS_H
-#define PHX_TRACE_PATHS_H
+#ifndef PHX_PASS_BODIES_H
+#define PHX_PASS_BODIES_H
 
 #include "phx_device.h"
 
@@ -182,11 +182,11 @@ PHX_DEV void camera_path(const SceneView &sc, const PhxCameraParams &cam, u32 w,
                     st(S.sh_o + slot, mk(hit_pos.x, hit_pos.y, hit_pos.z, f.dist - EPS));
                     st(S.sh_d + slot, f.dir);
                     st(S.sh_pre + slot, mk(pre.x, pre.y, pre.z, 0.0f));
-                    if (S.sh_scene)
-                        S.sh_scene[slot] = tag;
                 } else {
                     st(S.sh_pre + slot, mk(0.0f, 0.0f, 0.0f, -1.0f));
                 }
+                if (S.sh_scene)
+                    S.sh_scene[slot] = tag;
             }
         }
         if (diffuse)
@@ -424,6 +424,81 @@ PHX_DEV void wave_append(bool keep, u32 li, u32 *qcount, u32 *queue) {
     }
 }
 
+/* ---------------- gather (gather_flux.cl + photon_hash.h:56-103) ---------------- */
+struct GatherArgs {
+    const phx_float4 *hp_pos, *hp_normal, *hp_throughput, *hp_base_color, *hp_direct, *hp_emission;
+    u32 n_pixels;
+    const phx_float4 *ppos;
+    const u32 *ppow, *pdir, *cell_start, *cell_end;
+    PhxPhotonHashData d;
+    f32 alpha;
+    phx_float4 *flux;
+    f32 *photon_weight;
+    phx_float4 *radiance;
+};
+
+/* gather_flux.cl:24-77 for pixel tid; the grid cell h of the pixel's scene is
+ * entry bucket_base + h of cell_start / cell_end.  The arithmetic and the
+ * order of the sums are those of photon_hash.h:65-100. */
+PHX_DEV void gather_pixel(const GatherArgs &a, u32 tid, const PhxPhotonHashData &d, f32 alpha, u32 bucket_base) {
+    F4 ge = ld(a.hp_emission + tid);
+    F4 rsf = ld(a.radiance + tid);
+    F4 r = mk(ge.x, ge.y, ge.z, 0.0f);
+    if (ge.w == 0.0f) {
+        st(a.radiance + tid, rsf + r);
+        return;
+    }
+    F4 dr = ld(a.hp_direct + tid);
+    r.x += dr.x;
+    r.y += dr.y;
+    r.z += dr.z;
+    st(a.radiance + tid, rsf + r);
+    F4 fsf = ld(a.flux + tid);
+    f32 wsf = a.photon_weight[tid];
+    f32 r_sq = fsf.w;
+    if (r_sq < EPS) {
+        f32 hx = d.cell_sizes.x * 0.5f, hy = d.cell_sizes.y * 0.5f, hz = d.cell_sizes.z * 0.5f;
+        r_sq = hx * hx + hy * hy + hz * hz;
+    }
+    F4 pos = ld(a.hp_pos + tid), sn = ld(a.hp_normal + tid);
+    F4 nf = BLACK;
+    f32 found = 0.0f;
+    f32 two_r_sq = 2.0f * r_sq;
+    const u32 *cs = a.cell_start + bucket_base, *ce = a.cell_end + bucket_base;
+    for (u32 i = 0; i < 27; i++) {
+        i32 cx = (i32)(i % 3) - 1, cy = (i32)((i / 3) % 3) - 1, cz = (i32)((i / 9) % 3) - 1;
+        f32 qx = pos.x + (f32)cx * d.cell_sizes.x;
+        f32 qy = pos.y + (f32)cy * d.cell_sizes.y;
+        f32 qz = pos.z + (f32)cz * d.cell_sizes.z;
+        u32 h = get_hash(qx, qy, qz, d);
+        if (h == 0)
+            continue;
+        u32 e0 = ce[h];
+        for (u32 k = cs[h]; k < e0; k++) {
+            F4 dd = pos - ld(a.ppos + k);
+            f32 dsq = dot(dd, dd);
+            if (dsq >= r_sq)
+                continue;
+            F4 pd = decode_oct(a.pdir[k]);
+            if (dot(pd, sn) > 0.0f)
+                continue;
+            f32 wgt = GAUSS_ALPHA * (1.0f - (1.0f - phx_expf(-GAUSS_BETA * dsq / two_r_sq)) / GAUSS_DENOM);
+            nf = nf + wgt * decode_rgbe(a.ppow[k]);
+            found += wgt;
+        }
+    }
+    if (found > 0.0f) {
+        f32 npw = wsf + alpha * found;
+        f32 ratio = npw / (wsf + found);
+        f32 nr = r_sq * ratio;
+        F4 albedo = ld(a.hp_base_color + tid);
+        F4 contrib = ld(a.hp_throughput + tid) * (albedo / PI) * nf;
+        F4 tmp = (fsf + contrib) * ratio;
+        st(a.flux + tid, mk(tmp.x, tmp.y, tmp.z, nr));
+        a.photon_weight[tid] = npw;
+    }
+}
+
 /* ---------------- workspace layouts (kernels_trace.hip) ---------------- */
 WfState wf_layout(void *ws, u32 n_pixels, u32 direct_samples, bool with_scene);
 
@@ -442,4 +517,4 @@ hipError_t launch_emit_compact(const EmitLayout &L, u32 photons, u32 max_photons
 
 } // namespace phx
 
-#endif /* PHX_TRACE_PATHS_H */
+#endif /* PHX_PASS_BODIES_H */

@@ -27,62 +27,9 @@
 
 #include "internal.h"
 #include "phx_math.h"
+#include "scene_buffers.h"
 
 namespace phx {
-
-/* cl::Buffer stand-in: one device allocation, freed on destruction */
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() {
-        if (p)
-            (void)hipFree(p);
-    }
-    hipError_t alloc(size_t n) {
-        if (p) {
-            (void)hipFree(p);
-            p = nullptr;
-        }
-        bytes = n < 16 ? 16 : n;
-        return hipMalloc(&p, bytes);
-    }
-    template <class T> T *as() const { return (T *)p; }
-};
-
-/* rendering.cpp:3-13: zero items still allocate one item */
-static hipError_t copy_to_dev_buf(DevBuf &b, const void *src, size_t count, size_t item, hipStream_t s) {
-    hipError_t e = b.alloc(count ? count * item : item);
-    if (e != hipSuccess)
-        return e;
-    if (count && src)
-        return hipMemcpyAsync(b.p, src, count * item, hipMemcpyHostToDevice, s);
-    return hipMemsetAsync(b.p, 0, b.bytes, s);
-}
-
-struct GeometryBuffers { /* include/buffers/geometry.hpp */
-    DevBuf d_tri_verts, d_tri_normals, d_tri_tangents, d_tri_uvs, d_tri_mat_index;
-    uint32_t n_triangles = 0;
-    DevBuf d_etri_verts, d_etri_normals, d_etri_uvs, d_etri_mat_index;
-    uint32_t n_e_triangles = 0;
-    DevBuf d_bvh_nodes;
-};
-
-struct ShadingBuffers { /* include/buffers/shading.hpp */
-    DevBuf d_materials, d_uv_transforms;
-    uint32_t n_materials = 0;
-    DevBuf d_lights, d_lum_pref_sum;
-    uint32_t n_lights = 0;
-    float total_lum = 0.0f;
-    DevBuf d_tex_atlas, d_tex_meta;
-    uint32_t n_textures = 0;
-    phx_float4 scene_center{};
-    float scene_radius = 0.0f;
-    DevBuf d_envmap_data;
-    uint32_t envmap_width = 0, envmap_height = 0, has_envmap = 0;
-};
 
 struct PhotonBuffers { /* include/buffers/photon.hpp (sorted photon map) */
     DevBuf d_photon_pos, d_photon_power, d_photon_dir, d_photon_hashes;
@@ -97,44 +44,12 @@ struct ImageBuffers { /* include/buffers/image.hpp */
     uint32_t image_width = 0, image_height = 0, n_pixels = 0;
 };
 
-struct RenderingBuffers {
-    GeometryBuffers geometry_buf;
-    ShadingBuffers shading_buf;
+struct RenderingBuffers { /* include/buffers/rendering.hpp */
+    SceneBuffers scene; /* geometry_buf + shading_buf */
     PhotonBuffers photon_buf;
     ImageBuffers image_buf;
 
-    SceneArgs scene_args() const {
-        const GeometryBuffers &g = geometry_buf;
-        const ShadingBuffers &s = shading_buf;
-        SceneArgs a;
-        std::memset(&a, 0, sizeof(a));
-        a.tv = g.d_tri_verts.as<PhxTriangleVertices>();
-        a.tn = g.d_tri_normals.as<PhxTriangleNormals>();
-        a.tt = g.d_tri_tangents.as<PhxTriangleTangents>();
-        a.tuv = g.d_tri_uvs.as<PhxTriangleUvs>();
-        a.tmat = g.d_tri_mat_index.as<uint32_t>();
-        a.nodes = g.d_bvh_nodes.as<PhxBvhNode>();
-        a.n_tris = g.n_triangles;
-        a.etv = g.d_etri_verts.as<PhxTriangleVertices>();
-        a.etn = g.d_etri_normals.as<PhxTriangleNormals>();
-        a.etuv = g.d_etri_uvs.as<PhxTriangleUvs>();
-        a.etmat = g.d_etri_mat_index.as<uint32_t>();
-        a.materials = s.d_materials.as<PhxMaterial>();
-        a.uvt = s.d_uv_transforms.as<PhxUvTransformBlock>();
-        a.tex_meta = s.d_tex_meta.as<PhxTextureMeta>();
-        a.atlas = s.d_tex_atlas.as<uint8_t>();
-        a.lights = s.d_lights.as<PhxLight>();
-        a.n_lights = s.n_lights;
-        a.lum_pref_sum = s.d_lum_pref_sum.as<float>();
-        a.total_lum = s.total_lum;
-        a.scene_center = s.scene_center;
-        a.scene_radius = s.scene_radius;
-        a.envmap = s.d_envmap_data.as<float>();
-        a.env_w = s.envmap_width;
-        a.env_h = s.envmap_height;
-        a.has_env = s.has_envmap;
-        return a;
-    }
+    SceneArgs scene_args() const { return scene.args(); }
 };
 
 } // namespace phx
@@ -196,14 +111,6 @@ struct PhxRenderer {
     int gather_flux();
 };
 
-static uint32_t host_pcg_seed(uint32_t seed) { /* random.h:16-24 */
-    uint32_t s = 0u;
-    s = s * 747796405u + 2891336453u;
-    s += seed;
-    s = s * 747796405u + 2891336453u;
-    return s;
-}
-
 #define PHX_TRY(expr, what)                                                                                            \
     do {                                                                                                               \
         hipError_t e_ = (expr);                                                                                        \
@@ -211,112 +118,34 @@ static uint32_t host_pcg_seed(uint32_t seed) { /* random.h:16-24 */
             return hip_status(e_, what);                                                                               \
     } while (0)
 
+namespace phx {
+int check_render_args(const PhxRenderArgs &a, const char *who) {
+    std::string w(who);
+    if (a.width == 0 || a.height == 0 || a.photons_per_round == 0)
+        return fail(PHX_ERR_INVALID_ARG, w + ": zero image or photon count");
+    if (a.grid_res == 0 || a.grid_res > 255)
+        return fail(PHX_ERR_INVALID_ARG, w + ": grid_res outside 1..255 (cmd_args.hpp:51)");
+    if (a.max_photon_bounces == 0 || a.max_photon_bounces > 32)
+        return fail(PHX_ERR_INVALID_ARG, w + ": max_bounces outside 1..32 (cmd_args.hpp:49)");
+    if (a.trace_mode > 1)
+        return fail(PHX_ERR_INVALID_ARG, w + ": trace_mode must be 0 or 1");
+    if (a.direct_samples == 0 || a.direct_samples > 128)
+        return fail(PHX_ERR_INVALID_ARG, w + ": direct_samples outside 1..128 (cmd_args.hpp:59)");
+    if (a.flags & ~(uint32_t)PHX_PASS_GRAPH)
+        return fail(PHX_ERR_INVALID_ARG, w + ": unknown flags");
+    return PHX_OK;
+}
+} // namespace phx
+
 /* renderer.cpp:22-57 + rendering.cpp:15-30 + geometry.cpp / shading.cpp / image.cpp / photon.cpp */
 int PhxRenderer::set_scene(const PhxSceneDesc &sc) {
     hipStream_t s = stream;
     const PhxRenderArgs &a = args;
-    uint32_t n = sc.n_triangles, ne = sc.n_e_triangles;
-
-    /* Bvh bvh(scene.triangles) */
-    std::vector<PhxBvhNode> nodes;
-    if (sc.bvh_nodes && sc.n_bvh_nodes) {
-        nodes.assign(sc.bvh_nodes, sc.bvh_nodes + sc.n_bvh_nodes);
-    } else {
-        nodes.resize(2 * (size_t)n - 1);
-        uint32_t got = phx_build_bvh_median(sc.tri_verts, n, nodes.data());
-        if (got != nodes.size())
-            return fail(PHX_ERR_INVALID_ARG, "set_scene: BVH build failed");
-    }
-    const PhxBvhNode &root = nodes[0];
-    /* any-hit shadow rays are exact when every material is opaque to shadow
-       rays (light_sampling.h:316) and the traversal stack never overflows
-       (no interior node deeper than 22, hit.h:163) */
-    {
-        bool transmissive = false;
-        for (uint32_t i = 0; i < sc.n_materials; i++)
-            transmissive = transmissive || !(sc.materials[i].transmission < 1e-5f);
-        uint32_t max_depth = 0;
-        std::vector<std::pair<uint32_t, uint32_t>> todo{{0u, 0u}};
-        while (!todo.empty()) {
-            auto [i, dpt] = todo.back();
-            todo.pop_back();
-            if (phx_as_uint(nodes[i].min.w) != 0xFFFFFFFFu)
-                continue;
-            max_depth = std::max(max_depth, dpt);
-            todo.push_back({i + 1, dpt + 1});
-            todo.push_back({phx_as_uint(nodes[i].max.w), dpt + 1});
-        }
-        shadow_any_hit = (!transmissive && max_depth <= 22) ? 1u : 0u;
-    }
-
-    GeometryBuffers &g = buffers.geometry_buf;
-    g.n_triangles = n;
-    g.n_e_triangles = ne;
-    PHX_TRY(copy_to_dev_buf(g.d_tri_verts, sc.tri_verts, n, sizeof(PhxTriangleVertices), s), "upload");
-    PHX_TRY(copy_to_dev_buf(g.d_tri_normals, sc.tri_normals, n, sizeof(PhxTriangleNormals), s), "upload");
-    PHX_TRY(copy_to_dev_buf(g.d_tri_tangents, sc.tri_tangents, n, sizeof(PhxTriangleTangents), s), "upload");
-    PHX_TRY(copy_to_dev_buf(g.d_tri_uvs, sc.tri_uvs, n, sizeof(PhxTriangleUvs), s), "upload");
-    PHX_TRY(copy_to_dev_buf(g.d_tri_mat_index, sc.tri_mat_index, n, sizeof(uint32_t), s), "upload");
-    PHX_TRY(copy_to_dev_buf(g.d_etri_verts, sc.etri_verts, ne, sizeof(PhxTriangleVertices), s), "upload");
-    PHX_TRY(copy_to_dev_buf(g.d_etri_normals, sc.etri_normals, ne, sizeof(PhxTriangleNormals), s), "upload");
-    PHX_TRY(copy_to_dev_buf(g.d_etri_uvs, sc.etri_uvs, ne, sizeof(PhxTriangleUvs), s), "upload");
-    PHX_TRY(copy_to_dev_buf(g.d_etri_mat_index, sc.etri_mat_index, ne, sizeof(uint32_t), s), "upload");
-    PHX_TRY(copy_to_dev_buf(g.d_bvh_nodes, nodes.data(), nodes.size(), sizeof(PhxBvhNode), s), "upload");
-
-    ShadingBuffers &sh = buffers.shading_buf;
-    sh.n_materials = sc.n_materials;
-    sh.n_lights = sc.n_lights;
-    sh.n_textures = sc.n_textures;
-    PHX_TRY(copy_to_dev_buf(sh.d_materials, sc.materials, sc.n_materials, sizeof(PhxMaterial), s), "upload");
-    PHX_TRY(copy_to_dev_buf(sh.d_uv_transforms, sc.uv_transforms, sc.n_materials, sizeof(PhxUvTransformBlock), s),
-            "upload");
-    PHX_TRY(copy_to_dev_buf(sh.d_lights, sc.lights, sc.n_lights, sizeof(PhxLight), s), "upload");
-    /* scene_data.cpp:14-23 luminance prefix sum; shading.cpp:21-23 total */
-    std::vector<float> lum;
-    float running = 0.0f;
-    for (uint32_t i = 0; i < sc.n_lights; i++) {
-        const PhxLight &l = sc.lights[i];
-        float v = l.power.x * 0.2126f + l.power.y * 0.7152f + l.power.z * 0.0722f;
-        running += std::max(v, 0.0f);
-        lum.push_back(running);
-    }
-    sh.total_lum = lum.empty() ? 0.0f : lum.back();
-    PHX_TRY(copy_to_dev_buf(sh.d_lum_pref_sum, lum.data(), lum.size(), sizeof(float), s), "upload");
-    /* shading.cpp:25-43 texture atlas (never empty) */
-    if (sc.n_textures) {
-        PHX_TRY(copy_to_dev_buf(sh.d_tex_meta, sc.tex_meta, sc.n_textures, sizeof(PhxTextureMeta), s), "upload");
-        PHX_TRY(copy_to_dev_buf(sh.d_tex_atlas, sc.tex_atlas, (size_t)sc.tex_atlas_bytes, 1, s), "upload");
-    } else {
-        PhxTextureMeta m{};
-        uint8_t zero = 0;
-        PHX_TRY(copy_to_dev_buf(sh.d_tex_meta, &m, 1, sizeof(PhxTextureMeta), s), "upload");
-        PHX_TRY(copy_to_dev_buf(sh.d_tex_atlas, &zero, 1, 1, s), "upload");
-    }
-    /* shading.cpp:45-58 bounding sphere of the root bbox */
-    float cx = 0.5f * (root.min.x + root.max.x), cy = 0.5f * (root.min.y + root.max.y),
-          cz = 0.5f * (root.min.z + root.max.z);
-    sh.scene_center = phx_float4{cx, cy, cz, 0.0f};
-    float radius = 0.0f;
-    for (int c = 0; c < 8; c++) {
-        float dx = ((c & 1) ? root.max.x : root.min.x) - cx;
-        float dy = ((c & 2) ? root.max.y : root.min.y) - cy;
-        float dz = ((c & 4) ? root.max.z : root.min.z) - cz;
-        radius = std::max(radius, phx_sqrtf(dx * dx + dy * dy + dz * dz));
-    }
-    sh.scene_radius = radius;
-    /* shading.cpp:60-71 envmap */
-    if (sc.envmap && sc.envmap_width && sc.envmap_height) {
-        sh.has_envmap = 1;
-        sh.envmap_width = sc.envmap_width;
-        sh.envmap_height = sc.envmap_height;
-        PHX_TRY(copy_to_dev_buf(sh.d_envmap_data, sc.envmap, 3ull * sc.envmap_width * sc.envmap_height, sizeof(float),
-                                s),
-                "upload");
-    } else {
-        sh.has_envmap = 0;
-        sh.envmap_width = sh.envmap_height = 0;
-        PHX_TRY(copy_to_dev_buf(sh.d_envmap_data, nullptr, 0, sizeof(float), s), "upload");
-    }
+    int rc = upload_scene(sc, s, buffers.scene);
+    if (rc != PHX_OK)
+        return rc;
+    shadow_any_hit = buffers.scene.shadow_any_hit;
+    const PhxBvhNode &root = buffers.scene.root;
 
     /* alloc_image_buffers (image.cpp:11-21, zero-filled) */
     ImageBuffers &im = buffers.image_buf;
@@ -535,21 +364,10 @@ int phx_renderer_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, Ph
     if (!args || !scene || !out)
         return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: null argument");
     *out = nullptr;
-    if (scene->n_triangles == 0 || !scene->tri_verts || !scene->tri_normals || !scene->tri_tangents ||
-        !scene->tri_uvs || !scene->tri_mat_index || !scene->materials || scene->n_materials == 0 || !scene->camera)
-        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: empty or incomplete scene (main.cpp:40-47)");
-    if (args->width == 0 || args->height == 0 || args->photons_per_round == 0)
-        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: zero image or photon count");
-    if (args->grid_res == 0 || args->grid_res > 255)
-        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: grid_res outside 1..255 (cmd_args.hpp:51)");
-    if (args->max_photon_bounces == 0 || args->max_photon_bounces > 32)
-        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: max_bounces outside 1..32 (cmd_args.hpp:49)");
-    if (args->trace_mode > 1)
-        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: trace_mode must be 0 or 1");
-    if (args->flags & ~(uint32_t)PHX_PASS_GRAPH)
-        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: unknown flags");
-    if (args->direct_samples == 0 || args->direct_samples > 128)
-        return fail(PHX_ERR_INVALID_ARG, "phx_renderer_create: direct_samples outside 1..128 (cmd_args.hpp:59)");
+    if (check_scene_desc(*scene, "phx_renderer_create") != PHX_OK)
+        return PHX_ERR_INVALID_ARG;
+    if (check_render_args(*args, "phx_renderer_create") != PHX_OK)
+        return PHX_ERR_INVALID_ARG;
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
         return fail(PHX_ERR_NO_DEVICE, "phx_renderer_create: no HIP device");

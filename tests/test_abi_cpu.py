@@ -94,3 +94,22 @@ def test_no_device_reported_here(phx_lib):
     if os.path.exists("/dev/kfd"):
         pytest.skip("a GPU is present")
     assert phx_lib.phx_device_count() == 0
+
+
+def test_batch_rejects_bad_arguments(phx_lib):
+    from phosphor_amd import PassConfig, scenes
+
+    sc = scenes.cornell_box(8, 8)
+    good, bad = PassConfig(width=8, height=8, photons_per_round=16, grid_res=8), \
+        PassConfig(width=8, height=8, photons_per_round=16, max_photon_bounces=0)
+    args = (abi.RenderArgs * 2)(good.to_c(), bad.to_c())
+    descs = (abi.SceneDesc * 2)(sc.desc(), sc.desc())
+    h = C.c_void_p()
+    rc = phx_lib.phx_batch_create(C.cast(args, C.c_void_p), C.cast(descs, C.c_void_p), 2, None, C.byref(h))
+    assert rc == abi.PHX_ERR_INVALID_ARG and not h.value
+    assert b"scene 1" in phx_lib.phx_last_error() and b"max_bounces" in phx_lib.phx_last_error()
+    assert phx_lib.phx_batch_create(C.cast(args, C.c_void_p), C.cast(descs, C.c_void_p), 0, None,
+                                    C.byref(h)) == abi.PHX_ERR_INVALID_ARG
+    opt = abi.BatchOptions(0, 0, 0x80, 0)
+    assert phx_lib.phx_batch_create(C.cast(args, C.c_void_p), C.cast(descs, C.c_void_p), 1, C.byref(opt),
+                                    C.byref(h)) == abi.PHX_ERR_INVALID_ARG
