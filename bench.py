@@ -8,6 +8,11 @@ photons+rays/s per pass = (pixels·1 camera ray + photons per pass) / T_pass.
 Default workload: the caustic stress scene (Cornell box + two 20480-triangle
 GGX spheres, 1024², 2^21 photons/pass, r0 = 0.01).
 
+--workload batch64 measures multi-scene passes instead: 64 generated scenes
+(2k-200k triangles, 256²-1024², 2^16-2^21 photons per pass) rendered together,
+reported as scene-passes/s (scenes × passes / s) beside the same scenes
+rendered one after another with the one-scene driver.
+
 Prints one JSON line.  With --gpus > 1 it exits: this build measures one GPU.
 """
 from __future__ import annotations
@@ -80,12 +85,83 @@ def run_cpu_baseline(cfg, scene, passes: int, threads: int):
     return work / dt, dt, stages
 
 
+def bench_batch(args) -> int:
+    """64 scenes × `steps` passes as one batch (graph replay), then the same
+    scenes rendered one by one (graph replay each); both timed host-side
+    between stream synchronisations, scene upload excluded."""
+    import phosphor_amd as P
+    from phosphor_amd import scenes
+
+    t0 = time.perf_counter()
+    work = scenes.batch_workload(64, seed=1)
+    cfgs = [c for c, _ in work]
+    scs = [s for _, s in work]
+    for c in cfgs:
+        c.graph = True
+    n_px = sum(c.width * c.height for c in cfgs)
+    n_ph = sum(c.photons_per_round for c in cfgs)
+    print(f"[bench] batch64: {sum(s.n_triangles for s in scs)} triangles, {n_px} pixels, {n_ph} photons per pass "
+          f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    b = P.Batch(cfgs, scs, graph=True)
+    print(f"[bench] batch created in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        b.run_pass()
+    b.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.run_pass()
+    b.synchronize()
+    dt_batch = time.perf_counter() - t0
+    last_ms = b.pass_ms()
+    b.close()
+    print(f"[bench] batch: {dt_batch * 1e3 / args.steps:.1f} ms/pass (last {last_ms:.1f} ms)", file=sys.stderr,
+          flush=True)
+    dt_plain = 0.0
+    for i, (c, s) in enumerate(work):
+        r = P.Renderer(c, s)
+        for _ in range(args.warmup):
+            r.run_pass()
+        r.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r.run_pass()
+        r.synchronize()
+        dt_plain += time.perf_counter() - t0
+        r.close()
+        if i % 16 == 15:
+            print(f"[bench] one-by-one: {i + 1} scenes, {dt_plain:.2f} s so far", file=sys.stderr, flush=True)
+    value = 64 * args.steps / dt_batch
+    line = {
+        "metric": "batched SPPM scene-passes/s",
+        "value": value,
+        "unit": "scene-passes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt_batch * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batch64", "scenes": 64, "triangles": sum(s.n_triangles for s in scs),
+                   "pixels_per_pass": n_px, "photons_per_pass": n_ph, "parallelism": "single-gpu"},
+        "photons_rays_per_s": (n_px + n_ph) * args.steps / dt_batch,
+        "one_by_one": {"value": 64 * args.steps / dt_plain, "unit": "scene-passes/s",
+                       "ms_per_step": dt_plain * 1e3 / args.steps},
+        "speedup_vs_one_by_one": dt_plain / dt_batch,
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="caustic", choices=["caustic", "cornell", "atrium"])
+    ap.add_argument("--workload", default="caustic", choices=["caustic", "cornell", "atrium", "batch64"])
     ap.add_argument("--cpu-passes", type=int, default=1, help="oracle passes timed for the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0 = all cores of this host)")
     args = ap.parse_args()
@@ -94,6 +170,9 @@ def main() -> int:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               "(the pass is single-GPU; sharding is described in DESIGN.md)", file=sys.stderr)
         return 2
+
+    if args.workload == "batch64":
+        return bench_batch(args)
 
     import phosphor_amd as P
 

@@ -12,6 +12,7 @@ no-UV fallback produces (scene.cpp:80-91).
 """
 from __future__ import annotations
 
+import functools
 import math
 from dataclasses import dataclass, field
 
@@ -281,8 +282,10 @@ class SceneBuilder:
                      tex_meta=np.zeros(0, TEX_META_DTYPE), tex_atlas=np.zeros(1, np.uint8), name=name, meta=meta)
 
 
+@functools.lru_cache(maxsize=None)
 def icosphere(level: int) -> tuple[np.ndarray, np.ndarray]:
-    """Unit icosphere: 20·4^level triangles (level 5 → 20480), vertices on the unit sphere."""
+    """Unit icosphere: 20·4^level triangles (level 5 → 20480), vertices on the
+    unit sphere.  Cached: callers must not modify the arrays."""
     t = (1.0 + 5.0 ** 0.5) / 2.0
     verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
              (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
@@ -490,6 +493,64 @@ def atrium_scene(width: int = 1920, height: int = 1080, target_tris: int = 300_0
     env = sky_envmap(env_size[0], env_size[1], sun_dir=sun)
     cam = make_camera((-1.0, -1.5, 1.7), (ext * 0.6, ext * 0.5, 2.2), (0.0, 0.0, 1.0), 70.0, width / height)
     return b.build(cam, name="atrium", envmap=env)
+
+
+def sphere_room(width: int, height: int, target_tris: int, rng: np.random.Generator, sky: bool = False) -> Scene:
+    """Cornell room (5 walls, 2 boxes, quad light) filled with icospheres up to
+    about `target_tris` triangles: diffuse (albedo U[0.2, 0.8], roughness
+    U[0.05, 0.9]), conductor or dielectric.  With `sky`, misses through the open
+    front see a sky envmap and a weak directional light joins the area light."""
+    b = SceneBuilder()
+    white = b.material(default_material(WHITE))
+    red = b.material(default_material(RED))
+    green = b.material(default_material(GREEN))
+    light = b.material(default_material(WHITE, emissive=(17.0, 17.0, 17.0)))
+    _cornell_room(b, white, red, green, light)
+    budget = target_tris - b.n_tris
+    while budget >= 20:
+        level = 4
+        while level > 0 and 20 * 4 ** level > budget:
+            level -= 1
+        u = rng.uniform()
+        if u < 0.7:
+            m = default_material(tuple(rng.uniform(0.2, 0.8, size=3)), roughness=float(rng.uniform(0.05, 0.9)))
+        elif u < 0.85:
+            m = default_material(tuple(rng.uniform(0.5, 0.95, size=3)), metallic=1.0,
+                                 roughness=float(rng.uniform(0.05, 0.5)))
+        else:
+            m = default_material((1.0, 1.0, 1.0), transmission=1.0, ior=1.5, roughness=0.05)
+        r = float(rng.uniform(0.02, 0.09))
+        c = rng.uniform(r, 1.0 - r, size=3)
+        c[1] = rng.uniform(r, 0.8)
+        b.add_sphere(tuple(c), r, b.material(m), level)
+        budget -= 20 * 4 ** level
+    env = None
+    if sky:
+        b.add_directional_light((0.2, -0.9, -0.35), (0.4, 0.4, 0.38))
+        env = sky_envmap(256, 128, sun_radiance=50.0)
+    cam = make_camera((0.5, 0.5, 2.2), (0.5, 0.5, 0.0), (0.0, 1.0, 0.0), 45.0, width / height)
+    return b.build(cam, name="sphere_room", envmap=env)
+
+
+def batch_workload(n_scenes: int = 64, seed: int = 1):
+    """The batched benchmark: n scenes with triangle counts log-uniform in
+    [2k, 200k], square resolutions from {256, 512, 1024}, photons per pass
+    log-uniform in [2^16, 2^21]; reference defaults otherwise (grid 64,
+    16 NEE samples, 16 bounces, α 0.7).  Scene i draws from PCG64(seed + i).
+    Returns [(PassConfig, Scene)]."""
+    from . import PassConfig
+
+    out = []
+    for i in range(n_scenes):
+        rng = np.random.Generator(np.random.PCG64(seed + i))
+        tris = int(round(math.exp(rng.uniform(math.log(2000), math.log(200000)))))
+        res = int(rng.choice([256, 512, 1024]))
+        photons = int(round(math.exp(rng.uniform(math.log(1 << 16), math.log(1 << 21)))))
+        scene = sphere_room(res, res, tris, rng, sky=(i % 4 == 3))
+        cfg = PassConfig(width=res, height=res, photons_per_round=photons, max_photons_per_batch=1 << 20,
+                         max_photon_bounces=16, grid_res=64, direct_samples=16, seed=seed + i, sppm_alpha=0.7)
+        out.append((cfg, scene))
+    return out
 
 
 # Configurations of the benchmark workloads (sizes of the published configs)
