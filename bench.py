@@ -156,6 +156,32 @@ def bench_batch(args) -> int:
     return 0
 
 
+def copy_calibration(mib: int) -> int:
+    """Runs phx_device_copy over `mib` MiB three times: the known-traffic kernel
+    (reads and writes exactly that many bytes) that scales the FETCH_SIZE and
+    WRITE_SIZE counters of the profiled kernels."""
+    import ctypes as C
+
+    import phosphor_amd as P
+
+    L = P.lib()
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    P.check(L.phx_device_malloc(C.byref(a), n), "malloc")
+    P.check(L.phx_device_malloc(C.byref(b), n), "malloc")
+    P.check(L.phx_memset_device(a, 1, n), "memset")
+    t0 = time.perf_counter()
+    for _ in range(3):
+        P.check(L.phx_device_copy(b, a, n, None), "copy")
+    P.check(L.phx_device_synchronize(), "sync")
+    dt = time.perf_counter() - t0
+    L.phx_device_free(a)
+    L.phx_device_free(b)
+    print(json.dumps({"metric": "device copy", "bytes_per_launch": n, "launches": 3,
+                      "GB_per_s_read_plus_write": 2 * 3 * n / dt / 1e9}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,7 +190,11 @@ def main() -> int:
     ap.add_argument("--workload", default="caustic", choices=["caustic", "cornell", "atrium", "batch64"])
     ap.add_argument("--cpu-passes", type=int, default=1, help="oracle passes timed for the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0 = all cores of this host)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the known-size copy kernel (counter calibration) over MIB MiB")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "

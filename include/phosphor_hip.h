@@ -259,6 +259,10 @@ int phx_device_free(void *p);
 int phx_memcpy_htod(void *dst, const void *src, size_t bytes);
 int phx_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int phx_memset_device(void *dst, int value, size_t bytes);
+/* Device-to-device copy with a plain streaming kernel (16-byte loads and
+ * stores, bytes a multiple of 16): the known-traffic reference used to scale
+ * hardware byte counters. */
+int phx_device_copy(void *dst, const void *src, size_t bytes, void *stream);
 int phx_device_synchronize(void);
 
 /* ======================================================================

@@ -128,6 +128,7 @@ SIGNATURES = {
     "phx_memcpy_htod": (INT, VP, VP, SZ),
     "phx_memcpy_dtoh": (INT, VP, VP, SZ),
     "phx_memset_device": (INT, VP, INT, SZ),
+    "phx_device_copy": (INT, VP, VP, C.c_size_t, VP),
     "phx_device_synchronize": (INT,),
     "phx_build_bvh_median": (U32, VP, U32, VP),
     "phx_build_photon_hash_info": (INT, VP, U32, VP),

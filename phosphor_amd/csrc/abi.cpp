@@ -317,6 +317,15 @@ int phx_memset_device(void *dst, int value, size_t bytes) {
     return hip_status(hipMemset(dst, value, bytes), "phx_memset_device");
 }
 
+int phx_device_copy(void *dst, const void *src, size_t bytes, void *stream) {
+    if ((!dst || !src) && bytes)
+        return fail(PHX_ERR_INVALID_ARG, "phx_device_copy: null pointer");
+    if (bytes % 16 != 0 || ((uintptr_t)dst | (uintptr_t)src) % 16 != 0)
+        return fail(PHX_ERR_INVALID_ARG, "phx_device_copy: bytes and pointers must be multiples of 16");
+    return hip_status(launch_copy16((phx_float4 *)dst, (const phx_float4 *)src, bytes / 16, (hipStream_t)stream),
+                      "phx_device_copy");
+}
+
 int phx_device_synchronize(void) { return hip_status(hipDeviceSynchronize(), "phx_device_synchronize"); }
 
 } /* extern "C" */

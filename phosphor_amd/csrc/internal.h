@@ -53,6 +53,8 @@ hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *
                        hipStream_t s, const uint32_t *d_n = nullptr);
 
 hipError_t launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
+/* dst[i] = src[i] for n 16-byte items */
+hipError_t launch_copy16(phx_float4 *dst, const phx_float4 *src, size_t n, hipStream_t s);
 /* seed[i] = random_u32(state[i]) for i < n (PCG, random.h:29-34) */
 hipError_t launch_next_round_seed(uint32_t *state, uint32_t *seed, uint32_t n, hipStream_t s);
 /* info[0] = *count; info[1] counts passes with a zero count */

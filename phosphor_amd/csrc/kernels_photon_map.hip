@@ -41,6 +41,24 @@ hipError_t launch_fill_u32(u32 *p, u32 v, size_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void k_copy16(phx_float4 *__restrict__ dst, const phx_float4 *__restrict__ src,
+                                                size_t n) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * 256;
+    for (; i < n; i += stride)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy16(phx_float4 *dst, const phx_float4 *src, size_t n, hipStream_t s) {
+    if (n == 0)
+        return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 16384)
+        blocks = 16384;
+    hipLaunchKernelGGL(k_copy16, dim3((u32)blocks), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
 /* Round seeds of a pass (renderer.cpp:65, random_u32 of the host PCG), one
  * per scene, drawn on the device so a pass needs no host input. */
 __global__ void k_next_round_seed(u32 *state, u32 *seed, u32 n) {

@@ -437,9 +437,15 @@ struct GatherArgs {
     phx_float4 *radiance;
 };
 
+/* photons of a cell whose records are loaded together before any is used:
+ * one memory round trip per GATHER_BATCH photons instead of one or two each */
+constexpr u32 GATHER_BATCH = 4;
+
 /* gather_flux.cl:24-77 for pixel tid; the grid cell h of the pixel's scene is
  * entry bucket_base + h of cell_start / cell_end.  The arithmetic and the
- * order of the sums are those of photon_hash.h:65-100. */
+ * order of the sums are those of photon_hash.h:65-100: cells in offset order,
+ * photons of a cell in storage order (loads are batched, their use is not
+ * reordered). */
 PHX_DEV void gather_pixel(const GatherArgs &a, u32 tid, const PhxPhotonHashData &d, f32 alpha, u32 bucket_base) {
     F4 ge = ld(a.hp_emission + tid);
     F4 rsf = ld(a.radiance + tid);
@@ -474,17 +480,31 @@ PHX_DEV void gather_pixel(const GatherArgs &a, u32 tid, const PhxPhotonHashData 
         if (h == 0)
             continue;
         u32 e0 = ce[h];
-        for (u32 k = cs[h]; k < e0; k++) {
-            F4 dd = pos - ld(a.ppos + k);
-            f32 dsq = dot(dd, dd);
-            if (dsq >= r_sq)
-                continue;
-            F4 pd = decode_oct(a.pdir[k]);
-            if (dot(pd, sn) > 0.0f)
-                continue;
-            f32 wgt = GAUSS_ALPHA * (1.0f - (1.0f - phx_expf(-GAUSS_BETA * dsq / two_r_sq)) / GAUSS_DENOM);
-            nf = nf + wgt * decode_rgbe(a.ppow[k]);
-            found += wgt;
+        for (u32 k = cs[h]; k < e0; k += GATHER_BATCH) {
+            F4 pp[GATHER_BATCH];
+            u32 pw[GATHER_BATCH], pdr[GATHER_BATCH];
+#pragma unroll
+            for (u32 j = 0; j < GATHER_BATCH; j++) {
+                u32 kk = k + j < e0 ? k + j : e0 - 1; /* in range: k < e0 */
+                pp[j] = ld(a.ppos + kk);
+                pw[j] = a.ppow[kk];
+                pdr[j] = a.pdir[kk];
+            }
+#pragma unroll
+            for (u32 j = 0; j < GATHER_BATCH; j++) {
+                if (k + j >= e0)
+                    break;
+                F4 dd = pos - pp[j];
+                f32 dsq = dot(dd, dd);
+                if (dsq >= r_sq)
+                    continue;
+                F4 pd = decode_oct(pdr[j]);
+                if (dot(pd, sn) > 0.0f)
+                    continue;
+                f32 wgt = GAUSS_ALPHA * (1.0f - (1.0f - phx_expf(-GAUSS_BETA * dsq / two_r_sq)) / GAUSS_DENOM);
+                nf = nf + wgt * decode_rgbe(pw[j]);
+                found += wgt;
+            }
         }
     }
     if (found > 0.0f) {

@@ -56,6 +56,9 @@ struct RenderingBuffers { /* include/buffers/rendering.hpp */
 
 using namespace phx;
 
+/* emission workspace allowed for one launch sequence over a whole round */
+static const size_t EMIT_ONE_SHOT_BYTES = 4ull << 30;
+
 struct PhxRenderer {
     PhxRenderArgs args{};
     hipStream_t stream = nullptr;
@@ -74,6 +77,7 @@ struct PhxRenderer {
     DevBuf d_rng;  /* [0] PCG state of the round seeds, [1] the current round seed */
     DevBuf d_info; /* [0] last deposit count, [1] passes without deposits */
     uint32_t shadow_any_hit = 0;
+    uint32_t emit_chunk = 1; /* photons per emission launch sequence */
     uint32_t rounds = 0;
     uint64_t total_photons_emitted = 0;
     uint32_t zero_reported = 0;
@@ -188,7 +192,14 @@ int PhxRenderer::set_scene(const PhxSceneDesc &sc) {
     PHX_TRY(p.d_cell_counts.alloc((size_t)bucket_count * sizeof(uint32_t)), "alloc cells");
     PHX_TRY(p.d_cell_start.alloc((size_t)bucket_count * sizeof(uint32_t)), "alloc cells");
     PHX_TRY(p.d_cell_end.alloc((size_t)bucket_count * sizeof(uint32_t)), "alloc cells");
+    /* --batch-size (renderer.cpp:103-111) bounds the reference's device memory;
+       deposits do not depend on it here (canonical order), so the whole round
+       is emitted in one launch sequence whenever its workspace fits
+       EMIT_ONE_SHOT_BYTES, and in batches of --batch-size otherwise */
     uint32_t per_batch = std::min(a.photons_per_round, a.max_photons_per_batch ? a.max_photons_per_batch : 1u);
+    if (emit_workspace_bytes(a.photons_per_round, a.max_photon_bounces) <= EMIT_ONE_SHOT_BYTES)
+        per_batch = a.photons_per_round;
+    emit_chunk = per_batch;
     PHX_TRY(d_emit_ws.alloc(emit_workspace_bytes(per_batch, a.max_photon_bounces)), "alloc emit workspace");
     PHX_TRY(d_scan_ws.alloc(scan_workspace_bytes(bucket_count)), "alloc scan workspace");
     PHX_TRY(d_sort_ws.alloc(sort_workspace_bytes(photon_buf_size, bucket_count)), "alloc sort workspace");
@@ -229,7 +240,7 @@ int PhxRenderer::trace_rays() {
 int PhxRenderer::emit_photons() {
     PHX_TRY(launch_fill_u32(d_batch_size.as<uint32_t>(), 0u, 1, stream), "emit_photons: reset counter");
     uint32_t ppr = args.photons_per_round;
-    uint32_t per_batch = std::min(ppr, args.max_photons_per_batch ? args.max_photons_per_batch : 1u);
+    uint32_t per_batch = emit_chunk;
     SceneArgs sc = buffers.scene_args();
     for (uint32_t off = 0; off < ppr; off += per_batch) {
         uint32_t to_emit = std::min(per_batch, ppr - off);
