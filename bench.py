@@ -8,6 +8,10 @@ photons+rays/s per pass = (pixels·1 camera ray + photons per pass) / T_pass.
 Default workload: the caustic stress scene (Cornell box + two 20480-triangle
 GGX spheres, 1024², 2^21 photons/pass, r0 = 0.01).
 
+--workload adjoint512 measures gradients: forward + backward of a 32-pass
+render of the eight-material room (512², 2^20 photons per pass) with respect
+to 40 material parameters, reported as gradient evaluations per second.
+
 --workload batch64 measures multi-scene passes instead: 64 generated scenes
 (2k-200k triangles, 256²-1024², 2^16-2^21 photons per pass) rendered together,
 reported as scene-passes/s (scenes × passes / s) beside the same scenes
@@ -156,6 +160,58 @@ def bench_batch(args) -> int:
     return 0
 
 
+def bench_adjoint(args) -> int:
+    """Full gradient evaluations (forward over 32 passes + backward) per second."""
+    from phosphor_amd import PassConfig, scenes
+    from phosphor_amd.adjoint import Adjoint, l2_loss
+
+    res, rounds = 512, 32
+    cfg = PassConfig(width=res, height=res, photons_per_round=1 << 20, max_photons_per_batch=1 << 20,
+                     max_photon_bounces=16, grid_res=64, seed=1)
+    scene = scenes.adjoint_scene(res, res, seed=1)
+    adj = Adjoint(cfg, scene, max_rounds=rounds)
+    th_true = adj.theta0()
+    target = adj.forward(th_true, rounds)
+    th = th_true.copy()
+    th[:, :4] *= np.random.default_rng(7).uniform(0.8, 1.2, size=(th.shape[0], 4)).astype(np.float32)
+    evals = max(1, args.steps // 4)
+    for _ in range(max(1, args.warmup // 2)):
+        img = adj.forward(th, rounds)
+        adj.backward(l2_loss(img, target)[1])
+    t_fwd = t_bwd = 0.0
+    for _ in range(evals):
+        t0 = time.perf_counter()
+        img = adj.forward(th, rounds)
+        t1 = time.perf_counter()
+        grad = adj.backward(l2_loss(img, target)[1])
+        t2 = time.perf_counter()
+        t_fwd += t1 - t0
+        t_bwd += t2 - t1
+    adj.close()
+    line = {
+        "metric": "SPPM gradient evaluations/s",
+        "value": evals / (t_fwd + t_bwd),
+        "unit": "grads/s",
+        "n_gpus": 1,
+        "steps": evals,
+        "warmup": max(1, args.warmup // 2),
+        "ms_per_step": (t_fwd + t_bwd) * 1e3 / evals,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "adjoint512", "materials": int(th.shape[0]), "parameters": int(th.size),
+                   "width": res, "height": res, "photons_per_pass": cfg.photons_per_round, "passes": rounds,
+                   "parallelism": "single-gpu"},
+        "forward_ms": t_fwd * 1e3 / evals,
+        "backward_ms": t_bwd * 1e3 / evals,
+        "grad_norm": float(np.linalg.norm(grad)),
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def copy_calibration(mib: int) -> int:
     """Runs phx_device_copy over `mib` MiB three times: the known-traffic kernel
     (reads and writes exactly that many bytes) that scales the FETCH_SIZE and
@@ -187,7 +243,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="caustic", choices=["caustic", "cornell", "atrium", "batch64"])
+    ap.add_argument("--workload", default="caustic", choices=["caustic", "cornell", "atrium", "batch64", "adjoint512"])
     ap.add_argument("--cpu-passes", type=int, default=1, help="oracle passes timed for the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0 = all cores of this host)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -203,6 +259,8 @@ def main() -> int:
 
     if args.workload == "batch64":
         return bench_batch(args)
+    if args.workload == "adjoint512":
+        return bench_adjoint(args)
 
     import phosphor_amd as P
 

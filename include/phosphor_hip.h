@@ -406,6 +406,31 @@ int phx_batch_read_photon_map(PhxBatch *b, uint32_t scene, phx_float4 *pos, uint
                               uint32_t *cell_start, uint32_t *cell_end);
 int phx_batch_estimate(PhxBatch *b, uint32_t scene, float *rgb_out);
 
+/* ---- Gradients of a render ----
+ * No counterpart in the reference.  θ holds five floats per material m
+ * (n_materials ≤ 16, no textures): albedo r, g, b (base_color), GGX roughness,
+ * emission scale (multiplies the material's emissive colour and the power of
+ * the lights built from its triangles; 1 = as given).  phx_adjoint_forward
+ * renders `rounds` passes from a fresh state with θ and returns the linear
+ * estimate (H·W·3 floats); phx_adjoint_backward maps ∂loss/∂estimate (same
+ * layout) to ∂loss/∂θ by replaying the passes (detached sampling: every random
+ * decision is held, the weights are differentiated). */
+typedef struct PhxAdjoint PhxAdjoint;
+int phx_adjoint_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, uint32_t max_rounds,
+                       PhxAdjoint **out);
+void phx_adjoint_destroy(PhxAdjoint *a);
+uint32_t phx_adjoint_num_params(PhxAdjoint *a);
+int phx_adjoint_forward(PhxAdjoint *a, const float *theta, uint32_t rounds, float *image_out);
+int phx_adjoint_backward(PhxAdjoint *a, const float *dloss_dimage, float *grad_out);
+/* Replay check: the backward's replay kernels also write what they recompute
+ * for the first pass (per pixel emission and direct, per photon deposits),
+ * which must equal the forward kernels' results for that pass. */
+int phx_adjoint_set_replay_check(PhxAdjoint *a, uint32_t on);
+int phx_adjoint_read_replay(PhxAdjoint *a, phx_float4 *emission, phx_float4 *direct, uint32_t *counts,
+                            uint32_t *forward_counts);
+/* The renderer the adjoint drives (its hitpoints hold the last replayed pass). */
+PhxRenderer *phx_adjoint_renderer(PhxAdjoint *a);
+
 #ifdef __cplusplus
 }
 #endif

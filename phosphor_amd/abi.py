@@ -148,6 +148,14 @@ SIGNATURES = {
     "phx_renderer_read_photon_map": (INT, VP, VP, VP, VP, VP, VP),
     "phx_renderer_device_buffers": (INT, VP, VP, U32),
     "phx_renderer_estimate": (INT, VP, VP),
+    "phx_adjoint_create": (INT, C.POINTER(RenderArgs), C.POINTER(SceneDesc), U32, C.POINTER(VP)),
+    "phx_adjoint_destroy": (None, VP),
+    "phx_adjoint_num_params": (U32, VP),
+    "phx_adjoint_forward": (INT, VP, VP, U32, VP),
+    "phx_adjoint_backward": (INT, VP, VP, VP),
+    "phx_adjoint_set_replay_check": (INT, VP, U32),
+    "phx_adjoint_read_replay": (INT, VP, VP, VP, VP, VP),
+    "phx_adjoint_renderer": (VP, VP),
     "phx_batch_create": (INT, VP, VP, U32, VP, C.POINTER(VP)),
     "phx_batch_destroy": (None, VP),
     "phx_batch_run_pass": (INT, VP),

@@ -24,8 +24,8 @@ LIB = BUILD / "libphosphor_hip.so"
 ORACLE_DIR = ROOT / "oracle"
 ORACLE_LIB = ORACLE_DIR / "build" / "librefcpu.so"
 
-SOURCES = ["kernels_trace.hip", "kernels_photon_map.hip", "kernels_batch.hip", "abi.cpp", "renderer.cpp",
-           "batch.cpp", "scene_upload.cpp", "bvh.cpp"]
+SOURCES = ["kernels_trace.hip", "kernels_photon_map.hip", "kernels_batch.hip", "kernels_adjoint.hip", "abi.cpp",
+           "renderer.cpp", "batch.cpp", "adjoint.cpp", "scene_upload.cpp", "bvh.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -93,14 +93,14 @@ size_t sort_workspace_bytes(uint32_t n, uint32_t n_cells);
 hipError_t launch_sort(const phx_float4 *pos_in, const uint32_t *pow_in, const uint32_t *dir_in,
                        const uint32_t *hashes, const uint32_t *cell_start, uint32_t *cell_end, uint32_t n,
                        uint32_t n_cells, phx_float4 *pos_out, uint32_t *pow_out, uint32_t *dir_out, void *ws,
-                       hipStream_t s, const uint32_t *d_n = nullptr);
+                       hipStream_t s, const uint32_t *d_n = nullptr, uint32_t *perm_out = nullptr);
 
 hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const phx_float4 *hp_throughput,
                          const phx_float4 *hp_base_color, const phx_float4 *hp_direct, const phx_float4 *hp_emission,
                          uint32_t n_pixels, const phx_float4 *ppos, const uint32_t *ppow, const uint32_t *pdir,
                          const uint32_t *cell_start, const uint32_t *cell_end, const PhxPhotonHashData &d,
                          float alpha, phx_float4 *flux, float *photon_weight, phx_float4 *radiance, uint32_t width,
-                         hipStream_t s);
+                         hipStream_t s, float *rec_ratio = nullptr, float *rec_r2 = nullptr);
 
 /* ---- multi-scene passes ---- */
 /* One scene of a batch: its device scene, camera, grid and its ranges in the
@@ -152,6 +152,40 @@ size_t batch_emit_workspace_bytes(uint32_t max_photons, uint32_t max_bounces);
 uint32_t batch_camera_blocks();
 /* one pass over every scene of the plan (kernel launches only) */
 hipError_t launch_batch_pass(const BatchPlan &P, hipStream_t s);
+
+/* ---- adjoint ---- */
+/* what the gradient needs per material (host-filled, device table) */
+struct AdjMaterial {
+    float inv_bc[3];  /* 1 / base_color */
+    float rough_gate; /* dα/dθ = 2·roughness where the roughness is not clamped, else 0 */
+    float inv_emis;   /* 1 / emission scale (0 for materials that do not emit) */
+    float pad[3];
+};
+
+/* everything one pass of the backward sweep reads and writes */
+struct AdjPass {
+    SceneArgs sc;
+    PhxCameraParams cam;
+    PhxPhotonHashData hash;
+    uint32_t w, h, n_direct, photons, max_bounces, M, max_deposits;
+    const uint32_t *d_seed;
+    const AdjMaterial *am;
+    const uint32_t *light_mat;
+    const phx_float4 *hp_pos, *hp_normal, *hp_throughput, *hp_base_color, *hp_emission;
+    const phx_float4 *ppos;
+    const uint32_t *ppow, *pdir, *cell_start, *cell_end, *perm, *d_count;
+    void *emit_ws;
+    const float *ratio, *r2;
+    float *R;
+    const phx_float4 *lam_rad, *lam_tau;
+    phx_float4 *nf, *adj_sorted, *adj_dep;
+    float *grad;
+    phx_float4 *chk_emission, *chk_direct;
+    uint32_t *chk_count;
+};
+hipError_t launch_adjoint_pass(const AdjPass &p, hipStream_t s);
+/* per-photon deposit counts of the last emission in an emission workspace */
+const uint32_t *adjoint_emit_counts(void *emit_ws, uint32_t photons, uint32_t max_bounces);
 
 /* error bookkeeping of the C ABI */
 void set_error(const std::string &msg);

@@ -390,7 +390,8 @@ hipError_t launch_batch_pass(const BatchPlan &P, hipStream_t s) {
 
     /* gather_flux over every tile of every scene */
     GatherArgs ga{P.hp[0], P.hp[1], P.hp[2], P.hp[3], P.hp[4], P.hp[5], 0u, P.map_pos, P.map_power, P.map_dir,
-                  P.cell_start, P.cell_end, PhxPhotonHashData{}, 0.0f, P.flux, P.photon_weight, P.radiance};
+                  P.cell_start, P.cell_end, PhxPhotonHashData{}, 0.0f, P.flux, P.photon_weight, P.radiance,
+                  nullptr, nullptr};
     BGatherArgs gargs{P.scenes, P.tile_scene, P.total_tiles, ga};
     hipLaunchKernelGGL(k_bgather, dim3((P.total_tiles + 3) / 4), dim3(256), 0, s, gargs);
     return hipGetLastError();

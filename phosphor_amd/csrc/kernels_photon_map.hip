@@ -360,7 +360,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(
     const u32 *__restrict__ keys_in, const u32 *__restrict__ vals_in, u32 n, const u32 *d_n, u32 shift,
     const u32 *__restrict__ offs, u32 *__restrict__ keys_out, u32 *__restrict__ vals_out, int final_pass,
     const phx_float4 *__restrict__ pos_in, const u32 *__restrict__ pow_in, const u32 *__restrict__ dir_in,
-    phx_float4 *__restrict__ pos_out, u32 *__restrict__ pow_out, u32 *__restrict__ dir_out) {
+    phx_float4 *__restrict__ pos_out, u32 *__restrict__ pow_out, u32 *__restrict__ dir_out, u32 *__restrict__ perm) {
     __shared__ u32 cnt[SORT_WAVES][RADIX];
     u32 t = threadIdx.x, lane = t & 63u, w = t >> 6;
     n = sort_count(n, d_n);
@@ -417,6 +417,8 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(
                 pos_out[dst] = pos_in[src];
                 pow_out[dst] = pow_in[src];
                 dir_out[dst] = dir_in[src];
+                if (perm)
+                    perm[dst] = src;
             } else {
                 keys_out[dst] = key[k];
                 vals_out[dst] = val[k];
@@ -463,7 +465,7 @@ constexpr u32 SORT_DEV_BLOCKS = 2048;
 
 hipError_t launch_sort(const phx_float4 *pos_in, const u32 *pow_in, const u32 *dir_in, const u32 *hashes,
                        const u32 *cell_start, u32 *cell_end, u32 n, u32 n_cells, phx_float4 *pos_out, u32 *pow_out,
-                       u32 *dir_out, void *ws, hipStream_t s, const u32 *d_n) {
+                       u32 *dir_out, void *ws, hipStream_t s, const u32 *d_n, u32 *perm_out) {
     hipError_t e;
     if (n > 0) {
         u32 n_tiles = (u32)(((size_t)n + SORT_TILE - 1) / SORT_TILE);
@@ -506,7 +508,8 @@ hipError_t launch_sort(const phx_float4 *pos_in, const u32 *pow_in, const u32 *d
                 return e;
             u32 *kout = kbuf[pass & 1], *vout = vbuf[pass & 1];
             hipLaunchKernelGGL(k_radix_scatter, dim3(grid), dim3(SORT_BLOCK), 0, s, kin, vin, n, d_n, shift, offs,
-                               kout, vout, last ? 1 : 0, pos_in, pow_in, dir_in, pos_out, pow_out, dir_out);
+                               kout, vout, last ? 1 : 0, pos_in, pow_in, dir_in, pos_out, pow_out, dir_out,
+                               perm_out);
             if ((e = hipGetLastError()) != hipSuccess)
                 return e;
             kin = kout;
@@ -553,11 +556,14 @@ hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, 
                          const phx_float4 *hp_base_color, const phx_float4 *hp_direct, const phx_float4 *hp_emission,
                          u32 n_pixels, const phx_float4 *ppos, const u32 *ppow, const u32 *pdir, const u32 *cell_start,
                          const u32 *cell_end, const PhxPhotonHashData &d, f32 alpha, phx_float4 *flux,
-                         f32 *photon_weight, phx_float4 *radiance, u32 width, hipStream_t s) {
+                         f32 *photon_weight, phx_float4 *radiance, u32 width, hipStream_t s, f32 *rec_ratio,
+                         f32 *rec_r2) {
     if (n_pixels == 0)
         return hipSuccess;
-    GatherArgs a{hp_pos, hp_normal, hp_throughput, hp_base_color, hp_direct, hp_emission, n_pixels, ppos, ppow,
-                 pdir,   cell_start, cell_end,   d,             alpha,     flux,      photon_weight, radiance};
+    GatherArgs a{hp_pos,     hp_normal, hp_throughput, hp_base_color, hp_direct, hp_emission,
+                 n_pixels,   ppos,      ppow,          pdir,          cell_start, cell_end,
+                 d,          alpha,     flux,          photon_weight, radiance,  rec_ratio,
+                 rec_r2};
     if (width == 0) { /* image shape unknown (C ABI call): flat pixel range */
         hipLaunchKernelGGL(k_gather, dim3((n_pixels + 255) / 256), dim3(256), 0, s, a, 0u, 0u);
         return hipGetLastError();

@@ -66,6 +66,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(TraceArgs a) {
     Bsdf bs;
     bs.dir = ZERO;
     bs.throughput = ZERO;
+    bs.h = ZERO;
     bs.event = BSDF_DIFFUSE;
     /* state of the current shadow ray (light_sampling.h:302-333) */
     F4 sh_dir = ZERO, sh_pre = ZERO, sh_trans = WHITE;

@@ -85,6 +85,7 @@ PHX_DEV void camera_path(const SceneView &sc, const PhxCameraParams &cam, u32 w,
         Bsdf bs;
         bs.dir = ZERO;
         bs.throughput = ZERO;
+        bs.h = ZERO;
         bs.event = BSDF_TRANSMIT;
         F4 hit_n = ZERO;
         if (active) {
@@ -435,6 +436,9 @@ struct GatherArgs {
     phx_float4 *flux;
     f32 *photon_weight;
     phx_float4 *radiance;
+    /* optional (adjoint forward): the pass's radius ratio (1 where the state
+       is not updated) and r² per pixel */
+    f32 *rec_ratio, *rec_r2;
 };
 
 /* photons of a cell whose records are loaded together before any is used:
@@ -452,6 +456,8 @@ PHX_DEV void gather_pixel(const GatherArgs &a, u32 tid, const PhxPhotonHashData 
     F4 r = mk(ge.x, ge.y, ge.z, 0.0f);
     if (ge.w == 0.0f) {
         st(a.radiance + tid, rsf + r);
+        if (a.rec_ratio)
+            a.rec_ratio[tid] = 1.0f;
         return;
     }
     F4 dr = ld(a.hp_direct + tid);
@@ -507,15 +513,20 @@ PHX_DEV void gather_pixel(const GatherArgs &a, u32 tid, const PhxPhotonHashData 
             }
         }
     }
+    f32 ratio = 1.0f;
     if (found > 0.0f) {
         f32 npw = wsf + alpha * found;
-        f32 ratio = npw / (wsf + found);
+        ratio = npw / (wsf + found);
         f32 nr = r_sq * ratio;
         F4 albedo = ld(a.hp_base_color + tid);
         F4 contrib = ld(a.hp_throughput + tid) * (albedo / PI) * nf;
         F4 tmp = (fsf + contrib) * ratio;
         st(a.flux + tid, mk(tmp.x, tmp.y, tmp.z, nr));
         a.photon_weight[tid] = npw;
+    }
+    if (a.rec_ratio) {
+        a.rec_ratio[tid] = ratio;
+        a.rec_r2[tid] = r_sq;
     }
 }
 

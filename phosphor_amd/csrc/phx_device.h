@@ -1,12 +1,13 @@
 /*
  * phx_device.h — gfx950 device helpers of the SPPM pass.
  *
- * Restates the reference's device helpers (include/device/*.h, the OpenCL
- * halves of include/common/*.h) for HIP.  Arithmetic follows the reference
- * expression by expression with IEEE division/sqrt, FP contraction off and the
- * elementary functions of phx_math.h, so a kernel built from these helpers is
- * a deterministic function of its inputs.  Vector builtins use one fixed
- * evaluation order: dot = ((x·x + y·y) + z·z) + w·w, length = sqrt(dot),
+ * Restates the reference's device helpers (the headers of include/device/ and
+ * the OpenCL halves of those of include/common/) for HIP.  Arithmetic follows
+ * the reference expression by expression with IEEE division/sqrt, FP
+ * contraction off and the elementary functions of phx_math.h, so a kernel
+ * built from these helpers is a deterministic function of its inputs.
+ * Vector builtins use one fixed evaluation order:
+ * dot = ((x·x + y·y) + z·z) + w·w, length = sqrt(dot),
  * normalize(v) = v · (1/length(v)) with the zero vector left unchanged.
  *
  * Layout choices for gfx950: nodes, vertices and materials are read with
@@ -631,12 +632,14 @@ PHX_DEV f32 ggx_ndf(f32 ndh, f32 alpha) {
 
 struct Bsdf {
     F4 dir, throughput;
+    F4 h; /* the sampled GGX half vector (read by the adjoint) */
     u32 event;
 };
 /* shading.h:150-213 (the order of random draws is part of the contract) */
 PHX_DEV Bsdf sample_bsdf(u32 &rng, const Ctx &c, F4 sn, F4 gn, F4 view, IorStack &ior, bool front) {
     Bsdf s;
     F4 h = ggx_sample_vndf(rng, sn, gn, view, c.roughness);
+    s.h = h;
     f32 alpha = c.roughness * c.roughness;
     if (random_float(rng) < c.metallic) {
         s.dir = safe_reflect(-view, h, gn);
@@ -696,6 +699,7 @@ struct EmissivePoint {
 struct LightFragment {
     F4 radiance, dir;
     f32 dist, n_dot_dir;
+    u32 light; /* index of the sampled light (read by the adjoint) */
 };
 PHX_DEV f32 spotlight_falloff(f32 ca, f32 ci, f32 co) {
     f32 t = phx_clampf((ca - co) / (ci - co + EPS), 0.0f, 1.0f);
@@ -773,15 +777,19 @@ PHX_DEV u32 select_light(u32 &rng, const SceneView &sc, f32 &scale) {
     return lo;
 }
 /* light_sampling.h:200-227 */
-PHX_DEV void sample_light(u32 &rng, const SceneView &sc, F4 &origin, F4 &dir, F4 &power) {
+PHX_DEV void sample_light(u32 &rng, const SceneView &sc, F4 &origin, F4 &dir, F4 &power, u32 *light = nullptr) {
     if (sc.n_lights == 0 || sc.total_lum < EPS) {
         origin = mk(0.0f, 0.0f, 0.0f, 0.0f);
         dir = mk(0.0f, 1.0f, 0.0f, 0.0f);
         power = BLACK;
+        if (light)
+            *light = 0xFFFFFFFFu;
         return;
     }
     f32 scale;
     u32 idx = select_light(rng, sc, scale);
+    if (light)
+        *light = idx;
     const PhxLight &l = sc.lights[idx];
     u32 kind = l.kind;
     if (kind == PHX_LIGHT_POINT) {
@@ -881,6 +889,7 @@ PHX_DEV bool sample_light_fragment(u32 &rng, const SceneView &sc, F4 pos, F4 nor
     out.dir = dir;
     out.dist = dist;
     out.n_dot_dir = ndd;
+    out.light = li;
     return true;
 }
 

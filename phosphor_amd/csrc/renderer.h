@@ -1,0 +1,102 @@
+/*
+ * renderer.h — the one-scene pass driver's state (src/renderer/renderer.hpp,
+ * include/buffers/photon.hpp, include/buffers/image.hpp,
+ * include/buffers/rendering.hpp), shared by renderer.cpp and the adjoint
+ * driver, which replays the driver's stages.
+ */
+#ifndef PHX_RENDERER_H
+#define PHX_RENDERER_H
+
+#include "internal.h"
+#include "scene_buffers.h"
+
+namespace phx {
+
+struct PhotonBuffers { /* include/buffers/photon.hpp (sorted photon map) */
+    DevBuf d_photon_pos, d_photon_power, d_photon_dir, d_photon_hashes;
+    DevBuf d_cell_counts, d_cell_start, d_cell_end;
+    uint32_t n_photons = 0;
+};
+
+struct ImageBuffers { /* include/buffers/image.hpp */
+    DevBuf d_hp_pos, d_hp_normal, d_hp_throughput, d_hp_base_color, d_hp_direct, d_hp_emission;
+    DevBuf d_flux, d_photon_weight, d_radiance;
+    PhxCameraParams camera{};
+    uint32_t image_width = 0, image_height = 0, n_pixels = 0;
+};
+
+struct RenderingBuffers { /* include/buffers/rendering.hpp */
+    SceneBuffers scene; /* geometry_buf + shading_buf */
+    PhotonBuffers photon_buf;
+    ImageBuffers image_buf;
+
+    SceneArgs scene_args() const { return scene.args(); }
+};
+
+} // namespace phx
+
+/* the C ABI's opaque renderer (kept outside the namespace: include/phosphor_hip.h
+   declares it at global scope) */
+struct PhxRenderer {
+    typedef phx::DevBuf DevBuf;
+    PhxRenderArgs args{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[5] = {};
+    hipEvent_t ev_info = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    uint32_t *h_info = nullptr; /* pinned: last deposit count, passes without deposits */
+    phx::RenderingBuffers buffers;
+    PhxPhotonHashData photon_hash_info{};
+    uint32_t bucket_count = 0;
+    uint32_t photon_buf_size = 0;
+    /* Renderer's own unsorted photon buffers (renderer.hpp:48-51) */
+    DevBuf d_photon_pos, d_photon_power, d_photon_dir, d_batch_size;
+    DevBuf d_emit_ws, d_scan_ws, d_sort_ws, d_trace_ws;
+    DevBuf d_rng;  /* [0] PCG state of the round seeds, [1] the current round seed */
+    DevBuf d_info; /* [0] last deposit count, [1] passes without deposits */
+    uint32_t shadow_any_hit = 0;
+    uint32_t emit_chunk = 1; /* photons per emission launch sequence */
+    uint32_t rounds = 0;
+    uint64_t total_photons_emitted = 0;
+    uint32_t zero_reported = 0;
+    bool timed = false;
+    bool info_pending = false;
+    /* set by the adjoint driver: the gather records each pixel's radius
+       ratio and r² of the pass, and the sort records its permutation */
+    float *rec_ratio = nullptr, *rec_r2 = nullptr;
+    uint32_t *sort_perm = nullptr;
+
+    ~PhxRenderer() {
+        if (stream)
+            (void)hipStreamSynchronize(stream);
+        if (graph_exec)
+            (void)hipGraphExecDestroy(graph_exec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        for (hipEvent_t &e : ev)
+            if (e)
+                (void)hipEventDestroy(e);
+        if (ev_info)
+            (void)hipEventDestroy(ev_info);
+        if (h_info)
+            (void)hipHostFree(h_info);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+
+    uint32_t *seed_ptr() const { return d_rng.as<uint32_t>() + 1; }
+    int set_scene(const PhxSceneDesc &scene);
+    int run_pass();
+    int enqueue_pass(bool events);
+    int capture_pass();
+    int collect();
+    int check_passes();
+    int trace_rays();
+    int emit_photons();
+    int hash_and_sort_photons();
+    int gather_flux();
+};
+
+
+#endif /* PHX_RENDERER_H */

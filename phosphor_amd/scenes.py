@@ -532,6 +532,35 @@ def sphere_room(width: int, height: int, target_tris: int, rng: np.random.Genera
     return b.build(cam, name="sphere_room", envmap=env)
 
 
+def adjoint_scene(width: int = 512, height: int = 512, seed: int = 1) -> Scene:
+    """Eight-material room for gradient checks: floor, ceiling + back wall,
+    left wall, right wall, tall box, short box, a GGX conductor sphere and the
+    quad light; hidden albedo and roughness U[0.1, 0.9] drawn from PCG64(seed).
+    Material order = parameter order."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = SceneBuilder()
+
+    def mat(**kw):
+        return b.material(default_material(tuple(rng.uniform(0.1, 0.9, size=3)),
+                                           roughness=float(rng.uniform(0.1, 0.9)), **kw))
+
+    floor, back, left, right, tall, short = (mat() for _ in range(6))
+    sphere = mat(metallic=1.0)
+    light = b.material(default_material((0.8, 0.8, 0.8), emissive=(17.0, 17.0, 17.0)))
+    b.add_quad((0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0), floor)
+    b.add_quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1), back)
+    b.add_quad((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), back)
+    b.add_quad((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1), left)
+    b.add_quad((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0), right)
+    b.add_box((0.18, 0.0, 0.18), (0.48, 0.6, 0.48), tall)
+    b.add_box((0.52, 0.0, 0.50), (0.82, 0.3, 0.80), short)
+    b.add_sphere((0.68, 0.42, 0.65), 0.12, sphere, 3)
+    y = 0.999
+    b.add_quad((0.375, y, 0.375), (0.625, y, 0.375), (0.625, y, 0.625), (0.375, y, 0.625), light)
+    cam = make_camera((0.5, 0.5, 2.2), (0.5, 0.5, 0.0), (0.0, 1.0, 0.0), 45.0, width / height)
+    return b.build(cam, name="adjoint8")
+
+
 def batch_workload(n_scenes: int = 64, seed: int = 1):
     """The batched benchmark: n scenes with triangle counts log-uniform in
     [2k, 200k], square resolutions from {256, 512, 1024}, photons per pass
