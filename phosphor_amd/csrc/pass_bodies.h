@@ -443,40 +443,52 @@ struct GatherArgs {
 
 /* photons of a cell whose records are loaded together before any is used:
  * one memory round trip per GATHER_BATCH photons instead of one or two each */
-constexpr u32 GATHER_BATCH = 4;
+constexpr u32 GATHER_BATCH = 8;
 
-/* gather_flux.cl:24-77 for pixel tid; the grid cell h of the pixel's scene is
- * entry bucket_base + h of cell_start / cell_end.  The arithmetic and the
- * order of the sums are those of photon_hash.h:65-100: cells in offset order,
- * photons of a cell in storage order (loads are batched, their use is not
- * reordered). */
-PHX_DEV void gather_pixel(const GatherArgs &a, u32 tid, const PhxPhotonHashData &d, f32 alpha, u32 bucket_base) {
-    F4 ge = ld(a.hp_emission + tid);
-    F4 rsf = ld(a.radiance + tid);
-    F4 r = mk(ge.x, ge.y, ge.z, 0.0f);
-    if (ge.w == 0.0f) {
-        st(a.radiance + tid, rsf + r);
-        if (a.rec_ratio)
-            a.rec_ratio[tid] = 1.0f;
-        return;
+/* The photons of one cell, records [s0, e0), that lie within the radius of
+ * a hitpoint (photon_hash.h:74-98): Gaussian weights and weighted powers added
+ * to nf / found in storage order. */
+PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32 r_sq, f32 two_r_sq, F4 &nf,
+                         f32 &found) {
+    for (u32 k = s0; k < e0; k += GATHER_BATCH) {
+        F4 pp[GATHER_BATCH];
+        u32 pw[GATHER_BATCH], pdr[GATHER_BATCH];
+#pragma unroll
+        for (u32 j = 0; j < GATHER_BATCH; j++) {
+            u32 kk = k + j < e0 ? k + j : e0 - 1; /* in range: k < e0 */
+            pp[j] = ld(a.ppos + kk);
+            pw[j] = a.ppow[kk];
+            pdr[j] = a.pdir[kk];
+        }
+#pragma unroll
+        for (u32 j = 0; j < GATHER_BATCH; j++) {
+            if (k + j >= e0)
+                break;
+            F4 dd = pos - pp[j];
+            f32 dsq = dot(dd, dd);
+            if (dsq >= r_sq)
+                continue;
+            F4 pd = decode_oct(pdr[j]);
+            if (dot(pd, sn) > 0.0f)
+                continue;
+            f32 wgt = GAUSS_ALPHA * (1.0f - (1.0f - phx_expf(-GAUSS_BETA * dsq / two_r_sq)) / GAUSS_DENOM);
+            nf = nf + wgt * decode_rgbe(pw[j]);
+            found += wgt;
+        }
     }
-    F4 dr = ld(a.hp_direct + tid);
-    r.x += dr.x;
-    r.y += dr.y;
-    r.z += dr.z;
-    st(a.radiance + tid, rsf + r);
-    F4 fsf = ld(a.flux + tid);
-    f32 wsf = a.photon_weight[tid];
-    f32 r_sq = fsf.w;
-    if (r_sq < EPS) {
-        f32 hx = d.cell_sizes.x * 0.5f, hy = d.cell_sizes.y * 0.5f, hz = d.cell_sizes.z * 0.5f;
-        r_sq = hx * hx + hy * hy + hz * hz;
-    }
-    F4 pos = ld(a.hp_pos + tid), sn = ld(a.hp_normal + tid);
-    F4 nf = BLACK;
-    f32 found = 0.0f;
-    f32 two_r_sq = 2.0f * r_sq;
-    const u32 *cs = a.cell_start + bucket_base, *ce = a.cell_end + bucket_base;
+}
+
+/* The 27-cell query of one lane (photon_hash.h:65-100).  A cell whose box
+ * lies farther than the radius from the hitpoint is skipped: no photon stored
+ * in it can pass the distance test.  The box is that of the cell get_hash
+ * returned; its gap to the hitpoint is shrunk by 1e-4 cell sizes per axis, far
+ * more than the rounding of a photon's cell assignment or of the distance, so
+ * a skipped cell never holds a photon the test would accept and the sums are
+ * unchanged.  (The cells are 1.155 radii wide when their half-diagonal is the
+ * radius, so about 16 of the 27 survive on average.) */
+PHX_DEV void gather_cells_lane(const GatherArgs &a, const u32 *cs, const u32 *ce, const PhxPhotonHashData &d, F4 pos,
+                               F4 sn, f32 r_sq, f32 two_r_sq, F4 &nf, f32 &found) {
+    const f32 mx = 1e-4f * d.cell_sizes.x, my = 1e-4f * d.cell_sizes.y, mz = 1e-4f * d.cell_sizes.z;
     for (u32 i = 0; i < 27; i++) {
         i32 cx = (i32)(i % 3) - 1, cy = (i32)((i / 3) % 3) - 1, cz = (i32)((i / 9) % 3) - 1;
         f32 qx = pos.x + (f32)cx * d.cell_sizes.x;
@@ -485,34 +497,53 @@ PHX_DEV void gather_pixel(const GatherArgs &a, u32 tid, const PhxPhotonHashData 
         u32 h = get_hash(qx, qy, qz, d);
         if (h == 0)
             continue;
-        u32 e0 = ce[h];
-        for (u32 k = cs[h]; k < e0; k += GATHER_BATCH) {
-            F4 pp[GATHER_BATCH];
-            u32 pw[GATHER_BATCH], pdr[GATHER_BATCH];
-#pragma unroll
-            for (u32 j = 0; j < GATHER_BATCH; j++) {
-                u32 kk = k + j < e0 ? k + j : e0 - 1; /* in range: k < e0 */
-                pp[j] = ld(a.ppos + kk);
-                pw[j] = a.ppow[kk];
-                pdr[j] = a.pdir[kk];
-            }
-#pragma unroll
-            for (u32 j = 0; j < GATHER_BATCH; j++) {
-                if (k + j >= e0)
-                    break;
-                F4 dd = pos - pp[j];
-                f32 dsq = dot(dd, dd);
-                if (dsq >= r_sq)
-                    continue;
-                F4 pd = decode_oct(pdr[j]);
-                if (dot(pd, sn) > 0.0f)
-                    continue;
-                f32 wgt = GAUSS_ALPHA * (1.0f - (1.0f - phx_expf(-GAUSS_BETA * dsq / two_r_sq)) / GAUSS_DENOM);
-                nf = nf + wgt * decode_rgbe(pw[j]);
-                found += wgt;
-            }
-        }
+        /* get_hash's cell indices, recovered from the same quotients */
+        u32 ix = (u32)((qx - d.origin.x) / d.cell_sizes.x);
+        u32 iy = (u32)((qy - d.origin.y) / d.cell_sizes.y);
+        u32 iz = (u32)((qz - d.origin.z) / d.cell_sizes.z);
+        f32 lx = d.origin.x + (f32)ix * d.cell_sizes.x, ly = d.origin.y + (f32)iy * d.cell_sizes.y,
+            lz = d.origin.z + (f32)iz * d.cell_sizes.z;
+        f32 gx = phx_fmaxf(0.0f, phx_fmaxf(lx - pos.x, pos.x - (lx + d.cell_sizes.x)) - mx);
+        f32 gy = phx_fmaxf(0.0f, phx_fmaxf(ly - pos.y, pos.y - (ly + d.cell_sizes.y)) - my);
+        f32 gz = phx_fmaxf(0.0f, phx_fmaxf(lz - pos.z, pos.z - (lz + d.cell_sizes.z)) - mz);
+        if (gx * gx + gy * gy + gz * gz >= r_sq)
+            continue;
+        gather_cell(a, cs[h], ce[h], pos, sn, r_sq, two_r_sq, nf, found);
     }
+}
+
+/* radiance update of gather_flux.cl:24-45 and the pixel's SPPM inputs;
+ * returns whether the pixel has a diffuse hitpoint to gather for */
+PHX_DEV bool gather_begin(const GatherArgs &a, u32 tid, const PhxPhotonHashData &d, F4 &pos, F4 &sn, F4 &fsf,
+                          f32 &wsf, f32 &r_sq) {
+    F4 ge = ld(a.hp_emission + tid);
+    F4 rsf = ld(a.radiance + tid);
+    F4 r = mk(ge.x, ge.y, ge.z, 0.0f);
+    if (ge.w == 0.0f) {
+        st(a.radiance + tid, rsf + r);
+        if (a.rec_ratio)
+            a.rec_ratio[tid] = 1.0f;
+        return false;
+    }
+    F4 dr = ld(a.hp_direct + tid);
+    r.x += dr.x;
+    r.y += dr.y;
+    r.z += dr.z;
+    st(a.radiance + tid, rsf + r);
+    fsf = ld(a.flux + tid);
+    wsf = a.photon_weight[tid];
+    r_sq = fsf.w;
+    if (r_sq < EPS) {
+        f32 hx = d.cell_sizes.x * 0.5f, hy = d.cell_sizes.y * 0.5f, hz = d.cell_sizes.z * 0.5f;
+        r_sq = hx * hx + hy * hy + hz * hz;
+    }
+    pos = ld(a.hp_pos + tid);
+    sn = ld(a.hp_normal + tid);
+    return true;
+}
+
+/* SPPM update of gather_flux.cl:65-77 */
+PHX_DEV void gather_end(const GatherArgs &a, u32 tid, f32 alpha, F4 fsf, f32 wsf, f32 r_sq, F4 nf, f32 found) {
     f32 ratio = 1.0f;
     if (found > 0.0f) {
         f32 npw = wsf + alpha * found;
@@ -528,6 +559,20 @@ PHX_DEV void gather_pixel(const GatherArgs &a, u32 tid, const PhxPhotonHashData 
         a.rec_ratio[tid] = ratio;
         a.rec_r2[tid] = r_sq;
     }
+}
+
+/* gather_flux.cl:24-77 for pixel tid, one lane on its own; the grid cell h
+ * of the pixel's scene is entry bucket_base + h of cell_start / cell_end. */
+PHX_DEV void gather_pixel(const GatherArgs &a, u32 tid, const PhxPhotonHashData &d, f32 alpha, u32 bucket_base) {
+    F4 pos, sn, fsf;
+    f32 wsf, r_sq;
+    if (!gather_begin(a, tid, d, pos, sn, fsf, wsf, r_sq))
+        return;
+    F4 nf = BLACK;
+    f32 found = 0.0f;
+    gather_cells_lane(a, a.cell_start + bucket_base, a.cell_end + bucket_base, d, pos, sn, r_sq, 2.0f * r_sq, nf,
+                      found);
+    gather_end(a, tid, alpha, fsf, wsf, r_sq, nf, found);
 }
 
 /* ---------------- workspace layouts (kernels_trace.hip) ---------------- */
