@@ -534,7 +534,7 @@ hipError_t launch_sort(const phx_float4 *pos_in, const u32 *pow_in, const u32 *d
  * assigned XCD by XCD so each L2 holds the photons of one image region.  Per
  * pixel the arithmetic and the order of the sums are those of
  * photon_hash.h:65-100. */
-__global__ __launch_bounds__(256) void k_gather(GatherArgs a, u32 width, u32 tiles_x) {
+template <bool DECODED> __global__ __launch_bounds__(256) void k_gather(GatherArgs a, u32 width, u32 tiles_x) {
     u32 tid;
     if (tiles_x == 0) { /* no image shape known: consecutive pixels */
         tid = blockIdx.x * 256 + threadIdx.x;
@@ -549,7 +549,28 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a, u32 width, u32 til
     }
     if (tid >= a.n_pixels)
         return;
-    gather_pixel(a, tid, a.d, a.alpha, 0u);
+    gather_pixel<DECODED>(a, tid, a.d, a.alpha, 0u);
+}
+
+/* decode_oct / decode_rgbe of every stored photon record, once per pass, for
+ * the gather (the same values it would decode at every visit) */
+__global__ __launch_bounds__(256) void k_decode_photons(const u32 *__restrict__ ppow, const u32 *__restrict__ pdir,
+                                                        const u32 *__restrict__ d_n, u32 max_n,
+                                                        phx_float4 *__restrict__ powf, phx_float4 *__restrict__ dirf) {
+    u32 n = d_n ? *d_n : max_n;
+    n = n < max_n ? n : max_n;
+    for (u32 i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        st(powf + i, decode_rgbe(ppow[i]));
+        st(dirf + i, decode_oct(pdir[i]));
+    }
+}
+
+hipError_t launch_decode_photons(const u32 *ppow, const u32 *pdir, const u32 *d_n, u32 max_n, phx_float4 *powf,
+                                 phx_float4 *dirf, hipStream_t s) {
+    u32 blocks = (max_n + 255) / 256;
+    blocks = blocks > 8192 ? 8192 : (blocks ? blocks : 1);
+    hipLaunchKernelGGL(k_decode_photons, dim3(blocks), dim3(256), 0, s, ppow, pdir, d_n, max_n, powf, dirf);
+    return hipGetLastError();
 }
 
 hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const phx_float4 *hp_throughput,
@@ -557,21 +578,28 @@ hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, 
                          u32 n_pixels, const phx_float4 *ppos, const u32 *ppow, const u32 *pdir, const u32 *cell_start,
                          const u32 *cell_end, const PhxPhotonHashData &d, f32 alpha, phx_float4 *flux,
                          f32 *photon_weight, phx_float4 *radiance, u32 width, hipStream_t s, f32 *rec_ratio,
-                         f32 *rec_r2) {
+                         f32 *rec_r2, const phx_float4 *pdirf, const phx_float4 *ppowf) {
     if (n_pixels == 0)
         return hipSuccess;
     GatherArgs a{hp_pos,     hp_normal, hp_throughput, hp_base_color, hp_direct, hp_emission,
                  n_pixels,   ppos,      ppow,          pdir,          cell_start, cell_end,
                  d,          alpha,     flux,          photon_weight, radiance,  rec_ratio,
-                 rec_r2};
+                 rec_r2,     pdirf,     ppowf};
+    bool dec = pdirf && ppowf;
     if (width == 0) { /* image shape unknown (C ABI call): flat pixel range */
-        hipLaunchKernelGGL(k_gather, dim3((n_pixels + 255) / 256), dim3(256), 0, s, a, 0u, 0u);
+        if (dec)
+            hipLaunchKernelGGL(k_gather<true>, dim3((n_pixels + 255) / 256), dim3(256), 0, s, a, 0u, 0u);
+        else
+            hipLaunchKernelGGL(k_gather<false>, dim3((n_pixels + 255) / 256), dim3(256), 0, s, a, 0u, 0u);
         return hipGetLastError();
     }
     u32 h = (n_pixels + width - 1) / width;
     u32 tiles_x = (width + 7) / 8, tiles_y = (h + 7) / 8;
     u32 waves = tiles_x * tiles_y;
-    hipLaunchKernelGGL(k_gather, dim3((waves + 3) / 4), dim3(256), 0, s, a, width, tiles_x);
+    if (dec)
+        hipLaunchKernelGGL(k_gather<true>, dim3((waves + 3) / 4), dim3(256), 0, s, a, width, tiles_x);
+    else
+        hipLaunchKernelGGL(k_gather<false>, dim3((waves + 3) / 4), dim3(256), 0, s, a, width, tiles_x);
     return hipGetLastError();
 }
 

@@ -439,17 +439,50 @@ struct GatherArgs {
     /* optional (adjoint forward): the pass's radius ratio (1 where the state
        is not updated) and r² per pixel */
     f32 *rec_ratio, *rec_r2;
+    /* optional: every stored record's direction and power already decoded
+       (decode_oct / decode_rgbe of pdir / ppow, the same values) */
+    const phx_float4 *pdirf, *ppowf;
 };
 
 /* photons of a cell whose records are loaded together before any is used:
- * one memory round trip per GATHER_BATCH photons instead of one or two each */
+ * one memory round trip per batch of photons instead of one or two each */
 constexpr u32 GATHER_BATCH = 8;
+constexpr u32 GATHER_BATCH_DECODED = 4;
 
 /* The photons of one cell, records [s0, e0), that lie within the radius of
  * a hitpoint (photon_hash.h:74-98): Gaussian weights and weighted powers added
- * to nf / found in storage order. */
+ * to nf / found in storage order.  DECODED reads the records' decoded
+ * direction and power instead of decoding them per visit. */
+template <bool DECODED>
 PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32 r_sq, f32 two_r_sq, F4 &nf,
                          f32 &found) {
+    if (DECODED) {
+        for (u32 k = s0; k < e0; k += GATHER_BATCH_DECODED) {
+            F4 pp[GATHER_BATCH_DECODED], pd[GATHER_BATCH_DECODED], pw[GATHER_BATCH_DECODED];
+#pragma unroll
+            for (u32 j = 0; j < GATHER_BATCH_DECODED; j++) {
+                u32 kk = k + j < e0 ? k + j : e0 - 1; /* in range: k < e0 */
+                pp[j] = ld(a.ppos + kk);
+                pd[j] = ld(a.pdirf + kk);
+                pw[j] = ld(a.ppowf + kk);
+            }
+#pragma unroll
+            for (u32 j = 0; j < GATHER_BATCH_DECODED; j++) {
+                if (k + j >= e0)
+                    break;
+                F4 dd = pos - pp[j];
+                f32 dsq = dot(dd, dd);
+                if (dsq >= r_sq)
+                    continue;
+                if (dot(pd[j], sn) > 0.0f)
+                    continue;
+                f32 wgt = GAUSS_ALPHA * (1.0f - (1.0f - phx_expf(-GAUSS_BETA * dsq / two_r_sq)) / GAUSS_DENOM);
+                nf = nf + wgt * pw[j];
+                found += wgt;
+            }
+        }
+        return;
+    }
     for (u32 k = s0; k < e0; k += GATHER_BATCH) {
         F4 pp[GATHER_BATCH];
         u32 pw[GATHER_BATCH], pdr[GATHER_BATCH];
@@ -486,6 +519,7 @@ PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32
  * a skipped cell never holds a photon the test would accept and the sums are
  * unchanged.  (The cells are 1.155 radii wide when their half-diagonal is the
  * radius, so about 16 of the 27 survive on average.) */
+template <bool DECODED>
 PHX_DEV void gather_cells_lane(const GatherArgs &a, const u32 *cs, const u32 *ce, const PhxPhotonHashData &d, F4 pos,
                                F4 sn, f32 r_sq, f32 two_r_sq, F4 &nf, f32 &found) {
     const f32 mx = 1e-4f * d.cell_sizes.x, my = 1e-4f * d.cell_sizes.y, mz = 1e-4f * d.cell_sizes.z;
@@ -508,7 +542,7 @@ PHX_DEV void gather_cells_lane(const GatherArgs &a, const u32 *cs, const u32 *ce
         f32 gz = phx_fmaxf(0.0f, phx_fmaxf(lz - pos.z, pos.z - (lz + d.cell_sizes.z)) - mz);
         if (gx * gx + gy * gy + gz * gz >= r_sq)
             continue;
-        gather_cell(a, cs[h], ce[h], pos, sn, r_sq, two_r_sq, nf, found);
+        gather_cell<DECODED>(a, cs[h], ce[h], pos, sn, r_sq, two_r_sq, nf, found);
     }
 }
 
@@ -563,6 +597,7 @@ PHX_DEV void gather_end(const GatherArgs &a, u32 tid, f32 alpha, F4 fsf, f32 wsf
 
 /* gather_flux.cl:24-77 for pixel tid, one lane on its own; the grid cell h
  * of the pixel's scene is entry bucket_base + h of cell_start / cell_end. */
+template <bool DECODED>
 PHX_DEV void gather_pixel(const GatherArgs &a, u32 tid, const PhxPhotonHashData &d, f32 alpha, u32 bucket_base) {
     F4 pos, sn, fsf;
     f32 wsf, r_sq;
@@ -570,8 +605,8 @@ PHX_DEV void gather_pixel(const GatherArgs &a, u32 tid, const PhxPhotonHashData 
         return;
     F4 nf = BLACK;
     f32 found = 0.0f;
-    gather_cells_lane(a, a.cell_start + bucket_base, a.cell_end + bucket_base, d, pos, sn, r_sq, 2.0f * r_sq, nf,
-                      found);
+    gather_cells_lane<DECODED>(a, a.cell_start + bucket_base, a.cell_end + bucket_base, d, pos, sn, r_sq, 2.0f * r_sq,
+                               nf, found);
     gather_end(a, tid, alpha, fsf, wsf, r_sq, nf, found);
 }
 
