@@ -327,6 +327,8 @@ typedef struct PhxRenderArgs {
     float initial_radius;           /* 0 → reference rule (cell half-diagonal, gather_flux.cl:51-56) */
     uint32_t trace_mode;            /* 0 = wavefront trace (default), 1 = one-kernel trace; same results */
     uint32_t flags;                 /* PHX_PASS_* */
+    uint32_t emit_wave_bounces;     /* bounce-synchronous emission launches before the finishing one
+                                       (0 = default 5); same results for every value */
 } PhxRenderArgs;
 
 /* Capture the pass into a hipGraph on the first run_pass and replay it. */

@@ -62,6 +62,7 @@ class RenderArgs(C.Structure):
         ("max_photons_per_batch", C.c_uint32), ("max_photon_bounces", C.c_uint32), ("grid_res", C.c_uint32),
         ("direct_samples", C.c_uint32), ("seed", C.c_uint32), ("sppm_alpha", C.c_float),
         ("initial_radius", C.c_float), ("trace_mode", C.c_uint32), ("flags", C.c_uint32),
+        ("emit_wave_bounces", C.c_uint32),
     ]
 
 

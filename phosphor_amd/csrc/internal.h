@@ -77,7 +77,7 @@ size_t emit_workspace_bytes(uint32_t photons_to_emit, uint32_t max_bounces);
 hipError_t launch_emit(const SceneArgs &sc, uint32_t max_photons, uint32_t offset, uint32_t photons_to_emit,
                        uint32_t max_bounces, uint32_t seed, phx_float4 *photon_pos, uint32_t *photon_power,
                        uint32_t *photon_dir, uint32_t *n_photons, void *ws, hipStream_t s,
-                       const uint32_t *d_seed = nullptr);
+                       const uint32_t *d_seed = nullptr, uint32_t wave_bounces = 0);
 
 hipError_t launch_hash(const phx_float4 *pos, uint32_t n, const PhxPhotonHashData &d, uint32_t *hashes,
                        uint32_t *counts, hipStream_t s);

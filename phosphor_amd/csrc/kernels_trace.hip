@@ -581,7 +581,7 @@ hipError_t launch_emit_compact(const EmitLayout &L, u32 photons, u32 max_photons
 
 hipError_t launch_emit(const SceneArgs &sc, u32 max_photons, u32 offset, u32 photons, u32 max_bounces, u32 seed,
                        phx_float4 *photon_pos, u32 *photon_power, u32 *photon_dir, u32 *n_photons, void *ws,
-                       hipStream_t s, const u32 *d_seed) {
+                       hipStream_t s, const u32 *d_seed, u32 wave_bounces) {
     if (photons == 0)
         return hipSuccess;
     EmitLayout L = emit_layout(ws, photons, max_bounces);
@@ -609,7 +609,9 @@ hipError_t launch_emit(const SceneArgs &sc, u32 max_photons, u32 offset, u32 pho
         blocks = 4096;
     /* bounce-synchronous launches while most photons are alive, then one
        launch for the rest of every path */
-    u32 wave_bounces = max_bounces < EMIT_WF_BOUNCES ? max_bounces : EMIT_WF_BOUNCES;
+    if (wave_bounces == 0)
+        wave_bounces = EMIT_WF_BOUNCES;
+    wave_bounces = max_bounces < wave_bounces ? max_bounces : wave_bounces;
     for (u32 depth = 0; depth < wave_bounces; depth++) {
         a.depth = depth;
         hipLaunchKernelGGL(k_emit_wf_bounce, dim3(blocks), dim3(EMIT_WF_BLOCK), 0, s, a);

@@ -57,6 +57,8 @@ int check_render_args(const PhxRenderArgs &a, const char *who) {
         return fail(PHX_ERR_INVALID_ARG, w + ": direct_samples outside 1..128 (cmd_args.hpp:59)");
     if (a.flags & ~(uint32_t)PHX_PASS_GRAPH)
         return fail(PHX_ERR_INVALID_ARG, w + ": unknown flags");
+    if (a.emit_wave_bounces > 32)
+        return fail(PHX_ERR_INVALID_ARG, w + ": emit_wave_bounces outside 0..32");
     return PHX_OK;
 }
 } // namespace phx
@@ -168,7 +170,7 @@ int PhxRenderer::emit_photons() {
         uint32_t to_emit = std::min(per_batch, ppr - off);
         PHX_TRY(launch_emit(sc, photon_buf_size, off, to_emit, args.max_photon_bounces, 0u,
                             d_photon_pos.as<phx_float4>(), d_photon_power.as<uint32_t>(), d_photon_dir.as<uint32_t>(),
-                            d_batch_size.as<uint32_t>(), d_emit_ws.p, stream, seed_ptr()),
+                            d_batch_size.as<uint32_t>(), d_emit_ws.p, stream, seed_ptr(), args.emit_wave_bounces),
                 "emit_photons");
     }
     PHX_TRY(launch_note_count(d_batch_size.as<uint32_t>(), d_info.as<uint32_t>(), stream), "emit_photons: count");

@@ -1,0 +1,101 @@
+"""Autotuner of the pass driver's settings for one scene and configuration.
+
+Two kinds of knobs:
+
+* exact knobs change how the pass is launched, never its results: trace form
+  (wavefront or one kernel), emission batch size (``--batch-size``,
+  cmd_args.hpp:42) and the number of bounce-synchronous emission launches.
+  A candidate is accepted only if its SPPM state after the measured passes is
+  bit-identical to the state under the given settings.
+* the grid resolution (``--grid-res``, cmd_args.hpp:51) changes the hash
+  keys; with an explicit initial radius a grid whose cells are at least one
+  radius wide finds the same photons, so only the order of the gather's sums
+  changes.  Such a candidate is accepted when the estimate's relative RMSE
+  against the given settings is within ``rmse_tol``.  With the reference
+  radius rule (initial_radius 0) the grid sets the radius, so it is not
+  tuned.
+
+The search is coordinate-wise (one knob at a time, keeping the best so far),
+each candidate timed over ``passes`` passes after ``warmup`` passes with the
+renderer's own pass events.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from . import PassConfig, Renderer
+from .output import relative_rmse
+
+
+def _run(cfg: PassConfig, scene, warmup: int, passes: int):
+    r = Renderer(cfg, scene)
+    times = []
+    for i in range(warmup + passes):
+        r.run_pass()
+        r.synchronize()
+        if i >= warmup:
+            times.append(r.pass_ms())
+    flux, weight, rad = r.read_state()
+    est = r.estimate()
+    r.close()
+    return float(np.median(times)), (flux, weight, rad), est
+
+
+def _same_state(a, b) -> bool:
+    return all(np.array_equal(np.ascontiguousarray(x).view(np.uint32), np.ascontiguousarray(y).view(np.uint32))
+               for x, y in zip(a, b))
+
+
+def grid_for_cell(extent: float, cell: float) -> int:
+    return max(1, min(255, int(math.floor(extent / cell))))
+
+
+@dataclasses.dataclass
+class TuneResult:
+    config: PassConfig
+    ms_per_pass: float
+    baseline_ms: float
+    trials: list
+
+    @property
+    def speedup(self) -> float:
+        return self.baseline_ms / self.ms_per_pass
+
+
+def autotune(cfg: PassConfig, scene, passes: int = 3, warmup: int = 1, wave_bounces=(1, 2, 3, 4, 6),
+             batch_sizes=None, trace_modes=(0, 1), cell_factors=(1.0, 1.155, 1.5, 2.0),
+             rmse_tol: float = 1e-4) -> TuneResult:
+    """Returns the fastest accepted configuration and every trial
+    (knob, value, ms per pass, exact, relative RMSE, accepted)."""
+    base_ms, base_state, base_est = _run(cfg, scene, warmup, passes)
+    best, best_ms = dataclasses.replace(cfg), base_ms
+    trials = [("baseline", None, base_ms, True, 0.0, True)]
+
+    def consider(knob, values, exact):
+        nonlocal best, best_ms
+        for v in values:
+            cand = dataclasses.replace(best, **{knob: v})
+            if cand == best:
+                continue
+            ms, state, est = _run(cand, scene, warmup, passes)
+            rm = relative_rmse(est, base_est)
+            same = _same_state(state, base_state)
+            ok = same if exact else rm <= rmse_tol
+            trials.append((knob, v, ms, same, rm, ok))
+            if ok and ms < best_ms:
+                best, best_ms = cand, ms
+
+    consider("trace_mode", trace_modes, True)
+    consider("emit_wave_bounces", [w for w in wave_bounces if w <= cfg.max_photon_bounces], True)
+    if batch_sizes is None:
+        batch_sizes = sorted({min(cfg.photons_per_round, b) for b in (1 << 18, 1 << 20, cfg.photons_per_round)})
+    consider("max_photons_per_batch", batch_sizes, True)
+    if cfg.initial_radius > 0.0:
+        lo, hi = scene.bounds()
+        extent = float((hi - lo).max()) + 2e-3
+        grids = [grid_for_cell(extent, f * cfg.initial_radius) for f in cell_factors]
+        consider("grid_res", sorted(set(grids)), False)
+    return TuneResult(best, best_ms, base_ms, trials)
