@@ -99,3 +99,27 @@ def autotune(cfg: PassConfig, scene, passes: int = 3, warmup: int = 1, wave_boun
         grids = [grid_for_cell(extent, f * cfg.initial_radius) for f in cell_factors]
         consider("grid_res", sorted(set(grids)), False)
     return TuneResult(best, best_ms, base_ms, trials)
+
+
+def main(argv=None) -> int:
+    """python -m phosphor_amd.tune --workload caustic: tunes a named workload
+    and prints every trial and the chosen configuration."""
+    import argparse
+
+    from . import config_for
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--workload", default="caustic")
+    ap.add_argument("--passes", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    args = ap.parse_args(argv)
+    cfg, scene, _ = config_for(args.workload)
+    res = autotune(cfg, scene, passes=args.passes, warmup=args.warmup, wave_bounces=(1, 2, 3, 4, 5, 6, 8))
+    for t in res.trials:
+        print(t, flush=True)
+    print("best", res.config, res.ms_per_pass, "baseline", res.baseline_ms, "speedup", res.speedup)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
