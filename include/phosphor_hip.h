@@ -333,6 +333,10 @@ typedef struct PhxRenderArgs {
 
 /* Capture the pass into a hipGraph on the first run_pass and replay it. */
 #define PHX_PASS_GRAPH 1u
+/* Visit BVH nodes in the reference's depth-first order (hit.h:136-172) even
+ * where the near-first order finds the same hits (a BVH no deeper than 22
+ * interior levels); results are identical either way. */
+#define PHX_DFS_ORDER 2u
 
 typedef struct PhxRenderer PhxRenderer;
 
@@ -384,7 +388,7 @@ typedef struct PhxBatch PhxBatch;
 typedef struct PhxBatchOptions {
     uint32_t trace_group_pixels; /* pixels traced per group (0 → 2 Mi) */
     uint32_t emit_group_photons; /* photons emitted per group (0 → 8 Mi) */
-    uint32_t flags;              /* PHX_PASS_GRAPH */
+    uint32_t flags;              /* PHX_PASS_GRAPH | PHX_DFS_ORDER */
     uint32_t reserved;
 } PhxBatchOptions;
 

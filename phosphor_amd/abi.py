@@ -67,6 +67,7 @@ class RenderArgs(C.Structure):
 
 
 PHX_PASS_GRAPH = 1
+PHX_DFS_ORDER = 2
 
 
 class BatchOptions(C.Structure):

@@ -110,6 +110,8 @@ int PhxBatch::build(const PhxSceneDesc *descs) {
         BatchScene &B = table[i];
         std::memset(&B, 0, sizeof(B));
         B.sc = scenes[i]->args();
+        if ((opt.flags | a.flags) & PHX_DFS_ORDER)
+            B.sc.node_order = nullptr;
         B.cam = *descs[i].camera;
         phx_build_photon_hash_info(&scenes[i]->root, a.grid_res, &B.hash);
         B.width = a.width;
@@ -273,7 +275,7 @@ int phx_batch_create(const PhxRenderArgs *args, const PhxSceneDesc *scenes, uint
             return fail(PHX_ERR_INVALID_ARG, "phx_batch_create: scene " + std::to_string(i) + ": " +
                                                  std::string(phx_last_error()));
     }
-    if (options && (options->flags & ~(uint32_t)PHX_PASS_GRAPH))
+    if (options && (options->flags & ~(uint32_t)(PHX_PASS_GRAPH | PHX_DFS_ORDER)))
         return fail(PHX_ERR_INVALID_ARG, "phx_batch_create: unknown flags");
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)

@@ -345,8 +345,75 @@ PHX_DEV bool traverse(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, Hit
     return found;
 }
 
+/* The closest hit of traverse<false>, found in a different node order: at
+ * an interior node the child that lies nearer along the ray on the node's
+ * split axis (sc.node_order) is entered first, so the closest hit tends to be
+ * found early and more of the other subtrees fail their box test.
+ *
+ * Why the result is the same: the reference's depth-first search returns the
+ * hit with the least t, and among hits at that same t the one met last, i.e.
+ * the one in the leaf of largest preorder index (is_triangle_hit accepts
+ * t <= t_max, so a later equal hit overwrites an earlier one, and nodes are
+ * stored in preorder).  Leaf boxes are expanded by 1e-3 (bvh.cpp), so a node
+ * holding a hit at t <= closest has its entry below t and passes the slab
+ * test whenever it is tested; a node is dropped only when its entry is at or
+ * beyond closest, where it holds no hit at or below closest.  So the least t,
+ * with ties going to the larger leaf index, is the reference's hit whatever
+ * the order -- provided the reference loses nothing to its stack-overflow rule
+ * (hit.h:163).  The host sets node_order only when no interior node is deeper
+ * than 22: then neither search can overflow (both keep one pending sibling
+ * per interior ancestor at most). */
+PHX_DEV bool traverse_near_first(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
+                                 u32 stride) {
+    f32 ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+    u32 neg = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    f32 closest = t_max;
+    bool found = false;
+    u32 best = 0;
+    int q = -1;
+    u32 index = 0;
+    for (;;) {
+        const PhxBvhNode *node = sc.nodes + index;
+        F4 nmin = ld(&node->min), nmax = ld(&node->max);
+        u32 hint = sc.node_order[index];
+        f32 lo = t_min, hi = closest;
+        if (slab(nmin.x, nmax.x, o.x, ix, lo, hi) && slab(nmin.y, nmax.y, o.y, iy, lo, hi) &&
+            slab(nmin.z, nmax.z, o.z, iz, lo, hi)) {
+            u32 tri = phx_as_uint(nmin.w);
+            if (tri != 0xFFFFFFFFu) {
+                HitRecord cand;
+                if (is_triangle_hit(sc, tri, o, d, t_min, closest, cand) &&
+                    (cand.t < closest || !found || index > best)) {
+                    closest = cand.t;
+                    rec = cand;
+                    best = index;
+                    found = true;
+                }
+            } else {
+                /* right first when the ray runs against the left-to-right
+                   direction of the split axis */
+                bool right_first = (((neg >> (hint & 3u)) ^ (hint >> 2)) & 1u) != 0;
+                u32 l = index + 1, r = phx_as_uint(nmax.w);
+                q++;
+                stk[q * stride] = right_first ? l : r;
+                index = right_first ? r : l;
+                continue;
+            }
+        }
+        if (q < 0)
+            break;
+        index = stk[q * stride];
+        q--;
+    }
+    if (found)
+        rec.mat_index = sc.tmat[rec.tri_index];
+    return found;
+}
+
 PHX_DEV bool scene_intersect(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
                              u32 stride) {
+    if (sc.node_order)
+        return traverse_near_first(sc, o, d, t_min, t_max, rec, stk, stride);
     return traverse<false>(sc, o, d, t_min, t_max, rec, stk, stride);
 }
 

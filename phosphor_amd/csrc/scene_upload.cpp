@@ -51,6 +51,7 @@ SceneArgs SceneBuffers::args() const {
     a.env_w = s.envmap_width;
     a.env_h = s.envmap_height;
     a.has_env = s.has_envmap;
+    a.node_order = shallow_bvh ? g.d_node_order.as<uint8_t>() : nullptr;
     return a;
 }
 
@@ -87,6 +88,8 @@ int upload_scene(const PhxSceneDesc &sc, hipStream_t s, SceneBuffers &out) {
         for (uint32_t i = 0; i < sc.n_materials; i++)
             transmissive = transmissive || !(sc.materials[i].transmission < 1e-5f);
         uint32_t max_depth = 0;
+        std::vector<uint8_t> &order = out.geometry.node_order;
+        order.assign(nodes.size(), 0);
         std::vector<std::pair<uint32_t, uint32_t>> todo{{0u, 0u}};
         while (!todo.empty()) {
             auto [i, dpt] = todo.back();
@@ -94,10 +97,21 @@ int upload_scene(const PhxSceneDesc &sc, hipStream_t s, SceneBuffers &out) {
             if (phx_as_uint(nodes[i].min.w) != 0xFFFFFFFFu)
                 continue;
             max_depth = std::max(max_depth, dpt);
-            todo.push_back({i + 1, dpt + 1});
-            todo.push_back({phx_as_uint(nodes[i].max.w), dpt + 1});
+            uint32_t l = i + 1, r = phx_as_uint(nodes[i].max.w);
+            if (l >= nodes.size() || r >= nodes.size() || r <= i || dpt >= nodes.size())
+                return fail(PHX_ERR_INVALID_ARG, "set_scene: BVH child index out of range");
+            /* the builder's split axis (longest box extent, bvh.cpp:18-84) and
+               whether the left child starts above the right one along it */
+            const PhxBvhNode &nd = nodes[i];
+            float ex = nd.max.x - nd.min.x, ey = nd.max.y - nd.min.y, ez = nd.max.z - nd.min.z;
+            int axis = ex > ey ? (ex > ez ? 0 : 2) : (ey > ez ? 1 : 2);
+            auto lo = [axis](const PhxBvhNode &b) { return axis == 0 ? b.min.x : axis == 1 ? b.min.y : b.min.z; };
+            order[i] = (uint8_t)(axis | (lo(nodes[l]) > lo(nodes[r]) ? 4 : 0));
+            todo.push_back({l, dpt + 1});
+            todo.push_back({r, dpt + 1});
         }
         out.shadow_any_hit = (!transmissive && max_depth <= 22) ? 1u : 0u;
+        out.shallow_bvh = max_depth <= 22 ? 1u : 0u;
     }
 
     GeometryBuffers &g = out.geometry;
@@ -113,6 +127,7 @@ int upload_scene(const PhxSceneDesc &sc, hipStream_t s, SceneBuffers &out) {
     PHX_TRY(copy_to_dev_buf(g.d_etri_uvs, sc.etri_uvs, ne, sizeof(PhxTriangleUvs), s), "upload");
     PHX_TRY(copy_to_dev_buf(g.d_etri_mat_index, sc.etri_mat_index, ne, sizeof(uint32_t), s), "upload");
     PHX_TRY(copy_to_dev_buf(g.d_bvh_nodes, nodes.data(), nodes.size(), sizeof(PhxBvhNode), s), "upload");
+    PHX_TRY(copy_to_dev_buf(g.d_node_order, g.node_order.data(), g.node_order.size(), 1, s), "upload");
 
     ShadingBuffers &sh = out.shading;
     sh.n_materials = sc.n_materials;

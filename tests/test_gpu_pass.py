@@ -58,6 +58,38 @@ def test_passes_bit_exact(case):
     gpu.close()
 
 
+def _near_first_cases():
+    return [
+        ("caustic level 5", lambda: scenes.caustic_scene(128, 96, level=5), 128, 96, 87, 0.01),
+        ("atrium 60k", lambda: scenes.atrium_scene(128, 72, target_tris=60_000, env_size=(256, 128)), 128, 72, 64,
+         0.0),
+        ("sphere room 40k", lambda: scenes.sphere_room(112, 80, 40_000, np.random.default_rng(3)), 112, 80, 64, 0.0),
+    ]
+
+
+@pytest.mark.parametrize("index", [0, 1, 2])
+def test_near_first_traversal_matches_depth_first(index):
+    """Closest-hit searches visit BVH nodes near-first when the tree is no
+    deeper than 22 interior levels; PHX_DFS_ORDER forces the reference's
+    depth-first order.  Hitpoints, photon map and SPPM state are identical."""
+    name, make, w, h, grid, radius = _near_first_cases()[index]
+    scene = make()
+    base = PassConfig(width=w, height=h, photons_per_round=1 << 16,
+                      max_photons_per_batch=1 << 16, max_photon_bounces=16, grid_res=grid, seed=11,
+                      initial_radius=radius)
+    print(f"case {name}: {scene.n_triangles} triangles", flush=True)
+    near = Renderer(base, scene)
+    dfs = Renderer(PassConfig(**{**base.__dict__, "dfs_order": True}), scene)
+    for p in range(2):
+        near.run_pass()
+        dfs.run_pass()
+        near.synchronize()
+        dfs.synchronize()
+        _compare_pass(near, dfs, f"{name} pass {p}")
+    near.close()
+    dfs.close()
+
+
 def test_reference_radius_rule_and_determinism():
     """initial_radius 0 → the reference's cell half-diagonal rule (gather_flux.cl:51-56);
     two renders with the same seed are identical."""
