@@ -111,7 +111,7 @@ int PhxBatch::build(const PhxSceneDesc *descs) {
         std::memset(&B, 0, sizeof(B));
         B.sc = scenes[i]->args();
         if ((opt.flags | a.flags) & PHX_DFS_ORDER)
-            B.sc.node_order = nullptr;
+            B.sc.near_first = 0;
         B.cam = *descs[i].camera;
         phx_build_photon_hash_info(&scenes[i]->root, a.grid_res, &B.hash);
         B.width = a.width;

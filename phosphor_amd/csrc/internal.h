@@ -41,11 +41,11 @@ struct SceneArgs {
     float scene_radius;
     const float *envmap;
     uint32_t env_w, env_h, has_env;
-    /* Per BVH node: split axis (bits 0-1), left child above the right one
-     * (bit 2).  Set only when no interior node is deeper than 22, so that the
-     * closest-hit search never overflows its stack and may visit the child
-     * nearer along the ray first; null keeps the depth-first order. */
-    const uint8_t *node_order;
+    /* 1: interior BVH nodes carry a visiting-order hint in min.w and none is
+     * deeper than 22, so the closest-hit search never overflows its stack and
+     * may visit the child nearer along the ray first; 0 keeps the
+     * depth-first order. */
+    uint32_t near_first;
 };
 
 /* ---- scan (single pass, decoupled look-back) ---- */

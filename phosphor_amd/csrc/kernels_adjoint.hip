@@ -326,7 +326,7 @@ PHX_DEV F4 shadow_trans_adj(const SceneView &sc, F4 o, F4 d, f32 t_max, u32 *stk
     n_occ = 0;
     for (u32 depth = 0; depth < MAX_RAY_BOUNCES; depth++) {
         HitRecord rec;
-        if (!traverse<false>(sc, o, d, t_min, t_max, rec, stk, ADJ_BLOCK))
+        if (!scene_intersect(sc, o, d, t_min, t_max, rec, stk, ADJ_BLOCK))
             break;
         const PhxMaterial &mat = sc.materials[rec.mat_index];
         if (mat.transmission < EPS)

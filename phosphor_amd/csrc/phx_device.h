@@ -38,6 +38,10 @@ constexpr f32 INF = 1e9f;
 constexpr f32 PI = 3.141592654f;
 constexpr f32 U32_MAX_F = 4294967296.0f;
 constexpr u32 NO_TEXTURE = 0xFFFFFFFFu;
+/* min.w of a BVH node: a triangle index below this marks a leaf.  The
+ * reference marks interior nodes with 0xFFFFFFFF; uploaded scenes may carry
+ * a visiting-order hint in the low three bits instead (0xFFFFFFF8 | hint). */
+constexpr u32 BVH_INTERIOR = 0xFFFFFFF8u;
 constexpr f32 AIR_IOR = 1.0f;
 constexpr u32 MAX_RAY_BOUNCES = 5;
 constexpr int BVH_STACK_SIZE = 24;
@@ -319,7 +323,7 @@ PHX_DEV bool traverse(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, Hit
                    slab(nmin.z, nmax.z, o.z, iz, lo, hi);
         if (box) {
             u32 tri = phx_as_uint(nmin.w);
-            if (tri != 0xFFFFFFFFu) {
+            if (tri < BVH_INTERIOR) {
                 HitRecord cand;
                 if (is_triangle_hit(sc, tri, o, d, t_min, closest, cand)) {
                     closest = cand.t;
@@ -347,7 +351,8 @@ PHX_DEV bool traverse(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, Hit
 
 /* The closest hit of traverse<false>, found in a different node order: at
  * an interior node the child that lies nearer along the ray on the node's
- * split axis (sc.node_order) is entered first, so the closest hit tends to be
+ * split axis (hint bits 0-1 of min.w, bit 2 set when the left child lies
+ * above the right one) is entered first, so the closest hit tends to be
  * found early and more of the other subtrees fail their box test.
  *
  * Why the result is the same: the reference's depth-first search returns the
@@ -360,7 +365,7 @@ PHX_DEV bool traverse(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, Hit
  * beyond closest, where it holds no hit at or below closest.  So the least t,
  * with ties going to the larger leaf index, is the reference's hit whatever
  * the order -- provided the reference loses nothing to its stack-overflow rule
- * (hit.h:163).  The host sets node_order only when no interior node is deeper
+ * (hit.h:163).  The host sets near_first only when no interior node is deeper
  * than 22: then neither search can overflow (both keep one pending sibling
  * per interior ancestor at most). */
 PHX_DEV bool traverse_near_first(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
@@ -375,12 +380,11 @@ PHX_DEV bool traverse_near_first(const SceneView &sc, F4 o, F4 d, f32 t_min, f32
     for (;;) {
         const PhxBvhNode *node = sc.nodes + index;
         F4 nmin = ld(&node->min), nmax = ld(&node->max);
-        u32 hint = sc.node_order[index];
         f32 lo = t_min, hi = closest;
         if (slab(nmin.x, nmax.x, o.x, ix, lo, hi) && slab(nmin.y, nmax.y, o.y, iy, lo, hi) &&
             slab(nmin.z, nmax.z, o.z, iz, lo, hi)) {
             u32 tri = phx_as_uint(nmin.w);
-            if (tri != 0xFFFFFFFFu) {
+            if (tri < BVH_INTERIOR) {
                 HitRecord cand;
                 if (is_triangle_hit(sc, tri, o, d, t_min, closest, cand) &&
                     (cand.t < closest || !found || index > best)) {
@@ -392,6 +396,7 @@ PHX_DEV bool traverse_near_first(const SceneView &sc, F4 o, F4 d, f32 t_min, f32
             } else {
                 /* right first when the ray runs against the left-to-right
                    direction of the split axis */
+                u32 hint = tri & 7u;
                 bool right_first = (((neg >> (hint & 3u)) ^ (hint >> 2)) & 1u) != 0;
                 u32 l = index + 1, r = phx_as_uint(nmax.w);
                 q++;
@@ -412,7 +417,7 @@ PHX_DEV bool traverse_near_first(const SceneView &sc, F4 o, F4 d, f32 t_min, f32
 
 PHX_DEV bool scene_intersect(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
                              u32 stride) {
-    if (sc.node_order)
+    if (sc.near_first)
         return traverse_near_first(sc, o, d, t_min, t_max, rec, stk, stride);
     return traverse<false>(sc, o, d, t_min, t_max, rec, stk, stride);
 }

@@ -7,7 +7,6 @@
 #define PHX_SCENE_BUFFERS_H
 
 #include <cstring>
-#include <vector>
 
 #include "internal.h"
 
@@ -44,10 +43,6 @@ struct GeometryBuffers { /* include/buffers/geometry.hpp */
     DevBuf d_etri_verts, d_etri_normals, d_etri_uvs, d_etri_mat_index;
     uint32_t n_e_triangles = 0;
     DevBuf d_bvh_nodes;
-    /* per node: split axis (bits 0-1) and "left child lies above" (bit 2),
-       the visiting-order hint of the near-first closest-hit search */
-    DevBuf d_node_order;
-    std::vector<uint8_t> node_order;
 };
 
 struct ShadingBuffers { /* include/buffers/shading.hpp */
@@ -73,8 +68,9 @@ struct SceneBuffers {
        to shadow rays (light_sampling.h:316) and no interior BVH node is deeper
        than 22, so the traversal stack never overflows (hit.h:163) */
     uint32_t shadow_any_hit = 0;
-    /* no interior BVH node is deeper than 22: closest-hit searches may visit
-       nodes near-first (same hits, SceneArgs::node_order) */
+    /* no interior BVH node is deeper than 22 and the uploaded interior nodes
+       carry visiting-order hints: closest-hit searches may visit nodes
+       near-first (same hits, SceneArgs::near_first) */
     uint32_t shallow_bvh = 0;
 
     SceneArgs args() const;

@@ -51,7 +51,7 @@ SceneArgs SceneBuffers::args() const {
     a.env_w = s.envmap_width;
     a.env_h = s.envmap_height;
     a.has_env = s.has_envmap;
-    a.node_order = shallow_bvh ? g.d_node_order.as<uint8_t>() : nullptr;
+    a.near_first = shallow_bvh;
     return a;
 }
 
@@ -59,6 +59,8 @@ int check_scene_desc(const PhxSceneDesc &sc, const char *who) {
     if (sc.n_triangles == 0 || !sc.tri_verts || !sc.tri_normals || !sc.tri_tangents || !sc.tri_uvs ||
         !sc.tri_mat_index || !sc.materials || sc.n_materials == 0 || !sc.camera)
         return fail(PHX_ERR_INVALID_ARG, std::string(who) + ": empty or incomplete scene (main.cpp:40-47)");
+    if (sc.n_triangles >= 0x80000000u)
+        return fail(PHX_ERR_INVALID_ARG, std::string(who) + ": at most 2^31 - 1 triangles");
     return PHX_OK;
 }
 
@@ -88,8 +90,7 @@ int upload_scene(const PhxSceneDesc &sc, hipStream_t s, SceneBuffers &out) {
         for (uint32_t i = 0; i < sc.n_materials; i++)
             transmissive = transmissive || !(sc.materials[i].transmission < 1e-5f);
         uint32_t max_depth = 0;
-        std::vector<uint8_t> &order = out.geometry.node_order;
-        order.assign(nodes.size(), 0);
+        std::vector<std::pair<uint32_t, uint32_t>> hints;
         std::vector<std::pair<uint32_t, uint32_t>> todo{{0u, 0u}};
         while (!todo.empty()) {
             auto [i, dpt] = todo.back();
@@ -106,12 +107,17 @@ int upload_scene(const PhxSceneDesc &sc, hipStream_t s, SceneBuffers &out) {
             float ex = nd.max.x - nd.min.x, ey = nd.max.y - nd.min.y, ez = nd.max.z - nd.min.z;
             int axis = ex > ey ? (ex > ez ? 0 : 2) : (ey > ez ? 1 : 2);
             auto lo = [axis](const PhxBvhNode &b) { return axis == 0 ? b.min.x : axis == 1 ? b.min.y : b.min.z; };
-            order[i] = (uint8_t)(axis | (lo(nodes[l]) > lo(nodes[r]) ? 4 : 0));
+            hints.push_back({i, (uint32_t)axis | (lo(nodes[l]) > lo(nodes[r]) ? 4u : 0u)});
             todo.push_back({l, dpt + 1});
             todo.push_back({r, dpt + 1});
         }
         out.shadow_any_hit = (!transmissive && max_depth <= 22) ? 1u : 0u;
         out.shallow_bvh = max_depth <= 22 ? 1u : 0u;
+        /* interior marker 0xFFFFFFF8 | hint: split axis in bits 0-1, bit 2
+           set when the left child lies above the right one along it */
+        if (out.shallow_bvh)
+            for (auto [i, h] : hints)
+                nodes[i].min.w = phx_as_float(0xFFFFFFF8u | h);
     }
 
     GeometryBuffers &g = out.geometry;
@@ -127,7 +133,6 @@ int upload_scene(const PhxSceneDesc &sc, hipStream_t s, SceneBuffers &out) {
     PHX_TRY(copy_to_dev_buf(g.d_etri_uvs, sc.etri_uvs, ne, sizeof(PhxTriangleUvs), s), "upload");
     PHX_TRY(copy_to_dev_buf(g.d_etri_mat_index, sc.etri_mat_index, ne, sizeof(uint32_t), s), "upload");
     PHX_TRY(copy_to_dev_buf(g.d_bvh_nodes, nodes.data(), nodes.size(), sizeof(PhxBvhNode), s), "upload");
-    PHX_TRY(copy_to_dev_buf(g.d_node_order, g.node_order.data(), g.node_order.size(), 1, s), "upload");
 
     ShadingBuffers &sh = out.shading;
     sh.n_materials = sc.n_materials;
