@@ -275,6 +275,17 @@ int phx_device_synchronize(void);
  * 2*n_tris-1 nodes.  Returns the node count (0 on error). */
 uint32_t phx_build_bvh_median(const PhxTriangleVertices *tri_verts, uint32_t n_tris, PhxBvhNode *out_nodes);
 
+/* No counterpart in the reference: a binned surface-area-heuristic tree over
+ * the same expanded triangle boxes, one triangle per leaf, no interior node
+ * deeper than `max_depth`.  Leaf max.w = as_float(leaf_key[triangle]) (the
+ * tie-break key of the near-first closest-hit search: the triangle's leaf
+ * index in the reference tree); interior min.w = as_float(0xFFFFFFF8 | hint),
+ * hint = split axis | 4 when the left child lies above the right one.
+ * `out_nodes` must hold 2*n_tris-1 nodes.  Returns the node count, 0 on error
+ * or when no tree within max_depth exists (n_tris > 2^(max_depth+1)). */
+uint32_t phx_build_bvh_sah(const PhxTriangleVertices *tri_verts, uint32_t n_tris, const uint32_t *leaf_key,
+                           uint32_t max_depth, PhxBvhNode *out_nodes);
+
 /* photon_hash.h:41-50 build_photon_hash_info(bbox, grid_res), written to *out. */
 int phx_build_photon_hash_info(const PhxBvhNode *root, uint32_t grid_res, PhxPhotonHashData *out);
 
@@ -333,10 +344,11 @@ typedef struct PhxRenderArgs {
 
 /* Capture the pass into a hipGraph on the first run_pass and replay it. */
 #define PHX_PASS_GRAPH 1u
-/* Visit BVH nodes in the reference's depth-first order (hit.h:136-172) even
- * where the near-first order finds the same hits (a BVH no deeper than 22
- * interior levels); results are identical either way. */
-#define PHX_DFS_ORDER 2u
+/* Trace on the reference's median-split BVH in its depth-first node order
+ * (bvh.cpp:18-84, hit.h:136-172) even where the library would use its SAH
+ * tree with near-first order (reference tree no deeper than 22 interior
+ * levels, built by the library); results are identical either way. */
+#define PHX_REFERENCE_BVH 2u
 
 typedef struct PhxRenderer PhxRenderer;
 
@@ -388,7 +400,7 @@ typedef struct PhxBatch PhxBatch;
 typedef struct PhxBatchOptions {
     uint32_t trace_group_pixels; /* pixels traced per group (0 → 2 Mi) */
     uint32_t emit_group_photons; /* photons emitted per group (0 → 8 Mi) */
-    uint32_t flags;              /* PHX_PASS_GRAPH | PHX_DFS_ORDER */
+    uint32_t flags;              /* PHX_PASS_GRAPH | PHX_REFERENCE_BVH */
     uint32_t reserved;
 } PhxBatchOptions;
 

@@ -67,7 +67,7 @@ class RenderArgs(C.Structure):
 
 
 PHX_PASS_GRAPH = 1
-PHX_DFS_ORDER = 2
+PHX_REFERENCE_BVH = 2
 
 
 class BatchOptions(C.Structure):
@@ -133,6 +133,7 @@ SIGNATURES = {
     "phx_device_copy": (INT, VP, VP, C.c_size_t, VP),
     "phx_device_synchronize": (INT,),
     "phx_build_bvh_median": (U32, VP, U32, VP),
+    "phx_build_bvh_sah": (U32, VP, U32, VP, U32, VP),
     "phx_build_photon_hash_info": (INT, VP, U32, VP),
     "phx_renderer_create": (INT, C.POINTER(RenderArgs), C.POINTER(SceneDesc), C.POINTER(VP)),
     "phx_renderer_destroy": (None, VP),

@@ -104,14 +104,12 @@ int PhxBatch::build(const PhxSceneDesc *descs) {
     for (uint32_t i = 0; i < S; i++) {
         const PhxRenderArgs &a = args[i];
         scenes.emplace_back(new SceneBuffers());
-        int rc = upload_scene(descs[i], s, *scenes[i]);
+        int rc = upload_scene(descs[i], s, ((opt.flags | a.flags) & PHX_REFERENCE_BVH) != 0, *scenes[i]);
         if (rc != PHX_OK)
             return rc;
         BatchScene &B = table[i];
         std::memset(&B, 0, sizeof(B));
         B.sc = scenes[i]->args();
-        if ((opt.flags | a.flags) & PHX_DFS_ORDER)
-            B.sc.near_first = 0;
         B.cam = *descs[i].camera;
         phx_build_photon_hash_info(&scenes[i]->root, a.grid_res, &B.hash);
         B.width = a.width;
@@ -275,7 +273,7 @@ int phx_batch_create(const PhxRenderArgs *args, const PhxSceneDesc *scenes, uint
             return fail(PHX_ERR_INVALID_ARG, "phx_batch_create: scene " + std::to_string(i) + ": " +
                                                  std::string(phx_last_error()));
     }
-    if (options && (options->flags & ~(uint32_t)(PHX_PASS_GRAPH | PHX_DFS_ORDER)))
+    if (options && (options->flags & ~(uint32_t)(PHX_PASS_GRAPH | PHX_REFERENCE_BVH)))
         return fail(PHX_ERR_INVALID_ARG, "phx_batch_create: unknown flags");
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)

@@ -11,6 +11,10 @@
  * children are processed concurrently (std::async down to a size cutoff); the
  * permutation is the same as the sequential recursion because each range is
  * sorted by std::sort with the same comparator on the same input sequence.
+ *
+ * phx_build_bvh_sah is this library's own tree for tracing: the same leaves
+ * (one expanded triangle box each) grouped by a binned surface-area
+ * heuristic, within a depth limit, in the same node format.
  */
 #include <algorithm>
 #include <cstring>
@@ -120,6 +124,169 @@ struct Builder {
     }
 };
 
+/* Binned surface-area-heuristic build over the same expanded triangle boxes.
+ * Leaves hold one triangle, as in the median tree, so a closest-hit search
+ * over this tree meets the same candidate triangles; only the boxes (and so
+ * the number of nodes visited) change.  Every split keeps the tree within
+ * `max_depth` interior levels: a candidate split is taken only if both sides
+ * can still be built balanced within the limit, otherwise the range is split
+ * at its object median, which always can. */
+struct SahNode {
+    Box box;
+    int tri;        /* -1 for interior */
+    unsigned right; /* right child id (interior) */
+    unsigned hint;  /* split axis | 4 when the left child lies above the right */
+};
+
+unsigned ceil_log2(unsigned c) {
+    unsigned k = 0;
+    while ((1u << k) < c)
+        k++;
+    return k;
+}
+
+float half_area(const Box &b) {
+    float e[3];
+    for (int a = 0; a < 3; a++)
+        e[a] = std::max(0.0f, b.hi[a] - b.lo[a]);
+    return e[0] * e[1] + e[1] * e[2] + e[2] * e[0];
+}
+
+void grow(Box &b, const Box &c) {
+    for (int a = 0; a < 3; a++) {
+        b.lo[a] = std::min(b.lo[a], c.lo[a]);
+        b.hi[a] = std::max(b.hi[a], c.hi[a]);
+    }
+}
+
+Box empty_box() {
+    Box b;
+    for (int a = 0; a < 3; a++) {
+        b.lo[a] = BIG;
+        b.hi[a] = -BIG;
+    }
+    return b;
+}
+
+struct SahBuilder {
+    static constexpr int BINS = 32;
+    const std::vector<Box> &boxes;
+    const std::vector<float> *cen; /* [3] centroid per triangle */
+    std::vector<unsigned> &order;
+    unsigned max_depth;
+
+    std::vector<SahNode> build(unsigned start, unsigned end, unsigned depth) const {
+        std::vector<SahNode> out;
+        unsigned count = end - start;
+        if (count == 1) {
+            out.push_back(SahNode{boxes[order[start]], (int)order[start], 0u, 0u});
+            return out;
+        }
+        Box bb = empty_box(), cb = empty_box();
+        for (unsigned i = start; i < end; i++) {
+            unsigned t = order[i];
+            grow(bb, boxes[t]);
+            for (int a = 0; a < 3; a++) {
+                cb.lo[a] = std::min(cb.lo[a], cen[a][t]);
+                cb.hi[a] = std::max(cb.hi[a], cen[a][t]);
+            }
+        }
+        /* best binned split: least area-weighted count, among those that keep
+           both children buildable within max_depth */
+        int best_axis = -1, best_bin = 0;
+        float best_cost = 0.0f;
+        for (int a = 0; a < 3; a++) {
+            float ext = cb.hi[a] - cb.lo[a];
+            if (!(ext > 0.0f))
+                continue;
+            float scale = BINS / ext;
+            Box bin_box[BINS];
+            unsigned bin_n[BINS] = {};
+            for (int k = 0; k < BINS; k++)
+                bin_box[k] = empty_box();
+            for (unsigned i = start; i < end; i++) {
+                unsigned t = order[i];
+                int k = std::min(BINS - 1, (int)((cen[a][t] - cb.lo[a]) * scale));
+                bin_n[k]++;
+                grow(bin_box[k], boxes[t]);
+            }
+            float left_area[BINS];
+            unsigned left_n[BINS];
+            Box acc = empty_box();
+            unsigned n_acc = 0;
+            for (int k = 0; k < BINS; k++) {
+                if (bin_n[k])
+                    grow(acc, bin_box[k]);
+                n_acc += bin_n[k];
+                left_area[k] = half_area(acc);
+                left_n[k] = n_acc;
+            }
+            acc = empty_box();
+            n_acc = 0;
+            for (int k = BINS - 1; k > 0; k--) {
+                if (bin_n[k])
+                    grow(acc, bin_box[k]);
+                n_acc += bin_n[k];
+                unsigned nl = left_n[k - 1], nr = n_acc;
+                if (nl == 0 || nr == 0 || depth + ceil_log2(std::max(nl, nr)) > max_depth)
+                    continue;
+                float cost = left_area[k - 1] * (float)nl + half_area(acc) * (float)nr;
+                if (best_axis < 0 || cost < best_cost) {
+                    best_axis = a;
+                    best_bin = k;
+                    best_cost = cost;
+                }
+            }
+        }
+        unsigned mid;
+        int axis;
+        if (best_axis >= 0) {
+            axis = best_axis;
+            float lo = cb.lo[axis], scale = BINS / (cb.hi[axis] - cb.lo[axis]);
+            const std::vector<float> &c = cen[axis];
+            int bin = best_bin;
+            auto it = std::stable_partition(order.begin() + start, order.begin() + end, [&](unsigned t) {
+                return std::min(BINS - 1, (int)((c[t] - lo) * scale)) < bin;
+            });
+            mid = (unsigned)(it - order.begin());
+        } else {
+            /* object median along the longest centroid extent (index order
+               breaks ties, so the build is deterministic) */
+            float ex = cb.hi[0] - cb.lo[0], ey = cb.hi[1] - cb.lo[1], ez = cb.hi[2] - cb.lo[2];
+            axis = ex >= ey ? (ex >= ez ? 0 : 2) : (ey >= ez ? 1 : 2);
+            const std::vector<float> &c = cen[axis];
+            std::sort(order.begin() + start, order.begin() + end,
+                      [&c](unsigned i, unsigned j) { return c[i] < c[j] || (c[i] == c[j] && i < j); });
+            mid = start + count / 2;
+        }
+        std::vector<SahNode> left, right;
+        if (depth < 6 && count > 4096) {
+            auto fut = std::async(std::launch::async, [this, start, mid, depth] { return build(start, mid, depth + 1); });
+            right = build(mid, end, depth + 1);
+            left = fut.get();
+        } else {
+            left = build(start, mid, depth + 1);
+            right = build(mid, end, depth + 1);
+        }
+        const Box &lb = left[0].box, &rb = right[0].box;
+        unsigned swap = (lb.lo[axis] + lb.hi[axis]) > (rb.lo[axis] + rb.hi[axis]) ? 4u : 0u;
+        out.reserve(1 + left.size() + right.size());
+        out.push_back(SahNode{bb, -1, (unsigned)(1 + left.size()), (unsigned)axis | swap});
+        unsigned rofs = (unsigned)(1 + left.size());
+        for (SahNode n : left) {
+            if (n.tri < 0)
+                n.right += 1;
+            out.push_back(n);
+        }
+        for (SahNode n : right) {
+            if (n.tri < 0)
+                n.right += rofs;
+            out.push_back(n);
+        }
+        return out;
+    }
+};
+
 } // namespace
 
 extern "C" uint32_t phx_build_bvh_median(const PhxTriangleVertices *tri_verts, uint32_t n_tris, PhxBvhNode *out_nodes) {
@@ -144,6 +311,40 @@ extern "C" uint32_t phx_build_bvh_median(const PhxTriangleVertices *tri_verts, u
         o.max.z = n.box.hi[2];
         o.min.w = phx_as_float((uint32_t)n.tri);
         o.max.w = phx_as_float(n.tri < 0 ? n.right : 0u);
+    }
+    return (uint32_t)nodes.size();
+}
+
+extern "C" uint32_t phx_build_bvh_sah(const PhxTriangleVertices *tri_verts, uint32_t n_tris, const uint32_t *leaf_key,
+                                      uint32_t max_depth, PhxBvhNode *out_nodes) {
+    if (!tri_verts || !leaf_key || !out_nodes || n_tris == 0 || n_tris >= 0x80000000u)
+        return 0;
+    if (ceil_log2(n_tris) > max_depth + 1)
+        return 0; /* not even a balanced tree fits */
+    std::vector<Box> boxes(n_tris);
+    std::vector<float> cen[3];
+    for (int a = 0; a < 3; a++)
+        cen[a].resize(n_tris);
+    for (uint32_t i = 0; i < n_tris; i++) {
+        boxes[i] = triangle_box(tri_verts[i]);
+        for (int a = 0; a < 3; a++)
+            cen[a][i] = 0.5f * (boxes[i].lo[a] + boxes[i].hi[a]);
+    }
+    std::vector<unsigned> order(n_tris);
+    std::iota(order.begin(), order.end(), 0u);
+    SahBuilder b{boxes, cen, order, max_depth};
+    std::vector<SahNode> nodes = b.build(0, n_tris, 0);
+    for (size_t i = 0; i < nodes.size(); i++) {
+        const SahNode &n = nodes[i];
+        PhxBvhNode &o = out_nodes[i];
+        o.min.x = n.box.lo[0];
+        o.min.y = n.box.lo[1];
+        o.min.z = n.box.lo[2];
+        o.max.x = n.box.hi[0];
+        o.max.y = n.box.hi[1];
+        o.max.z = n.box.hi[2];
+        o.min.w = phx_as_float(n.tri < 0 ? 0xFFFFFFF8u | n.hint : (uint32_t)n.tri);
+        o.max.w = phx_as_float(n.tri < 0 ? n.right : leaf_key[n.tri]);
     }
     return (uint32_t)nodes.size();
 }

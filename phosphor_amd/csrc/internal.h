@@ -41,10 +41,9 @@ struct SceneArgs {
     float scene_radius;
     const float *envmap;
     uint32_t env_w, env_h, has_env;
-    /* 1: interior BVH nodes carry a visiting-order hint in min.w and none is
-     * deeper than 22, so the closest-hit search never overflows its stack and
-     * may visit the child nearer along the ray first; 0 keeps the
-     * depth-first order. */
+    /* 1: `nodes` is the library's SAH tree (interior min.w = 0xFFFFFFF8 |
+     * visiting hint, leaf max.w = reference leaf index), searched near-first;
+     * 0: `nodes` is the reference tree, searched depth-first. */
     uint32_t near_first;
 };
 

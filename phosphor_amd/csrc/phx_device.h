@@ -349,32 +349,35 @@ PHX_DEV bool traverse(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, Hit
     return found;
 }
 
-/* The closest hit of traverse<false>, found in a different node order: at
- * an interior node the child that lies nearer along the ray on the node's
- * split axis (hint bits 0-1 of min.w, bit 2 set when the left child lies
- * above the right one) is entered first, so the closest hit tends to be
- * found early and more of the other subtrees fail their box test.
+/* The closest hit of traverse<false> on the reference tree, found on the
+ * library's SAH tree of the same leaves (bvh.cpp phx_build_bvh_sah).  At an
+ * interior node the child nearer along the ray on the split axis (hint bits
+ * 0-1 of min.w; bit 2 set when the left child lies above the right one) is
+ * entered first, so the closest hit tends to be found early and more of the
+ * other subtrees fail their box test.
  *
  * Why the result is the same: the reference's depth-first search returns the
  * hit with the least t, and among hits at that same t the one met last, i.e.
  * the one in the leaf of largest preorder index (is_triangle_hit accepts
  * t <= t_max, so a later equal hit overwrites an earlier one, and nodes are
- * stored in preorder).  Leaf boxes are expanded by 1e-3 (bvh.cpp), so a node
- * holding a hit at t <= closest has its entry below t and passes the slab
- * test whenever it is tested; a node is dropped only when its entry is at or
- * beyond closest, where it holds no hit at or below closest.  So the least t,
- * with ties going to the larger leaf index, is the reference's hit whatever
- * the order -- provided the reference loses nothing to its stack-overflow rule
- * (hit.h:163).  The host sets near_first only when no interior node is deeper
- * than 22: then neither search can overflow (both keep one pending sibling
- * per interior ancestor at most). */
+ * stored in preorder).  Leaf boxes of both trees are the triangle boxes
+ * expanded by 1e-3, so a node holding a hit at t <= closest has its entry
+ * below t and passes the slab test whenever it is tested; a node is dropped
+ * only when its entry is at or beyond closest, where it holds no hit at or
+ * below closest.  So the least t, with ties going to the larger reference
+ * leaf index (kept in each SAH leaf's max.w), is the reference's hit whatever
+ * the tree and order -- provided the reference loses nothing to its
+ * stack-overflow rule (hit.h:163).  The host builds the SAH tree only when no
+ * interior node of the reference tree is deeper than 22, and builds it no
+ * deeper than 22 itself, so neither search can overflow (both keep one pending
+ * sibling per interior ancestor at most). */
 PHX_DEV bool traverse_near_first(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
                                  u32 stride) {
     f32 ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
     u32 neg = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
     f32 closest = t_max;
     bool found = false;
-    u32 best = 0;
+    u32 best_key = 0;
     int q = -1;
     u32 index = 0;
     for (;;) {
@@ -386,11 +389,12 @@ PHX_DEV bool traverse_near_first(const SceneView &sc, F4 o, F4 d, f32 t_min, f32
             u32 tri = phx_as_uint(nmin.w);
             if (tri < BVH_INTERIOR) {
                 HitRecord cand;
+                u32 key = phx_as_uint(nmax.w);
                 if (is_triangle_hit(sc, tri, o, d, t_min, closest, cand) &&
-                    (cand.t < closest || !found || index > best)) {
+                    (cand.t < closest || !found || key > best_key)) {
                     closest = cand.t;
                     rec = cand;
-                    best = index;
+                    best_key = key;
                     found = true;
                 }
             } else {

@@ -55,7 +55,7 @@ int check_render_args(const PhxRenderArgs &a, const char *who) {
         return fail(PHX_ERR_INVALID_ARG, w + ": trace_mode must be 0 or 1");
     if (a.direct_samples == 0 || a.direct_samples > 128)
         return fail(PHX_ERR_INVALID_ARG, w + ": direct_samples outside 1..128 (cmd_args.hpp:59)");
-    if (a.flags & ~(uint32_t)(PHX_PASS_GRAPH | PHX_DFS_ORDER))
+    if (a.flags & ~(uint32_t)(PHX_PASS_GRAPH | PHX_REFERENCE_BVH))
         return fail(PHX_ERR_INVALID_ARG, w + ": unknown flags");
     if (a.emit_wave_bounces > 32)
         return fail(PHX_ERR_INVALID_ARG, w + ": emit_wave_bounces outside 0..32");
@@ -67,12 +67,10 @@ int check_render_args(const PhxRenderArgs &a, const char *who) {
 int PhxRenderer::set_scene(const PhxSceneDesc &sc) {
     hipStream_t s = stream;
     const PhxRenderArgs &a = args;
-    int rc = upload_scene(sc, s, buffers.scene);
+    int rc = upload_scene(sc, s, (a.flags & PHX_REFERENCE_BVH) != 0, buffers.scene);
     if (rc != PHX_OK)
         return rc;
     shadow_any_hit = buffers.scene.shadow_any_hit;
-    if (a.flags & PHX_DFS_ORDER)
-        buffers.scene.shallow_bvh = 0;
     const PhxBvhNode &root = buffers.scene.root;
 
     /* alloc_image_buffers (image.cpp:11-21, zero-filled) */

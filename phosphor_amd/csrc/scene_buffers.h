@@ -68,9 +68,9 @@ struct SceneBuffers {
        to shadow rays (light_sampling.h:316) and no interior BVH node is deeper
        than 22, so the traversal stack never overflows (hit.h:163) */
     uint32_t shadow_any_hit = 0;
-    /* no interior BVH node is deeper than 22 and the uploaded interior nodes
-       carry visiting-order hints: closest-hit searches may visit nodes
-       near-first (same hits, SceneArgs::near_first) */
+    /* the uploaded tree is the library's SAH tree (the reference tree is no
+       deeper than 22 interior levels): closest-hit searches visit nodes
+       near-first with leaf-key tie breaks (same hits, SceneArgs::near_first) */
     uint32_t shallow_bvh = 0;
 
     SceneArgs args() const;
@@ -81,7 +81,7 @@ struct SceneBuffers {
  * (scene_data.cpp:14-23), texture atlas, bounding sphere (shading.cpp:45-58)
  * and envmap (shading.cpp:60-71).  Asynchronous on `s` except for the BVH
  * build; the caller synchronises before freeing host memory. */
-int upload_scene(const PhxSceneDesc &sc, hipStream_t s, SceneBuffers &out);
+int upload_scene(const PhxSceneDesc &sc, hipStream_t s, bool reference_bvh, SceneBuffers &out);
 
 /* Scene validity checks of main.cpp:40-47 shared by the drivers. */
 int check_scene_desc(const PhxSceneDesc &sc, const char *who);

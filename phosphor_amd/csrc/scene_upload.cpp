@@ -71,7 +71,7 @@ int check_scene_desc(const PhxSceneDesc &sc, const char *who) {
             return hip_status(e_, what);                                                                               \
     } while (0)
 
-int upload_scene(const PhxSceneDesc &sc, hipStream_t s, SceneBuffers &out) {
+int upload_scene(const PhxSceneDesc &sc, hipStream_t s, bool reference_bvh, SceneBuffers &out) {
     uint32_t n = sc.n_triangles, ne = sc.n_e_triangles;
 
     /* Bvh bvh(scene.triangles) */
@@ -90,7 +90,6 @@ int upload_scene(const PhxSceneDesc &sc, hipStream_t s, SceneBuffers &out) {
         for (uint32_t i = 0; i < sc.n_materials; i++)
             transmissive = transmissive || !(sc.materials[i].transmission < 1e-5f);
         uint32_t max_depth = 0;
-        std::vector<std::pair<uint32_t, uint32_t>> hints;
         std::vector<std::pair<uint32_t, uint32_t>> todo{{0u, 0u}};
         while (!todo.empty()) {
             auto [i, dpt] = todo.back();
@@ -101,23 +100,29 @@ int upload_scene(const PhxSceneDesc &sc, hipStream_t s, SceneBuffers &out) {
             uint32_t l = i + 1, r = phx_as_uint(nodes[i].max.w);
             if (l >= nodes.size() || r >= nodes.size() || r <= i || dpt >= nodes.size())
                 return fail(PHX_ERR_INVALID_ARG, "set_scene: BVH child index out of range");
-            /* the builder's split axis (longest box extent, bvh.cpp:18-84) and
-               whether the left child starts above the right one along it */
-            const PhxBvhNode &nd = nodes[i];
-            float ex = nd.max.x - nd.min.x, ey = nd.max.y - nd.min.y, ez = nd.max.z - nd.min.z;
-            int axis = ex > ey ? (ex > ez ? 0 : 2) : (ey > ez ? 1 : 2);
-            auto lo = [axis](const PhxBvhNode &b) { return axis == 0 ? b.min.x : axis == 1 ? b.min.y : b.min.z; };
-            hints.push_back({i, (uint32_t)axis | (lo(nodes[l]) > lo(nodes[r]) ? 4u : 0u)});
             todo.push_back({l, dpt + 1});
             todo.push_back({r, dpt + 1});
         }
         out.shadow_any_hit = (!transmissive && max_depth <= 22) ? 1u : 0u;
-        out.shallow_bvh = max_depth <= 22 ? 1u : 0u;
-        /* interior marker 0xFFFFFFF8 | hint: split axis in bits 0-1, bit 2
-           set when the left child lies above the right one along it */
-        if (out.shallow_bvh)
-            for (auto [i, h] : hints)
-                nodes[i].min.w = phx_as_float(0xFFFFFFF8u | h);
+        out.shallow_bvh = 0;
+        /* Where the reference tree never overflows its search stack, its
+           closest hit is the least t over all triangles, equal t going to the
+           larger leaf index: a property of the triangles, not of the tree.
+           Trace then on an SAH tree of the same leaves (far fewer nodes per
+           ray) carrying those leaf indices as tie-break keys. */
+        if (!sc.bvh_nodes && !reference_bvh && max_depth <= 22) {
+            std::vector<uint32_t> key(n);
+            for (uint32_t i = 0; i < nodes.size(); i++) {
+                uint32_t t = phx_as_uint(nodes[i].min.w);
+                if (t != 0xFFFFFFFFu)
+                    key[t] = i;
+            }
+            std::vector<PhxBvhNode> sah(nodes.size());
+            if (phx_build_bvh_sah(sc.tri_verts, n, key.data(), 22, sah.data()) == nodes.size()) {
+                nodes.swap(sah);
+                out.shallow_bvh = 1;
+            }
+        }
     }
 
     GeometryBuffers &g = out.geometry;

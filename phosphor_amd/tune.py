@@ -3,8 +3,8 @@
 Two kinds of knobs:
 
 * exact knobs change how the pass is launched, never its results: trace form
-  (wavefront or one kernel), BVH visiting order (near-first or the
-  reference's depth-first), emission batch size (``--batch-size``,
+  (wavefront or one kernel), BVH (SAH tree visited near-first, or the
+  reference's median tree visited depth-first), emission batch size (``--batch-size``,
   cmd_args.hpp:42) and the number of bounce-synchronous emission launches.
   A candidate is accepted only if its SPPM state after the measured passes is
   bit-identical to the state under the given settings.
@@ -90,7 +90,7 @@ def autotune(cfg: PassConfig, scene, passes: int = 3, warmup: int = 1, wave_boun
                 best, best_ms = cand, ms
 
     consider("trace_mode", trace_modes, True)
-    consider("dfs_order", (False, True), True)
+    consider("reference_bvh", (False, True), True)
     consider("emit_wave_bounces", [w for w in wave_bounces if w <= cfg.max_photon_bounces], True)
     if batch_sizes is None:
         batch_sizes = sorted({min(cfg.photons_per_round, b) for b in (1 << 18, 1 << 20, cfg.photons_per_round)})

@@ -88,6 +88,69 @@ def test_bvh_host_builder_matches_oracle(phx_lib, oracle_lib):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def _sah_tree(phx_lib, sc, max_depth=22):
+    n = sc.n_triangles
+    med = np.zeros((2 * n - 1, 8), np.float32)
+    assert phx_lib.phx_build_bvh_median(sc.tri_verts.ctypes.data, n, med.ctypes.data) == 2 * n - 1
+    key = np.zeros(n, np.uint32)
+    leaf = med[:, 3].view(np.uint32) != 0xFFFFFFFF
+    key[med[leaf, 3].view(np.uint32)] = np.nonzero(leaf)[0]
+    sah = np.zeros((2 * n - 1, 8), np.float32)
+    got = phx_lib.phx_build_bvh_sah(sc.tri_verts.ctypes.data, n, key.ctypes.data, max_depth, sah.ctypes.data)
+    return med, key, sah, got
+
+
+@pytest.mark.parametrize("which", ["cornell", "caustic", "room"])
+def test_sah_tree_structure(phx_lib, which):
+    """The SAH tree holds every triangle in exactly one leaf with the same
+    expanded box as the reference tree's leaf, carries the reference leaf
+    index as key, nests child boxes in parent boxes, encodes interior hints
+    as 0xFFFFFFF8 | (axis | 4·swap) and stays within the depth limit."""
+    from phosphor_amd import scenes
+
+    sc = {"cornell": lambda: scenes.cornell_box(8, 8), "caustic": lambda: scenes.caustic_scene(8, 8, level=4),
+          "room": lambda: scenes.sphere_room(8, 8, 20_000, np.random.default_rng(5))}[which]()
+    n = sc.n_triangles
+    med, key, sah, got = _sah_tree(phx_lib, sc)
+    assert got == 2 * n - 1
+    w_min, w_max = sah[:, 3].view(np.uint32), sah[:, 7].view(np.uint32)
+    interior = w_min >= 0xFFFFFFF8
+    assert interior.sum() == n - 1
+    hints = w_min[interior] & 7
+    assert np.all((hints & 3) <= 2)
+    tris = w_min[~interior]
+    assert np.array_equal(np.sort(tris), np.arange(n, dtype=np.uint32))
+    assert np.array_equal(w_max[~interior], key[tris])
+    med_leaf = med[:, 3].view(np.uint32) != 0xFFFFFFFF
+    med_box = np.zeros((n, 6), np.float32)
+    med_box[med[med_leaf, 3].view(np.uint32)] = med[med_leaf][:, [0, 1, 2, 4, 5, 6]]
+    assert np.array_equal(sah[~interior][:, [0, 1, 2, 4, 5, 6]], med_box[tris])
+    depth = np.zeros(2 * n - 1, np.int64)
+    for i in np.nonzero(interior)[0]:  # preorder: parents come first
+        l, r = i + 1, int(w_max[i])
+        assert i < r < 2 * n - 1
+        for c in (l, r):
+            depth[c] = depth[i] + 1
+            assert np.all(sah[c, :3] >= sah[i, :3]) and np.all(sah[c, 4:7] <= sah[i, 4:7])
+    assert depth[interior].max() <= 22
+
+
+def test_sah_tree_depth_limit(phx_lib):
+    """A limit no balanced tree meets is refused; a tight one is met."""
+    from phosphor_amd import scenes
+
+    sc = scenes.caustic_scene(8, 8, level=3)
+    k = int(np.ceil(np.log2(sc.n_triangles)))  # a balanced tree's deepest interior node is at depth k - 1
+    assert _sah_tree(phx_lib, sc, max_depth=k - 2)[3] == 0
+    med, key, sah, got = _sah_tree(phx_lib, sc, max_depth=k - 1)
+    assert got == 2 * sc.n_triangles - 1
+    w_min, w_max = sah[:, 3].view(np.uint32), sah[:, 7].view(np.uint32)
+    depth = np.zeros(len(sah), np.int64)
+    for i in np.nonzero(w_min >= 0xFFFFFFF8)[0]:
+        depth[i + 1] = depth[int(w_max[i])] = depth[i] + 1
+    assert depth[w_min >= 0xFFFFFFF8].max() == k - 1
+
+
 def test_no_device_reported_here(phx_lib):
     import os
 

@@ -68,10 +68,12 @@ def _near_first_cases():
 
 
 @pytest.mark.parametrize("index", [0, 1, 2])
-def test_near_first_traversal_matches_depth_first(index):
-    """Closest-hit searches visit BVH nodes near-first when the tree is no
-    deeper than 22 interior levels; PHX_DFS_ORDER forces the reference's
-    depth-first order.  Hitpoints, photon map and SPPM state are identical."""
+def test_sah_tree_matches_reference_tree(index):
+    """Where the reference's median tree is no deeper than 22 interior levels
+    the library traces on its own SAH tree, visiting nodes near-first and
+    breaking equal-distance ties by the reference tree's leaf order;
+    PHX_REFERENCE_BVH traces on the reference tree depth-first.  Hitpoints,
+    photon map and SPPM state are identical."""
     name, make, w, h, grid, radius = _near_first_cases()[index]
     scene = make()
     base = PassConfig(width=w, height=h, photons_per_round=1 << 16,
@@ -79,7 +81,7 @@ def test_near_first_traversal_matches_depth_first(index):
                       initial_radius=radius)
     print(f"case {name}: {scene.n_triangles} triangles", flush=True)
     near = Renderer(base, scene)
-    dfs = Renderer(PassConfig(**{**base.__dict__, "dfs_order": True}), scene)
+    dfs = Renderer(PassConfig(**{**base.__dict__, "reference_bvh": True}), scene)
     for p in range(2):
         near.run_pass()
         dfs.run_pass()
