@@ -243,7 +243,7 @@ template <bool ANY> PHX_DEV void shadow_one(const SceneView &sc, const WfState &
     F4 trans = WHITE;
     for (u32 depth = 0; depth < MAX_RAY_BOUNCES; depth++) {
         HitRecord rec;
-        if (!(ANY ? traverse<true>(sc, o, d, t_min, t_max, rec, stk, WF_SHADOW_BLOCK)
+        if (!(ANY ? scene_any_hit(sc, o, d, t_min, t_max, rec, stk, WF_SHADOW_BLOCK)
                   : scene_intersect(sc, o, d, t_min, t_max, rec, stk, WF_SHADOW_BLOCK)))
             break;
         const PhxMaterial &mat = sc.materials[rec.mat_index];

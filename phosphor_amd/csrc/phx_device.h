@@ -349,31 +349,49 @@ PHX_DEV bool traverse(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, Hit
     return found;
 }
 
-/* The closest hit of traverse<false> on the reference tree, found on the
- * library's SAH tree of the same leaves (bvh.cpp phx_build_bvh_sah).  At an
- * interior node the child nearer along the ray on the split axis (hint bits
- * 0-1 of min.w; bit 2 set when the left child lies above the right one) is
- * entered first, so the closest hit tends to be found early and more of the
- * other subtrees fail their box test.
+/* The hit traverse<ANY> finds on the reference tree, found on the library's
+ * SAH tree of the same leaves (bvh.cpp phx_build_bvh_sah).  At an interior
+ * node the child nearer along the ray on the split axis (hint bits 0-1 of
+ * min.w; bit 2 set when the left child lies above the right one) is entered
+ * first, so the closest hit tends to be found early and more of the other
+ * subtrees fail their box test.
  *
- * Why the result is the same: the reference's depth-first search returns the
- * hit with the least t, and among hits at that same t the one met last, i.e.
- * the one in the leaf of largest preorder index (is_triangle_hit accepts
- * t <= t_max, so a later equal hit overwrites an earlier one, and nodes are
- * stored in preorder).  Leaf boxes of both trees are the triangle boxes
- * expanded by 1e-3, so a node holding a hit at t <= closest has its entry
- * below t and passes the slab test whenever it is tested; a node is dropped
- * only when its entry is at or beyond closest, where it holds no hit at or
- * below closest.  So the least t, with ties going to the larger reference
- * leaf index (kept in each SAH leaf's max.w), is the reference's hit whatever
- * the tree and order -- provided the reference loses nothing to its
- * stack-overflow rule (hit.h:163).  The host builds the SAH tree only when no
- * interior node of the reference tree is deeper than 22, and builds it no
- * deeper than 22 itself, so neither search can overflow (both keep one pending
- * sibling per interior ancestor at most). */
-PHX_DEV bool traverse_near_first(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
-                                 u32 stride) {
-    f32 ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+ * Why the result is the same.  The reference's depth-first search with its
+ * strict box test (hi > lo) returns the hit of least t; among several hits at
+ * that t the one met last, i.e. in the leaf of largest preorder index
+ * (is_triangle_hit accepts t <= t_max, so a later equal hit overwrites an
+ * earlier one, and nodes are stored in preorder) -- except at t == t_min,
+ * where the first one met wins, because once closest == t_min every further
+ * box fails the strict test.  The leaf boxes of both trees are the triangle
+ * boxes expanded by 1e-3, so a leaf holding a hit at t in [t_min, closest] has
+ * entry < t < exit by that margin and passes the inclusive test below
+ * whenever it is tested; a node is dropped only when no point of its box lies
+ * in [t_min, closest].  So every candidate is met whatever the order, and
+ * "least t; at equal t the larger reference leaf index (kept in each SAH
+ * leaf's max.w), at t == t_min the smaller" is the reference's hit -- provided
+ * the reference loses nothing to its stack-overflow rule (hit.h:163).  The
+ * host builds the SAH tree only when no interior node of the reference tree
+ * is deeper than 22, and builds it no deeper than 22 itself, so neither
+ * search can overflow (both keep one pending sibling per interior ancestor at
+ * most), and only for scenes within |coordinate| <= 1024, where the box test's
+ * rounding (a few 1e-4 at most) stays inside the 1e-3 margin.  ANY (any hit
+ * in [t_min, t_max], for shadow rays in opaque scenes): the reference's
+ * closest-hit loop finds a hit exactly when one exists, and so does this
+ * search.
+ *
+ * The box test is the usual fused form t = box * inv + (-origin * inv) with
+ * min/max; it only decides which nodes are visited, never the hit record,
+ * which is_triangle_hit computes exactly as the reference. */
+template <bool ANY>
+PHX_DEV bool traverse_sah(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
+                          u32 stride) {
+    if (!(t_min < t_max))
+        return false; /* the reference's root test fails */
+    const f32 tiny = 1e-20f;
+    f32 ix = 1.0f / (phx_fabsf(d.x) > tiny ? d.x : __builtin_copysignf(tiny, d.x));
+    f32 iy = 1.0f / (phx_fabsf(d.y) > tiny ? d.y : __builtin_copysignf(tiny, d.y));
+    f32 iz = 1.0f / (phx_fabsf(d.z) > tiny ? d.z : __builtin_copysignf(tiny, d.z));
+    f32 ox = -o.x * ix, oy = -o.y * iy, oz = -o.z * iz;
     u32 neg = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
     f32 closest = t_max;
     bool found = false;
@@ -383,19 +401,30 @@ PHX_DEV bool traverse_near_first(const SceneView &sc, F4 o, F4 d, f32 t_min, f32
     for (;;) {
         const PhxBvhNode *node = sc.nodes + index;
         F4 nmin = ld(&node->min), nmax = ld(&node->max);
-        f32 lo = t_min, hi = closest;
-        if (slab(nmin.x, nmax.x, o.x, ix, lo, hi) && slab(nmin.y, nmax.y, o.y, iy, lo, hi) &&
-            slab(nmin.z, nmax.z, o.z, iz, lo, hi)) {
+        f32 x0 = __builtin_fmaf(nmin.x, ix, ox), x1 = __builtin_fmaf(nmax.x, ix, ox);
+        f32 y0 = __builtin_fmaf(nmin.y, iy, oy), y1 = __builtin_fmaf(nmax.y, iy, oy);
+        f32 z0 = __builtin_fmaf(nmin.z, iz, oz), z1 = __builtin_fmaf(nmax.z, iz, oz);
+        f32 lo = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(x0, x1), __builtin_fminf(y0, y1)),
+                                 __builtin_fmaxf(__builtin_fminf(z0, z1), t_min));
+        f32 hi = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(x0, x1), __builtin_fmaxf(y0, y1)),
+                                 __builtin_fminf(__builtin_fmaxf(z0, z1), closest));
+        if (lo <= hi) {
             u32 tri = phx_as_uint(nmin.w);
             if (tri < BVH_INTERIOR) {
                 HitRecord cand;
-                u32 key = phx_as_uint(nmax.w);
-                if (is_triangle_hit(sc, tri, o, d, t_min, closest, cand) &&
-                    (cand.t < closest || !found || key > best_key)) {
-                    closest = cand.t;
-                    rec = cand;
-                    best_key = key;
-                    found = true;
+                if (is_triangle_hit(sc, tri, o, d, t_min, closest, cand)) {
+                    u32 key = phx_as_uint(nmax.w);
+                    if (ANY) {
+                        rec = cand;
+                        found = true;
+                        break;
+                    }
+                    if (!found || cand.t < closest || (cand.t == t_min ? key < best_key : key > best_key)) {
+                        closest = cand.t;
+                        rec = cand;
+                        best_key = key;
+                        found = true;
+                    }
                 }
             } else {
                 /* right first when the ray runs against the left-to-right
@@ -419,10 +448,19 @@ PHX_DEV bool traverse_near_first(const SceneView &sc, F4 o, F4 d, f32 t_min, f32
     return found;
 }
 
+/* any-hit test of shadow rays in opaque scenes: the reference tree
+ * depth-first, or the SAH tree */
+PHX_DEV bool scene_any_hit(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
+                           u32 stride) {
+    if (sc.near_first)
+        return traverse_sah<true>(sc, o, d, t_min, t_max, rec, stk, stride);
+    return traverse<true>(sc, o, d, t_min, t_max, rec, stk, stride);
+}
+
 PHX_DEV bool scene_intersect(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
                              u32 stride) {
     if (sc.near_first)
-        return traverse_near_first(sc, o, d, t_min, t_max, rec, stk, stride);
+        return traverse_sah<false>(sc, o, d, t_min, t_max, rec, stk, stride);
     return traverse<false>(sc, o, d, t_min, t_max, rec, stk, stride);
 }
 

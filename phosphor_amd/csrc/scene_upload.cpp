@@ -3,6 +3,7 @@
  * rendering.cpp:15-30, geometry.cpp, shading.cpp) for both pass drivers.
  */
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <utility>
 #include <vector>
@@ -109,8 +110,13 @@ int upload_scene(const PhxSceneDesc &sc, hipStream_t s, bool reference_bvh, Scen
            closest hit is the least t over all triangles, equal t going to the
            larger leaf index: a property of the triangles, not of the tree.
            Trace then on an SAH tree of the same leaves (far fewer nodes per
-           ray) carrying those leaf indices as tie-break keys. */
-        if (!sc.bvh_nodes && !reference_bvh && max_depth <= 22) {
+           ray) carrying those leaf indices as tie-break keys -- within
+           |coordinate| <= 1024, where box-test rounding stays far inside the
+           1e-3 leaf-box margin that makes the search order-free. */
+        const PhxBvhNode &rt = nodes[0];
+        float extent = std::max({std::fabs(rt.min.x), std::fabs(rt.min.y), std::fabs(rt.min.z), std::fabs(rt.max.x),
+                                 std::fabs(rt.max.y), std::fabs(rt.max.z)});
+        if (!sc.bvh_nodes && !reference_bvh && max_depth <= 22 && extent <= 1024.0f) {
             std::vector<uint32_t> key(n);
             for (uint32_t i = 0; i < nodes.size(); i++) {
                 uint32_t t = phx_as_uint(nodes[i].min.w);
