@@ -450,6 +450,20 @@ struct GatherArgs {
 constexpr u32 GATHER_BATCH = 8;
 constexpr u32 GATHER_BATCH_DECODED = 4;
 
+/* Marks a loaded value as used here, so that its load stays with the rest of
+ * its batch instead of being moved into the branch that reads it (which would
+ * cost a memory round trip of its own on every pass through that branch). */
+PHX_DEV void keep_loaded(const F4 &v) { asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w)); }
+
+/* Gaussian weight of a photon at squared distance dsq < r_sq (photon_hash.h
+ * 86-90): the exponent lies in (-1, 0] (or is NaN), where phx_expf_normal
+ * gives phx_expf's bits without its range branches. */
+PHX_DEV f32 gauss_weight(f32 dsq, f32 two_r_sq) {
+    f32 x = -GAUSS_BETA * dsq / two_r_sq;
+    f32 e = (x == x) ? phx_expf_normal(x) : x;
+    return GAUSS_ALPHA * (1.0f - (1.0f - e) / GAUSS_DENOM);
+}
+
 /* The photons of one cell, records [s0, e0), that lie within the radius of
  * a hitpoint (photon_hash.h:74-98): Gaussian weights and weighted powers added
  * to nf / found in storage order.  DECODED reads the records' decoded
@@ -469,6 +483,11 @@ PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32
             }
 #pragma unroll
             for (u32 j = 0; j < GATHER_BATCH_DECODED; j++) {
+                keep_loaded(pd[j]);
+                keep_loaded(pw[j]);
+            }
+#pragma unroll
+            for (u32 j = 0; j < GATHER_BATCH_DECODED; j++) {
                 if (k + j >= e0)
                     break;
                 F4 dd = pos - pp[j];
@@ -477,7 +496,7 @@ PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32
                     continue;
                 if (dot(pd[j], sn) > 0.0f)
                     continue;
-                f32 wgt = GAUSS_ALPHA * (1.0f - (1.0f - phx_expf(-GAUSS_BETA * dsq / two_r_sq)) / GAUSS_DENOM);
+                f32 wgt = gauss_weight(dsq, two_r_sq);
                 nf = nf + wgt * pw[j];
                 found += wgt;
             }
@@ -505,7 +524,7 @@ PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32
             F4 pd = decode_oct(pdr[j]);
             if (dot(pd, sn) > 0.0f)
                 continue;
-            f32 wgt = GAUSS_ALPHA * (1.0f - (1.0f - phx_expf(-GAUSS_BETA * dsq / two_r_sq)) / GAUSS_DENOM);
+            f32 wgt = gauss_weight(dsq, two_r_sq);
             nf = nf + wgt * decode_rgbe(pw[j]);
             found += wgt;
         }

@@ -186,13 +186,9 @@ PHX_HD float phx_cosf(float x) {
 #define PHX_LN2_1 0.693115234375f         /* 0x1.62ep-1, 12 bits */
 #define PHX_LN2_2 3.194618329871446e-05f  /* 0x1.0bfbe8p-15 */
 
-PHX_HD float phx_expf(float x) {
-    if (x != x)
-        return x;
-    if (x > 88.72283935f)
-        return phx_as_float(0x7f800000u);
-    if (x < -103.97208405f)
-        return 0.0f;
+/* exp(x) = 2^k · p(r), k = nearest integer to x/ln2, r = x − k·ln2 (two-part
+ * ln2), p the degree-7 Taylor polynomial; returns k, p in *p_out. */
+PHX_HD int phx_exp_reduce(float x, float *p_out) {
     float k = phx_floorf(x * PHX_LOG2E + 0.5f);
     float r = (x - k * PHX_LN2_1) - k * PHX_LN2_2;
     float p = 1.984126984e-4f;          /* 1/5040 */
@@ -203,12 +199,32 @@ PHX_HD float phx_expf(float x) {
     p = 0.5f + r * p;
     p = 1.0f + r * p;
     p = 1.0f + r * p;
-    int ki = (int)k;
+    *p_out = p;
+    return (int)k;
+}
+
+PHX_HD float phx_expf(float x) {
+    if (x != x)
+        return x;
+    if (x > 88.72283935f)
+        return phx_as_float(0x7f800000u);
+    if (x < -103.97208405f)
+        return 0.0f;
+    float p;
+    int ki = phx_exp_reduce(x, &p);
     if (ki > 127)
         return (p * 2.0f) * phx_ldexp1(ki - 1);
     if (ki < -126)
         return (p * phx_ldexp1(ki + 64)) * 5.421010862427522e-20f; /* ·2^-64, one rounding */
     return p * phx_ldexp1(ki);
+}
+
+/* phx_expf for -87 <= x <= 88 (2^k a normal float): the same operations
+ * without the range branches, so the same bits. */
+PHX_HD float phx_expf_normal(float x) {
+    float p;
+    int ki = phx_exp_reduce(x, &p);
+    return p * phx_as_float((uint32_t)(ki + 127) << 23);
 }
 
 PHX_HD float phx_logf(float x) {
