@@ -61,6 +61,14 @@ typedef struct PHX_ALIGN16 PhxTriangleUvs {
  * max.w = as_float(index of the right child) (src/renderer/bvh.cpp:59-72). */
 typedef struct PHX_ALIGN16 PhxBvhNode { phx_float4 min, max; } PhxBvhNode;
 
+/* No counterpart in the reference: one interior node of a binary tree with
+ * both children's boxes, 64 bytes (one cache line), so a search step tests
+ * both children with one load.  lmin.w / rmin.w = as_float(child reference):
+ * PHX_PAIR_LEAF | triangle index for a leaf child, else the child's pair
+ * index; lmax.w = rmax.w = 0. */
+typedef struct PHX_ALIGN16 PhxBvhPair { phx_float4 lmin, lmax, rmin, rmax; } PhxBvhPair;
+#define PHX_PAIR_LEAF 0x80000000u
+
 /* include/common/material.h:7-27 — 32 / 192 bytes */
 typedef struct PHX_ALIGN16 PhxUvTransform {
     phx_float2 uv_offset;
@@ -287,6 +295,14 @@ uint32_t phx_build_bvh_sah(const PhxTriangleVertices *tri_verts, uint32_t n_tris
                            uint32_t max_depth, PhxBvhNode *out_nodes);
 
 /* photon_hash.h:41-50 build_photon_hash_info(bbox, grid_res), written to *out. */
+/* No counterpart in the reference: the interior nodes of a one-triangle-leaf
+ * tree in the node format above (n_nodes = 2·n_tris − 1, preorder, left child
+ * at index + 1, interior min.w >= 0xFFFFFFF8) as pairs, in the same preorder;
+ * out_pairs holds n_nodes / 2 entries (none for a single leaf).  *root_ref is
+ * the reference of node 0.  Returns the number of pairs, or -1 on a malformed
+ * tree. */
+int phx_build_bvh_pairs(const PhxBvhNode *tree, uint32_t n_nodes, PhxBvhPair *out_pairs, uint32_t *root_ref);
+
 int phx_build_photon_hash_info(const PhxBvhNode *root, uint32_t grid_res, PhxPhotonHashData *out);
 
 /* ======================================================================

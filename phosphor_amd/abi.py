@@ -134,6 +134,7 @@ SIGNATURES = {
     "phx_device_synchronize": (INT,),
     "phx_build_bvh_median": (U32, VP, U32, VP),
     "phx_build_bvh_sah": (U32, VP, U32, VP, U32, VP),
+    "phx_build_bvh_pairs": (INT, VP, U32, VP, VP),
     "phx_build_photon_hash_info": (INT, VP, U32, VP),
     "phx_renderer_create": (INT, C.POINTER(RenderArgs), C.POINTER(SceneDesc), C.POINTER(VP)),
     "phx_renderer_destroy": (None, VP),

@@ -349,6 +349,46 @@ extern "C" uint32_t phx_build_bvh_sah(const PhxTriangleVertices *tri_verts, uint
     return (uint32_t)nodes.size();
 }
 
+extern "C" int phx_build_bvh_pairs(const PhxBvhNode *tree, uint32_t n_nodes, PhxBvhPair *out_pairs,
+                                   uint32_t *root_ref) {
+    if (!tree || !root_ref || n_nodes == 0 || (n_nodes & 1u) == 0 || (n_nodes > 1 && !out_pairs))
+        return -1;
+    /* pair index of every interior node, in preorder */
+    std::vector<uint32_t> pair_of(n_nodes, 0xFFFFFFFFu);
+    uint32_t n_pairs = 0;
+    for (uint32_t i = 0; i < n_nodes; i++)
+        if (phx_as_uint(tree[i].min.w) >= 0xFFFFFFF8u)
+            pair_of[i] = n_pairs++;
+    if (n_pairs != n_nodes / 2)
+        return -1;
+    auto ref = [&](uint32_t i) -> uint32_t {
+        uint32_t w = phx_as_uint(tree[i].min.w);
+        return w >= 0xFFFFFFF8u ? pair_of[i] : (PHX_PAIR_LEAF | w);
+    };
+    for (uint32_t i = 0; i < n_nodes; i++) {
+        if (pair_of[i] == 0xFFFFFFFFu)
+            continue;
+        uint32_t l = i + 1, r = phx_as_uint(tree[i].max.w);
+        if (l >= n_nodes || r >= n_nodes || r <= i)
+            return -1;
+        if (phx_as_uint(tree[l].min.w) < 0xFFFFFFF8u && phx_as_uint(tree[l].min.w) >= PHX_PAIR_LEAF)
+            return -1;
+        if (phx_as_uint(tree[r].min.w) < 0xFFFFFFF8u && phx_as_uint(tree[r].min.w) >= PHX_PAIR_LEAF)
+            return -1;
+        PhxBvhPair &p = out_pairs[pair_of[i]];
+        p.lmin = tree[l].min;
+        p.lmax = tree[l].max;
+        p.rmin = tree[r].min;
+        p.rmax = tree[r].max;
+        p.lmin.w = phx_as_float(ref(l));
+        p.rmin.w = phx_as_float(ref(r));
+        p.lmax.w = 0.0f;
+        p.rmax.w = 0.0f;
+    }
+    *root_ref = ref(0);
+    return (int)n_pairs;
+}
+
 extern "C" int phx_build_photon_hash_info(const PhxBvhNode *root, uint32_t grid_res, PhxPhotonHashData *out) {
     /* photon_hash.h:41-50 */
     if (!root || !out || grid_res == 0)

@@ -41,10 +41,14 @@ struct SceneArgs {
     float scene_radius;
     const float *envmap;
     uint32_t env_w, env_h, has_env;
-    /* 1: `nodes` is the library's SAH tree (interior min.w = 0xFFFFFFF8 |
-     * visiting hint, leaf max.w = reference leaf index), searched near-first;
-     * 0: `nodes` is the reference tree, searched depth-first. */
+    /* 1: search `pairs`, the library's SAH tree (root `root_ref`) with both
+     * children's boxes in each node, nearer child first, equal-distance ties
+     * settled by tri_key (each triangle's leaf index in the reference tree);
+     * 0: search `nodes`, the reference tree, depth-first. */
     uint32_t near_first;
+    uint32_t root_ref;
+    const PhxBvhPair *pairs;
+    const uint32_t *tri_key;
 };
 
 /* ---- scan (single pass, decoupled look-back) ---- */

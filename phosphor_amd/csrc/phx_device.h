@@ -350,11 +350,10 @@ PHX_DEV bool traverse(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, Hit
 }
 
 /* The hit traverse<ANY> finds on the reference tree, found on the library's
- * SAH tree of the same leaves (bvh.cpp phx_build_bvh_sah).  At an interior
- * node the child nearer along the ray on the split axis (hint bits 0-1 of
- * min.w; bit 2 set when the left child lies above the right one) is entered
- * first, so the closest hit tends to be found early and more of the other
- * subtrees fail their box test.
+ * SAH tree of the same leaves (bvh.cpp phx_build_bvh_sah), stored as pairs:
+ * each step loads one 64-byte node holding both children's boxes, tests both,
+ * enters the one with the nearer entry point and pushes the other.  A leaf
+ * child is a triangle reference, tested when its turn comes.
  *
  * Why the result is the same.  The reference's depth-first search with its
  * strict box test (hi > lo) returns the hit of least t; among several hits at
@@ -365,82 +364,104 @@ PHX_DEV bool traverse(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, Hit
  * box fails the strict test.  The leaf boxes of both trees are the triangle
  * boxes expanded by 1e-3, so a leaf holding a hit at t in [t_min, closest] has
  * entry < t < exit by that margin and passes the inclusive test below
- * whenever it is tested; a node is dropped only when no point of its box lies
- * in [t_min, closest].  So every candidate is met whatever the order, and
- * "least t; at equal t the larger reference leaf index (kept in each SAH
- * leaf's max.w), at t == t_min the smaller" is the reference's hit -- provided
- * the reference loses nothing to its stack-overflow rule (hit.h:163).  The
- * host builds the SAH tree only when no interior node of the reference tree
- * is deeper than 22, and builds it no deeper than 22 itself, so neither
- * search can overflow (both keep one pending sibling per interior ancestor at
- * most), and only for scenes within |coordinate| <= 1024, where the box test's
- * rounding (a few 1e-4 at most) stays inside the 1e-3 margin.  ANY (any hit
- * in [t_min, t_max], for shadow rays in opaque scenes): the reference's
+ * whenever it is tested; a subtree is dropped only when no point of its box
+ * lies in [t_min, closest].  So every candidate triangle is met whatever the
+ * order, and "least t; at equal t the larger reference leaf index (tri_key),
+ * at t == t_min the smaller" is the reference's hit -- provided the reference
+ * loses nothing to its stack-overflow rule (hit.h:163).  The host builds the
+ * SAH tree only when no interior node of the reference tree is deeper than
+ * 22, and builds it no deeper than 22 itself, so neither search overflows (a
+ * pair search holds at most one pending child per interior level), and only
+ * for scenes within |coordinate| <= 1024, where the box test's rounding (a few
+ * 1e-4 at most) stays inside the 1e-3 leaf-box margin.  ANY (any hit in
+ * [t_min, t_max], for shadow rays in opaque scenes): the reference's
  * closest-hit loop finds a hit exactly when one exists, and so does this
  * search.
  *
  * The box test is the usual fused form t = box * inv + (-origin * inv) with
  * min/max; it only decides which nodes are visited, never the hit record,
  * which is_triangle_hit computes exactly as the reference. */
+struct RayInv {
+    f32 ix, iy, iz, ox, oy, oz;
+};
+
+PHX_DEV RayInv ray_inv(F4 o, F4 d) {
+    const f32 tiny = 1e-20f;
+    RayInv r;
+    r.ix = 1.0f / (phx_fabsf(d.x) > tiny ? d.x : __builtin_copysignf(tiny, d.x));
+    r.iy = 1.0f / (phx_fabsf(d.y) > tiny ? d.y : __builtin_copysignf(tiny, d.y));
+    r.iz = 1.0f / (phx_fabsf(d.z) > tiny ? d.z : __builtin_copysignf(tiny, d.z));
+    r.ox = -o.x * r.ix;
+    r.oy = -o.y * r.iy;
+    r.oz = -o.z * r.iz;
+    return r;
+}
+
+/* entry and exit of [t_min, closest] through a box; it is hit when lo <= hi */
+PHX_DEV void box_span(const RayInv &r, F4 mn, F4 mx, f32 t_min, f32 closest, f32 &lo, f32 &hi) {
+    f32 x0 = __builtin_fmaf(mn.x, r.ix, r.ox), x1 = __builtin_fmaf(mx.x, r.ix, r.ox);
+    f32 y0 = __builtin_fmaf(mn.y, r.iy, r.oy), y1 = __builtin_fmaf(mx.y, r.iy, r.oy);
+    f32 z0 = __builtin_fmaf(mn.z, r.iz, r.oz), z1 = __builtin_fmaf(mx.z, r.iz, r.oz);
+    lo = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(x0, x1), __builtin_fminf(y0, y1)),
+                         __builtin_fmaxf(__builtin_fminf(z0, z1), t_min));
+    hi = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(x0, x1), __builtin_fmaxf(y0, y1)),
+                         __builtin_fminf(__builtin_fmaxf(z0, z1), closest));
+}
+
 template <bool ANY>
-PHX_DEV bool traverse_sah(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
-                          u32 stride) {
+PHX_DEV bool traverse_pairs(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
+                            u32 stride) {
     if (!(t_min < t_max))
         return false; /* the reference's root test fails */
-    const f32 tiny = 1e-20f;
-    f32 ix = 1.0f / (phx_fabsf(d.x) > tiny ? d.x : __builtin_copysignf(tiny, d.x));
-    f32 iy = 1.0f / (phx_fabsf(d.y) > tiny ? d.y : __builtin_copysignf(tiny, d.y));
-    f32 iz = 1.0f / (phx_fabsf(d.z) > tiny ? d.z : __builtin_copysignf(tiny, d.z));
-    f32 ox = -o.x * ix, oy = -o.y * iy, oz = -o.z * iz;
-    u32 neg = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    RayInv ri = ray_inv(o, d);
     f32 closest = t_max;
     bool found = false;
-    u32 best_key = 0;
     int q = -1;
-    u32 index = 0;
+    u32 ref = sc.root_ref;
     for (;;) {
-        const PhxBvhNode *node = sc.nodes + index;
-        F4 nmin = ld(&node->min), nmax = ld(&node->max);
-        f32 x0 = __builtin_fmaf(nmin.x, ix, ox), x1 = __builtin_fmaf(nmax.x, ix, ox);
-        f32 y0 = __builtin_fmaf(nmin.y, iy, oy), y1 = __builtin_fmaf(nmax.y, iy, oy);
-        f32 z0 = __builtin_fmaf(nmin.z, iz, oz), z1 = __builtin_fmaf(nmax.z, iz, oz);
-        f32 lo = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(x0, x1), __builtin_fminf(y0, y1)),
-                                 __builtin_fmaxf(__builtin_fminf(z0, z1), t_min));
-        f32 hi = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(x0, x1), __builtin_fmaxf(y0, y1)),
-                                 __builtin_fminf(__builtin_fmaxf(z0, z1), closest));
-        if (lo <= hi) {
-            u32 tri = phx_as_uint(nmin.w);
-            if (tri < BVH_INTERIOR) {
-                HitRecord cand;
-                if (is_triangle_hit(sc, tri, o, d, t_min, closest, cand)) {
-                    u32 key = phx_as_uint(nmax.w);
-                    if (ANY) {
-                        rec = cand;
-                        found = true;
-                        break;
-                    }
-                    if (!found || cand.t < closest || (cand.t == t_min ? key < best_key : key > best_key)) {
-                        closest = cand.t;
-                        rec = cand;
-                        best_key = key;
-                        found = true;
-                    }
+        if (ref & PHX_PAIR_LEAF) {
+            u32 tri = ref & ~PHX_PAIR_LEAF;
+            HitRecord cand;
+            if (is_triangle_hit(sc, tri, o, d, t_min, closest, cand)) {
+                if (ANY) {
+                    rec = cand;
+                    found = true;
+                    break;
                 }
-            } else {
-                /* right first when the ray runs against the left-to-right
-                   direction of the split axis */
-                u32 hint = tri & 7u;
-                bool right_first = (((neg >> (hint & 3u)) ^ (hint >> 2)) & 1u) != 0;
-                u32 l = index + 1, r = phx_as_uint(nmax.w);
+                bool take = !found || cand.t < closest;
+                if (!take) { /* cand.t == closest: the reference's order decides */
+                    u32 kc = sc.tri_key[tri], kb = sc.tri_key[rec.tri_index];
+                    take = (cand.t == t_min) ? kc < kb : kc > kb;
+                }
+                if (take) {
+                    closest = cand.t;
+                    rec = cand;
+                    found = true;
+                }
+            }
+        } else {
+            const PhxBvhPair *p = sc.pairs + ref;
+            F4 lmin = ld(&p->lmin), lmax = ld(&p->lmax), rmin = ld(&p->rmin), rmax = ld(&p->rmax);
+            f32 llo, lhi, rlo, rhi;
+            box_span(ri, lmin, lmax, t_min, closest, llo, lhi);
+            box_span(ri, rmin, rmax, t_min, closest, rlo, rhi);
+            bool hl = llo <= lhi, hr = rlo <= rhi;
+            u32 lref = phx_as_uint(lmin.w), rref = phx_as_uint(rmin.w);
+            if (hl && hr) {
+                bool right_first = rlo < llo;
                 q++;
-                stk[q * stride] = right_first ? l : r;
-                index = right_first ? r : l;
+                stk[q * stride] = right_first ? lref : rref;
+                ref = right_first ? rref : lref;
+                continue;
+            }
+            if (hl || hr) {
+                ref = hl ? lref : rref;
                 continue;
             }
         }
         if (q < 0)
             break;
-        index = stk[q * stride];
+        ref = stk[q * stride];
         q--;
     }
     if (found)
@@ -453,14 +474,14 @@ PHX_DEV bool traverse_sah(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max,
 PHX_DEV bool scene_any_hit(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
                            u32 stride) {
     if (sc.near_first)
-        return traverse_sah<true>(sc, o, d, t_min, t_max, rec, stk, stride);
+        return traverse_pairs<true>(sc, o, d, t_min, t_max, rec, stk, stride);
     return traverse<true>(sc, o, d, t_min, t_max, rec, stk, stride);
 }
 
 PHX_DEV bool scene_intersect(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
                              u32 stride) {
     if (sc.near_first)
-        return traverse_sah<false>(sc, o, d, t_min, t_max, rec, stk, stride);
+        return traverse_pairs<false>(sc, o, d, t_min, t_max, rec, stk, stride);
     return traverse<false>(sc, o, d, t_min, t_max, rec, stk, stride);
 }
 

@@ -43,6 +43,10 @@ struct GeometryBuffers { /* include/buffers/geometry.hpp */
     DevBuf d_etri_verts, d_etri_normals, d_etri_uvs, d_etri_mat_index;
     uint32_t n_e_triangles = 0;
     DevBuf d_bvh_nodes;
+    /* the SAH tree's pairs and the triangles' reference leaf indices (empty
+       when the reference tree is searched) */
+    DevBuf d_bvh_pairs, d_tri_key;
+    uint32_t root_ref = 0;
 };
 
 struct ShadingBuffers { /* include/buffers/shading.hpp */

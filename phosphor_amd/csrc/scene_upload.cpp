@@ -53,6 +53,9 @@ SceneArgs SceneBuffers::args() const {
     a.env_h = s.envmap_height;
     a.has_env = s.has_envmap;
     a.near_first = shallow_bvh;
+    a.root_ref = g.root_ref;
+    a.pairs = g.d_bvh_pairs.as<PhxBvhPair>();
+    a.tri_key = g.d_tri_key.as<uint32_t>();
     return a;
 }
 
@@ -86,6 +89,8 @@ int upload_scene(const PhxSceneDesc &sc, hipStream_t s, bool reference_bvh, Scen
             return fail(PHX_ERR_INVALID_ARG, "set_scene: BVH build failed");
     }
     out.root = nodes[0];
+    std::vector<uint32_t> key;     /* uploaded below: live until the final synchronisation */
+    std::vector<PhxBvhPair> pairs;
     {
         bool transmissive = false;
         for (uint32_t i = 0; i < sc.n_materials; i++)
@@ -117,14 +122,23 @@ int upload_scene(const PhxSceneDesc &sc, hipStream_t s, bool reference_bvh, Scen
         float extent = std::max({std::fabs(rt.min.x), std::fabs(rt.min.y), std::fabs(rt.min.z), std::fabs(rt.max.x),
                                  std::fabs(rt.max.y), std::fabs(rt.max.z)});
         if (!sc.bvh_nodes && !reference_bvh && max_depth <= 22 && extent <= 1024.0f) {
-            std::vector<uint32_t> key(n);
+            key.assign(n, 0u);
             for (uint32_t i = 0; i < nodes.size(); i++) {
                 uint32_t t = phx_as_uint(nodes[i].min.w);
                 if (t != 0xFFFFFFFFu)
                     key[t] = i;
             }
             std::vector<PhxBvhNode> sah(nodes.size());
-            if (phx_build_bvh_sah(sc.tri_verts, n, key.data(), 22, sah.data()) == nodes.size()) {
+            pairs.resize(std::max<size_t>(1, nodes.size() / 2));
+            uint32_t root_ref = 0;
+            if (phx_build_bvh_sah(sc.tri_verts, n, key.data(), 22, sah.data()) == nodes.size() &&
+                phx_build_bvh_pairs(sah.data(), (uint32_t)sah.size(), pairs.data(), &root_ref) ==
+                    (int)(nodes.size() / 2)) {
+                GeometryBuffers &g = out.geometry;
+                PHX_TRY(copy_to_dev_buf(g.d_bvh_pairs, pairs.data(), nodes.size() / 2, sizeof(PhxBvhPair), s),
+                        "upload");
+                PHX_TRY(copy_to_dev_buf(g.d_tri_key, key.data(), n, sizeof(uint32_t), s), "upload");
+                g.root_ref = root_ref;
                 nodes.swap(sah);
                 out.shallow_bvh = 1;
             }
@@ -201,7 +215,7 @@ int upload_scene(const PhxSceneDesc &sc, hipStream_t s, bool reference_bvh, Scen
         sh.envmap_width = sh.envmap_height = 0;
         PHX_TRY(copy_to_dev_buf(sh.d_envmap_data, nullptr, 0, sizeof(float), s), "upload");
     }
-    /* `nodes` and `lum` are local: their uploads finish before they go */
+    /* `nodes`, `key`, `pairs` and `lum` are local: their uploads finish before they go */
     PHX_TRY(hipStreamSynchronize(s), "upload");
     return PHX_OK;
 }

@@ -133,6 +133,27 @@ def test_sah_tree_structure(phx_lib, which):
             depth[c] = depth[i] + 1
             assert np.all(sah[c, :3] >= sah[i, :3]) and np.all(sah[c, 4:7] <= sah[i, 4:7])
     assert depth[interior].max() <= 22
+    # the same tree as pairs: each interior node's two child boxes and references
+    pairs = np.zeros((n - 1, 16), np.float32)
+    root = np.zeros(1, np.uint32)
+    assert phx_lib.phx_build_bvh_pairs(sah.ctypes.data, 2 * n - 1, pairs.ctypes.data, root.ctypes.data) == n - 1
+    pair_of = np.full(2 * n - 1, -1, np.int64)
+    pair_of[np.nonzero(interior)[0]] = np.arange(n - 1)
+
+    def ref(i):
+        return int(pair_of[i]) if interior[i] else 0x80000000 | int(w_min[i])
+
+    assert root[0] == ref(0)
+    leaves = []
+    for i in np.nonzero(interior)[0]:
+        p = pairs[pair_of[i]]
+        l, r = i + 1, int(w_max[i])
+        assert np.array_equal(p[[0, 1, 2, 4, 5, 6]], sah[l, [0, 1, 2, 4, 5, 6]])
+        assert np.array_equal(p[[8, 9, 10, 12, 13, 14]], sah[r, [0, 1, 2, 4, 5, 6]])
+        refs = p[[3, 11]].view(np.uint32)
+        assert refs[0] == ref(l) and refs[1] == ref(r)
+        leaves += [int(x) & 0x7FFFFFFF for x in refs if x & 0x80000000]
+    assert sorted(leaves) == list(range(n))
 
 
 def test_sah_tree_depth_limit(phx_lib):
