@@ -39,7 +39,7 @@ struct PhxBatch {
     DevBuf d_table, d_chunk_scene, d_tile_scene, d_ph_base, d_dep_base, d_rng, d_round_seed, d_counts, d_zero;
     DevBuf d_tickets, d_counter;
     DevBuf d_hp[6], d_flux, d_weight, d_radiance;
-    DevBuf d_dep_pos, d_dep_power, d_dep_dir, d_map_pos, d_map_power, d_map_dir, d_hashes, d_map_powf, d_map_dirf;
+    DevBuf d_dep_pos, d_dep_power, d_dep_dir, d_map_pos, d_map_power, d_map_dir, d_hashes, d_map_dirpow;
     DevBuf d_cell_counts, d_cell_start, d_cell_end;
     DevBuf d_trace_ws, d_emit_ws, d_scan_ws, d_sort_ws;
     BatchPlan plan;
@@ -195,8 +195,7 @@ int PhxBatch::build(const PhxSceneDesc *descs) {
     PHX_TRY(d_map_power.alloc(deps * 4), "batch photon map");
     PHX_TRY(d_map_dir.alloc(deps * 4), "batch photon map");
     PHX_TRY(d_hashes.alloc(deps * 4), "batch photon map");
-    PHX_TRY(d_map_powf.alloc(deps * sizeof(phx_float4)), "batch photon map");
-    PHX_TRY(d_map_dirf.alloc(deps * sizeof(phx_float4)), "batch photon map");
+    PHX_TRY(d_map_dirpow.alloc(deps * sizeof(phx_float4)), "batch photon map");
     PHX_TRY(d_cell_counts.alloc(buckets * 4), "batch cells");
     PHX_TRY(d_cell_start.alloc(buckets * 4), "batch cells");
     PHX_TRY(d_cell_end.alloc(buckets * 4), "batch cells");
@@ -230,8 +229,7 @@ int PhxBatch::build(const PhxSceneDesc *descs) {
     P.map_power = d_map_power.as<uint32_t>();
     P.map_dir = d_map_dir.as<uint32_t>();
     P.hashes = d_hashes.as<uint32_t>();
-    P.map_powf = d_map_powf.as<phx_float4>();
-    P.map_dirf = d_map_dirf.as<phx_float4>();
+    P.map_dirpow = d_map_dirpow.as<phx_float4>();
     P.cell_counts = d_cell_counts.as<uint32_t>();
     P.cell_start = d_cell_start.as<uint32_t>();
     P.cell_end = d_cell_end.as<uint32_t>();

@@ -109,11 +109,11 @@ hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, 
                          const uint32_t *cell_start, const uint32_t *cell_end, const PhxPhotonHashData &d,
                          float alpha, phx_float4 *flux, float *photon_weight, phx_float4 *radiance, uint32_t width,
                          hipStream_t s, float *rec_ratio = nullptr, float *rec_r2 = nullptr,
-                         const phx_float4 *pdirf = nullptr, const phx_float4 *ppowf = nullptr);
-/* decoded direction and power of every stored photon record (count on the
- * device, clamped to max_n) */
+                         const phx_float4 *pdirpow = nullptr);
+/* every stored photon record's decoded direction with its packed power in w
+ * (count on the device, clamped to max_n) */
 hipError_t launch_decode_photons(const uint32_t *ppow, const uint32_t *pdir, const uint32_t *d_n, uint32_t max_n,
-                                 phx_float4 *powf, phx_float4 *dirf, hipStream_t s);
+                                 phx_float4 *dirpow, hipStream_t s);
 
 /* ---- multi-scene passes ---- */
 /* One scene of a batch: its device scene, camera, grid and its ranges in the
@@ -152,7 +152,7 @@ struct BatchPlan {
     phx_float4 *dep_pos = nullptr, *map_pos = nullptr;
     uint32_t *dep_power = nullptr, *dep_dir = nullptr, *map_power = nullptr, *map_dir = nullptr;
     uint32_t *hashes = nullptr, *cell_counts = nullptr, *cell_start = nullptr, *cell_end = nullptr;
-    phx_float4 *map_powf = nullptr, *map_dirf = nullptr; /* decoded records */
+    phx_float4 *map_dirpow = nullptr; /* decoded direction, packed power in w */
     uint32_t dep_cap = 0, total_buckets = 0, total_tiles = 0;
     void *trace_ws = nullptr, *emit_ws = nullptr, *scan_ws = nullptr, *sort_ws = nullptr;
     uint32_t trace_max_pixels = 0, max_direct_samples = 0, emit_max_photons = 0, max_bounces = 0;

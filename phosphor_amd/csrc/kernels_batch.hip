@@ -387,12 +387,12 @@ hipError_t launch_batch_pass(const BatchPlan &P, hipStream_t s) {
     BTRY(launch_scan(P.cell_counts, P.total_buckets, P.cell_start, P.cell_end, nullptr, P.scan_ws, s));
     BTRY(launch_sort(P.dep_pos, P.dep_power, P.dep_dir, P.hashes, P.cell_start, P.cell_end, P.dep_cap,
                      P.total_buckets, P.map_pos, P.map_power, P.map_dir, P.sort_ws, s, P.dep_counter));
-    BTRY(launch_decode_photons(P.map_power, P.map_dir, P.dep_counter, P.dep_cap, P.map_powf, P.map_dirf, s));
+    BTRY(launch_decode_photons(P.map_power, P.map_dir, P.dep_counter, P.dep_cap, P.map_dirpow, s));
 
     /* gather_flux over every tile of every scene */
     GatherArgs ga{P.hp[0], P.hp[1], P.hp[2], P.hp[3], P.hp[4], P.hp[5], 0u, P.map_pos, P.map_power, P.map_dir,
                   P.cell_start, P.cell_end, PhxPhotonHashData{}, 0.0f, P.flux, P.photon_weight, P.radiance,
-                  nullptr, nullptr, P.map_dirf, P.map_powf};
+                  nullptr, nullptr, P.map_dirpow};
     BGatherArgs gargs{P.scenes, P.tile_scene, P.total_tiles, ga};
     hipLaunchKernelGGL(k_bgather, dim3((P.total_tiles + 3) / 4), dim3(256), 0, s, gargs);
     return hipGetLastError();

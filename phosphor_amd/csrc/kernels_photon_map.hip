@@ -554,22 +554,25 @@ template <bool DECODED> __global__ __launch_bounds__(256) void k_gather(GatherAr
 
 /* decode_oct / decode_rgbe of every stored photon record, once per pass, for
  * the gather (the same values it would decode at every visit) */
+/* dirpow[i] = (decode_oct(pdir[i]).xyz, as_float(ppow[i])): the direction
+ * the gather tests decoded once per pass, the power still packed (decoded
+ * only for photons that pass both tests) */
 __global__ __launch_bounds__(256) void k_decode_photons(const u32 *__restrict__ ppow, const u32 *__restrict__ pdir,
                                                         const u32 *__restrict__ d_n, u32 max_n,
-                                                        phx_float4 *__restrict__ powf, phx_float4 *__restrict__ dirf) {
+                                                        phx_float4 *__restrict__ dirpow) {
     u32 n = d_n ? *d_n : max_n;
     n = n < max_n ? n : max_n;
     for (u32 i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        st(powf + i, decode_rgbe(ppow[i]));
-        st(dirf + i, decode_oct(pdir[i]));
+        F4 dir = decode_oct(pdir[i]);
+        st(dirpow + i, mk(dir.x, dir.y, dir.z, phx_as_float(ppow[i])));
     }
 }
 
-hipError_t launch_decode_photons(const u32 *ppow, const u32 *pdir, const u32 *d_n, u32 max_n, phx_float4 *powf,
-                                 phx_float4 *dirf, hipStream_t s) {
+hipError_t launch_decode_photons(const u32 *ppow, const u32 *pdir, const u32 *d_n, u32 max_n, phx_float4 *dirpow,
+                                 hipStream_t s) {
     u32 blocks = (max_n + 255) / 256;
     blocks = blocks > 8192 ? 8192 : (blocks ? blocks : 1);
-    hipLaunchKernelGGL(k_decode_photons, dim3(blocks), dim3(256), 0, s, ppow, pdir, d_n, max_n, powf, dirf);
+    hipLaunchKernelGGL(k_decode_photons, dim3(blocks), dim3(256), 0, s, ppow, pdir, d_n, max_n, dirpow);
     return hipGetLastError();
 }
 
@@ -578,14 +581,14 @@ hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, 
                          u32 n_pixels, const phx_float4 *ppos, const u32 *ppow, const u32 *pdir, const u32 *cell_start,
                          const u32 *cell_end, const PhxPhotonHashData &d, f32 alpha, phx_float4 *flux,
                          f32 *photon_weight, phx_float4 *radiance, u32 width, hipStream_t s, f32 *rec_ratio,
-                         f32 *rec_r2, const phx_float4 *pdirf, const phx_float4 *ppowf) {
+                         f32 *rec_r2, const phx_float4 *pdirpow) {
     if (n_pixels == 0)
         return hipSuccess;
     GatherArgs a{hp_pos,     hp_normal, hp_throughput, hp_base_color, hp_direct, hp_emission,
                  n_pixels,   ppos,      ppow,          pdir,          cell_start, cell_end,
                  d,          alpha,     flux,          photon_weight, radiance,  rec_ratio,
-                 rec_r2,     pdirf,     ppowf};
-    bool dec = pdirf && ppowf;
+                 rec_r2,     pdirpow};
+    bool dec = pdirpow != nullptr;
     if (width == 0) { /* image shape unknown (C ABI call): flat pixel range */
         if (dec)
             hipLaunchKernelGGL(k_gather<true>, dim3((n_pixels + 255) / 256), dim3(256), 0, s, a, 0u, 0u);

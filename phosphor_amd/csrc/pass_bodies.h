@@ -440,15 +440,15 @@ struct GatherArgs {
     /* optional (adjoint forward): the pass's radius ratio (1 where the state
        is not updated) and r² per pixel */
     f32 *rec_ratio, *rec_r2;
-    /* optional: every stored record's direction and power already decoded
-       (decode_oct / decode_rgbe of pdir / ppow, the same values) */
-    const phx_float4 *pdirf, *ppowf;
+    /* optional: every stored record's direction already decoded (decode_oct
+       of pdir, the same values) with its packed power (ppow) in w */
+    const phx_float4 *pdirpow;
 };
 
 /* photons of a cell whose records are loaded together before any is used:
  * one memory round trip per batch of photons instead of one or two each */
 constexpr u32 GATHER_BATCH = 8;
-constexpr u32 GATHER_BATCH_DECODED = 4;
+constexpr u32 GATHER_BATCH_DECODED = 8;
 
 /* Marks a loaded value as used here, so that its load stays with the rest of
  * its batch instead of being moved into the branch that reads it (which would
@@ -467,25 +467,25 @@ PHX_DEV f32 gauss_weight(f32 dsq, f32 two_r_sq) {
 /* The photons of one cell, records [s0, e0), that lie within the radius of
  * a hitpoint (photon_hash.h:74-98): Gaussian weights and weighted powers added
  * to nf / found in storage order.  DECODED reads the records' decoded
- * direction and power instead of decoding them per visit. */
+ * direction (with the packed power) instead of decoding it per visit: the
+ * direction test dot(decode_oct(d), sn) is ((x·sn.x + y·sn.y) + z·sn.z) +
+ * 0·sn.w, the last term (decode_oct's w is +0) being the lane's constant wz. */
 template <bool DECODED>
 PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32 r_sq, f32 two_r_sq, F4 &nf,
                          f32 &found) {
     if (DECODED) {
+        const f32 wz = 0.0f * sn.w;
         for (u32 k = s0; k < e0; k += GATHER_BATCH_DECODED) {
-            F4 pp[GATHER_BATCH_DECODED], pd[GATHER_BATCH_DECODED], pw[GATHER_BATCH_DECODED];
+            F4 pp[GATHER_BATCH_DECODED], pd[GATHER_BATCH_DECODED];
 #pragma unroll
             for (u32 j = 0; j < GATHER_BATCH_DECODED; j++) {
                 u32 kk = k + j < e0 ? k + j : e0 - 1; /* in range: k < e0 */
                 pp[j] = ld(a.ppos + kk);
-                pd[j] = ld(a.pdirf + kk);
-                pw[j] = ld(a.ppowf + kk);
+                pd[j] = ld(a.pdirpow + kk);
             }
 #pragma unroll
-            for (u32 j = 0; j < GATHER_BATCH_DECODED; j++) {
+            for (u32 j = 0; j < GATHER_BATCH_DECODED; j++)
                 keep_loaded(pd[j]);
-                keep_loaded(pw[j]);
-            }
 #pragma unroll
             for (u32 j = 0; j < GATHER_BATCH_DECODED; j++) {
                 if (k + j >= e0)
@@ -494,10 +494,10 @@ PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32
                 f32 dsq = dot(dd, dd);
                 if (dsq >= r_sq)
                     continue;
-                if (dot(pd[j], sn) > 0.0f)
+                if (((pd[j].x * sn.x + pd[j].y * sn.y) + pd[j].z * sn.z) + wz > 0.0f)
                     continue;
                 f32 wgt = gauss_weight(dsq, two_r_sq);
-                nf = nf + wgt * pw[j];
+                nf = nf + wgt * decode_rgbe(phx_as_uint(pd[j].w));
                 found += wgt;
             }
         }

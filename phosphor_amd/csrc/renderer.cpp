@@ -111,8 +111,7 @@ int PhxRenderer::set_scene(const PhxSceneDesc &sc) {
     PHX_TRY(p.d_photon_power.alloc((size_t)photon_buf_size * sizeof(uint32_t)), "alloc photon map");
     PHX_TRY(p.d_photon_dir.alloc((size_t)photon_buf_size * sizeof(uint32_t)), "alloc photon map");
     PHX_TRY(p.d_photon_hashes.alloc((size_t)photon_buf_size * sizeof(uint32_t)), "alloc photon map");
-    PHX_TRY(p.d_photon_powf.alloc((size_t)photon_buf_size * sizeof(phx_float4)), "alloc photon map");
-    PHX_TRY(p.d_photon_dirf.alloc((size_t)photon_buf_size * sizeof(phx_float4)), "alloc photon map");
+    PHX_TRY(p.d_photon_dirpow.alloc((size_t)photon_buf_size * sizeof(phx_float4)), "alloc photon map");
     PHX_TRY(p.d_cell_counts.alloc((size_t)bucket_count * sizeof(uint32_t)), "alloc cells");
     PHX_TRY(p.d_cell_start.alloc((size_t)bucket_count * sizeof(uint32_t)), "alloc cells");
     PHX_TRY(p.d_cell_end.alloc((size_t)bucket_count * sizeof(uint32_t)), "alloc cells");
@@ -196,7 +195,7 @@ int PhxRenderer::hash_and_sort_photons() {
                         sort_perm),
             "sort_photons");
     PHX_TRY(launch_decode_photons(p.d_photon_power.as<uint32_t>(), p.d_photon_dir.as<uint32_t>(), d_n, n_max,
-                                  p.d_photon_powf.as<phx_float4>(), p.d_photon_dirf.as<phx_float4>(), stream),
+                                  p.d_photon_dirpow.as<phx_float4>(), stream),
             "decode_photons");
     return PHX_OK;
 }
@@ -213,7 +212,7 @@ int PhxRenderer::gather_flux() {
                                     p.d_cell_end.as<uint32_t>(), photon_hash_info, args.sppm_alpha,
                                     i.d_flux.as<phx_float4>(), i.d_photon_weight.as<float>(),
                                     i.d_radiance.as<phx_float4>(), i.image_width, stream, rec_ratio, rec_r2,
-                                    p.d_photon_dirf.as<phx_float4>(), p.d_photon_powf.as<phx_float4>()),
+                                    p.d_photon_dirpow.as<phx_float4>()),
                       "gather_flux");
 }
 

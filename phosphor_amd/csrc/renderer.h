@@ -14,7 +14,7 @@ namespace phx {
 
 struct PhotonBuffers { /* include/buffers/photon.hpp (sorted photon map) */
     DevBuf d_photon_pos, d_photon_power, d_photon_dir, d_photon_hashes;
-    DevBuf d_photon_powf, d_photon_dirf; /* decoded power and direction of each record */
+    DevBuf d_photon_dirpow; /* each record's decoded direction, packed power in w */
     DevBuf d_cell_counts, d_cell_start, d_cell_end;
     uint32_t n_photons = 0;
 };
