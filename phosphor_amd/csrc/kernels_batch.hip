@@ -101,8 +101,10 @@ __global__ __launch_bounds__(WF_SHADOW_BLOCK) void k_bshadow(BShadowArgs a) {
     const WfState &S = a.st;
     u32 count = *S.counter;
     for (u32 slot = blockIdx.x * WF_SHADOW_BLOCK + threadIdx.x; slot < count; slot += gridDim.x * WF_SHADOW_BLOCK) {
-        if (S.sh_pre[slot].w != 0.0f)
-            continue; /* no shadow ray for this sample */
+        if (S.sh_pre[slot].w != 0.0f) { /* no shadow ray for this sample */
+            st(S.sh_tr + slot, BLACK);
+            continue;
+        }
         for_each_scene(S.sh_scene[slot], [&](u32 s0) {
             const BatchScene &B = a.scenes[s0];
             if (B.any_hit)

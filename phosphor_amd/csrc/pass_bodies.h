@@ -45,7 +45,7 @@ struct WfState {
     phx_float4 *sh_o;            /* origin.xyz, t_max in w */
     phx_float4 *sh_d;            /* direction */
     phx_float4 *sh_pre;          /* NEE weight; w = 0 valid, w = -1 no shadow ray (adds BLACK) */
-    phx_float4 *sh_tr;           /* shadow transmittance */
+    phx_float4 *sh_tr;           /* NEE weight × shadow transmittance (BLACK without a shadow ray) */
     u32 *sh_scene;               /* scene of each slot (multi-scene queues only, else null) */
     u32 cap;
 };
@@ -229,13 +229,16 @@ PHX_DEV void camera_path(const SceneView &sc, const PhxCameraParams &cam, u32 w,
     S.meta[wf] = nrec | (kinds << 3) | (env_flag << 15);
 }
 
-/* shadow_transmittance (light_sampling.h:302-333) of one queued shadow ray;
+/* shadow_transmittance (light_sampling.h:302-333) of one queued shadow ray,
+ * stored multiplied by the sample's NEE weight (the term resolve_pixel adds);
  * ANY = any-hit traversal (exact when no material transmits and the stack
  * never overflows). */
 template <bool ANY> PHX_DEV void shadow_one(const SceneView &sc, const WfState &S, u32 slot, u32 *stk) {
     F4 pre = ld(S.sh_pre + slot);
-    if (pre.w != 0.0f)
+    if (pre.w != 0.0f) {
+        st(S.sh_tr + slot, BLACK);
         return;
+    }
     F4 ot = ld(S.sh_o + slot);
     F4 o = mk(ot.x, ot.y, ot.z, 0.0f);
     F4 d = ld(S.sh_d + slot);
@@ -265,7 +268,7 @@ template <bool ANY> PHX_DEV void shadow_one(const SceneView &sc, const WfState &
         }
         t_min = rec.t + EPS;
     }
-    st(S.sh_tr + slot, trans);
+    st(S.sh_tr + slot, pre * trans);
 }
 
 /* emission and direct of one pixel from its records and shadow results */
@@ -281,15 +284,12 @@ PHX_DEV void resolve_pixel(const WfState &S, u32 n, u32 wf, u32 out, phx_float4 
         if (kind == NEE_NONE)
             continue;
         size_t base = S.rec_base[rbase + k];
+        /* light_sampling.h: sum of weight × transmittance over the samples,
+           BLACK for a sample without a shadow ray (products formed by
+           shadow_one) */
         F4 sum = BLACK;
-        for (u32 d = 0; d < n; d++) {
-            F4 pre = ld(S.sh_pre + base + d);
-            if (pre.w != 0.0f) {
-                sum = sum + BLACK;
-            } else {
-                sum = sum + pre * ld(S.sh_tr + base + d);
-            }
-        }
+        for (u32 d = 0; d < n; d++)
+            sum = sum + ld(S.sh_tr + base + d);
         F4 thr = ld(S.rec_thr + rbase + k);
         if (kind == NEE_SPECULAR) {
             e = e + thr * (sum / (f32)n);
