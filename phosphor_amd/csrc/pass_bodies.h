@@ -341,7 +341,7 @@ PHX_DEV void emit_start_one(const SceneView &sc, u32 seed, u32 index, u32 li, co
 PHX_DEV bool emit_step(const SceneView &sc, const EmitStaging &g, u32 li, u32 depth, u32 max_bounces, F4 &origin,
                        F4 &dir, F4 &power, F4 &throughput, u32 &rng, IorStack &ior, u32 &n, u32 *stk) {
     HitRecord rec;
-    if (!scene_intersect(sc, origin, dir, EPS, INF, rec, stk, EMIT_WF_BLOCK))
+    if (!scene_intersect<true>(sc, origin, dir, EPS, INF, rec, stk, EMIT_WF_BLOCK))
         return false;
     SurfaceHit sh = process_hit(sc, rec, origin, dir);
     const PhxMaterial &mat = sc.materials[sh.mat_index];

@@ -408,7 +408,12 @@ PHX_DEV void box_span(const RayInv &r, F4 mn, F4 mx, f32 t_min, f32 closest, f32
                          __builtin_fminf(__builtin_fmaxf(z0, z1), closest));
 }
 
-template <bool ANY>
+/* GROUP_LEAVES: interior steps run until the lane holds a leaf (or has
+ * nothing left) and the lanes of a wave then test their triangles together;
+ * otherwise each iteration takes one step or one triangle test.  Same hits
+ * either way; grouping pays for incoherent rays (photon paths), the plain
+ * loop for coherent ones (a pixel's shadow rays, camera rays). */
+template <bool ANY, bool GROUP_LEAVES>
 PHX_DEV bool traverse_pairs(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
                             u32 stride) {
     if (!(t_min < t_max))
@@ -418,52 +423,87 @@ PHX_DEV bool traverse_pairs(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_ma
     bool found = false;
     int q = -1;
     u32 ref = sc.root_ref;
-    for (;;) {
-        if (ref & PHX_PAIR_LEAF) {
-            u32 tri = ref & ~PHX_PAIR_LEAF;
-            HitRecord cand;
-            if (is_triangle_hit(sc, tri, o, d, t_min, closest, cand)) {
-                if (ANY) {
-                    rec = cand;
-                    found = true;
-                    break;
-                }
-                bool take = !found || cand.t < closest;
-                if (!take) { /* cand.t == closest: the reference's order decides */
-                    u32 kc = sc.tri_key[tri], kb = sc.tri_key[rec.tri_index];
-                    take = (cand.t == t_min) ? kc < kb : kc > kb;
-                }
-                if (take) {
-                    closest = cand.t;
-                    rec = cand;
-                    found = true;
-                }
-            }
-        } else {
-            const PhxBvhPair *p = sc.pairs + ref;
-            F4 lmin = ld(&p->lmin), lmax = ld(&p->lmax), rmin = ld(&p->rmin), rmax = ld(&p->rmax);
-            f32 llo, lhi, rlo, rhi;
-            box_span(ri, lmin, lmax, t_min, closest, llo, lhi);
-            box_span(ri, rmin, rmax, t_min, closest, rlo, rhi);
-            bool hl = llo <= lhi, hr = rlo <= rhi;
-            u32 lref = phx_as_uint(lmin.w), rref = phx_as_uint(rmin.w);
-            if (hl && hr) {
-                bool right_first = rlo < llo;
-                q++;
-                stk[q * stride] = right_first ? lref : rref;
-                ref = right_first ? rref : lref;
-                continue;
-            }
-            if (hl || hr) {
-                ref = hl ? lref : rref;
-                continue;
-            }
-        }
-        if (q < 0)
-            break;
-        ref = stk[q * stride];
-        q--;
+#define PHX_PAIR_LEAF_TEST(BREAK_STMT)                                                                                 \
+    {                                                                                                                  \
+        u32 tri = ref & ~PHX_PAIR_LEAF;                                                                                \
+        HitRecord cand;                                                                                                \
+        if (is_triangle_hit(sc, tri, o, d, t_min, closest, cand)) {                                                    \
+            if (ANY) {                                                                                                 \
+                rec = cand;                                                                                            \
+                found = true;                                                                                          \
+                BREAK_STMT;                                                                                            \
+            }                                                                                                          \
+            bool take = !found || cand.t < closest;                                                                    \
+            if (!take) { /* cand.t == closest: the reference's order decides */                                        \
+                u32 kc = sc.tri_key[tri], kb = sc.tri_key[rec.tri_index];                                              \
+                take = (cand.t == t_min) ? kc < kb : kc > kb;                                                          \
+            }                                                                                                          \
+            if (take) {                                                                                                \
+                closest = cand.t;                                                                                      \
+                rec = cand;                                                                                            \
+                found = true;                                                                                          \
+            }                                                                                                          \
+        }                                                                                                              \
     }
+#define PHX_PAIR_STEP(NEAR_FAR, ONE, NONE)                                                                             \
+    {                                                                                                                  \
+        const PhxBvhPair *p = sc.pairs + ref;                                                                          \
+        F4 lmin = ld(&p->lmin), lmax = ld(&p->lmax), rmin = ld(&p->rmin), rmax = ld(&p->rmax);                         \
+        f32 llo, lhi, rlo, rhi;                                                                                        \
+        box_span(ri, lmin, lmax, t_min, closest, llo, lhi);                                                            \
+        box_span(ri, rmin, rmax, t_min, closest, rlo, rhi);                                                            \
+        bool hl = llo <= lhi, hr = rlo <= rhi;                                                                         \
+        u32 lref = phx_as_uint(lmin.w), rref = phx_as_uint(rmin.w);                                                    \
+        if (hl && hr) {                                                                                                \
+            bool right_first = rlo < llo;                                                                              \
+            q++;                                                                                                       \
+            stk[q * stride] = right_first ? lref : rref;                                                               \
+            ref = right_first ? rref : lref;                                                                           \
+            NEAR_FAR;                                                                                                  \
+        } else if (hl || hr) {                                                                                         \
+            ref = hl ? lref : rref;                                                                                    \
+            ONE;                                                                                                       \
+        } else {                                                                                                       \
+            NONE;                                                                                                      \
+        }                                                                                                              \
+    }
+    if (GROUP_LEAVES) {
+        /* DONE has the leaf bit but names no triangle (fewer than 2^31 - 1) */
+        constexpr u32 DONE = 0xFFFFFFFFu;
+        for (;;) {
+            while (!(ref & PHX_PAIR_LEAF)) {
+                PHX_PAIR_STEP((void)0, (void)0, {
+                    if (q >= 0) {
+                        ref = stk[q * stride];
+                        q--;
+                    } else {
+                        ref = DONE;
+                    }
+                })
+            }
+            if (ref == DONE)
+                break;
+            PHX_PAIR_LEAF_TEST(break)
+            if (q < 0)
+                break;
+            ref = stk[q * stride];
+            q--;
+        }
+    } else {
+        for (;;) {
+            if (ref & PHX_PAIR_LEAF) {
+                PHX_PAIR_LEAF_TEST(break)
+            } else {
+                PHX_PAIR_STEP(continue, continue, (void)0)
+            }
+            if (q < 0)
+                break;
+            ref = stk[q * stride];
+            q--;
+        }
+    }
+#undef PHX_PAIR_STEP
+#undef PHX_PAIR_LEAF_TEST
     if (found)
         rec.mat_index = sc.tmat[rec.tri_index];
     return found;
@@ -474,14 +514,15 @@ PHX_DEV bool traverse_pairs(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_ma
 PHX_DEV bool scene_any_hit(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
                            u32 stride) {
     if (sc.near_first)
-        return traverse_pairs<true>(sc, o, d, t_min, t_max, rec, stk, stride);
+        return traverse_pairs<true, false>(sc, o, d, t_min, t_max, rec, stk, stride);
     return traverse<true>(sc, o, d, t_min, t_max, rec, stk, stride);
 }
 
+template <bool GROUP_LEAVES = false>
 PHX_DEV bool scene_intersect(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
                              u32 stride) {
     if (sc.near_first)
-        return traverse_pairs<false>(sc, o, d, t_min, t_max, rec, stk, stride);
+        return traverse_pairs<false, GROUP_LEAVES>(sc, o, d, t_min, t_max, rec, stk, stride);
     return traverse<false>(sc, o, d, t_min, t_max, rec, stk, stride);
 }
 
