@@ -456,12 +456,25 @@ constexpr u32 GATHER_BATCH_DECODED = 8;
 PHX_DEV void keep_loaded(const F4 &v) { asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w)); }
 
 /* Gaussian weight of a photon at squared distance dsq < r_sq (photon_hash.h
- * 86-90): the exponent lies in (-1, 0] (or is NaN), where phx_expf_normal
- * gives phx_expf's bits without its range branches. */
-PHX_DEV f32 gauss_weight(f32 dsq, f32 two_r_sq) {
-    f32 x = -GAUSS_BETA * dsq / two_r_sq;
-    f32 e = (x == x) ? phx_expf_normal(x) : x;
-    return GAUSS_ALPHA * (1.0f - (1.0f - e) / GAUSS_DENOM);
+ * 86-90), GAUSS_ALPHA·(1 − (1 − exp(−GAUSS_BETA·dsq / (2r²))) / GAUSS_DENOM)
+ * with the same roundings, inv_two_r_sq = RN(1 / (2r²)):
+ *  - the exponent lies in (-1, 0], where phx_expf_normal gives phx_expf's
+ *    bits without its range branches;
+ *  - both divisions are phx_div_recip, correctly rounded wherever the
+ *    numerator is at least 2^-100 in magnitude; 1 − e is 0 or at least
+ *    2^-24, and a smaller GAUSS_BETA·dsq (a photon within 1e-15 of the
+ *    hitpoint) gives an exponent below 2^-80 (r² is at least 7.5e-7, the
+ *    half-diagonal of a 0.001 cell, gather_begin), where exp = 1 whatever
+ *    its last bits;
+ *  - a NaN distance takes the literal expression (NaN payloads). */
+constexpr f32 GAUSS_INV_DENOM = 1.0f / GAUSS_DENOM;
+
+PHX_DEV f32 gauss_weight(f32 dsq, f32 two_r_sq, f32 inv_two_r_sq) {
+    if (dsq != dsq)
+        return GAUSS_ALPHA * (1.0f - (1.0f - phx_expf(-GAUSS_BETA * dsq / two_r_sq)) / GAUSS_DENOM);
+    f32 x = phx_div_recip(-GAUSS_BETA * dsq, two_r_sq, inv_two_r_sq);
+    f32 e = phx_expf_normal(x);
+    return GAUSS_ALPHA * (1.0f - phx_div_recip(1.0f - e, GAUSS_DENOM, GAUSS_INV_DENOM));
 }
 
 /* The photons of one cell, records [s0, e0), that lie within the radius of
@@ -471,8 +484,8 @@ PHX_DEV f32 gauss_weight(f32 dsq, f32 two_r_sq) {
  * direction test dot(decode_oct(d), sn) is ((x·sn.x + y·sn.y) + z·sn.z) +
  * 0·sn.w, the last term (decode_oct's w is +0) being the lane's constant wz. */
 template <bool DECODED>
-PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32 r_sq, f32 two_r_sq, F4 &nf,
-                         f32 &found) {
+PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32 r_sq, f32 two_r_sq,
+                         f32 inv_two_r_sq, F4 &nf, f32 &found) {
     if (DECODED) {
         const f32 wz = 0.0f * sn.w;
         for (u32 k = s0; k < e0; k += GATHER_BATCH_DECODED) {
@@ -496,7 +509,7 @@ PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32
                     continue;
                 if (((pd[j].x * sn.x + pd[j].y * sn.y) + pd[j].z * sn.z) + wz > 0.0f)
                     continue;
-                f32 wgt = gauss_weight(dsq, two_r_sq);
+                f32 wgt = gauss_weight(dsq, two_r_sq, inv_two_r_sq);
                 nf = nf + wgt * decode_rgbe(phx_as_uint(pd[j].w));
                 found += wgt;
             }
@@ -524,7 +537,7 @@ PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32
             F4 pd = decode_oct(pdr[j]);
             if (dot(pd, sn) > 0.0f)
                 continue;
-            f32 wgt = gauss_weight(dsq, two_r_sq);
+            f32 wgt = gauss_weight(dsq, two_r_sq, inv_two_r_sq);
             nf = nf + wgt * decode_rgbe(pw[j]);
             found += wgt;
         }
@@ -541,7 +554,7 @@ PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32
  * radius, so about 16 of the 27 survive on average.) */
 template <bool DECODED>
 PHX_DEV void gather_cells_lane(const GatherArgs &a, const u32 *cs, const u32 *ce, const PhxPhotonHashData &d, F4 pos,
-                               F4 sn, f32 r_sq, f32 two_r_sq, F4 &nf, f32 &found) {
+                               F4 sn, f32 r_sq, f32 two_r_sq, f32 inv_two_r_sq, F4 &nf, f32 &found) {
     const f32 mx = 1e-4f * d.cell_sizes.x, my = 1e-4f * d.cell_sizes.y, mz = 1e-4f * d.cell_sizes.z;
     for (u32 i = 0; i < 27; i++) {
         i32 cx = (i32)(i % 3) - 1, cy = (i32)((i / 3) % 3) - 1, cz = (i32)((i / 9) % 3) - 1;
@@ -562,7 +575,7 @@ PHX_DEV void gather_cells_lane(const GatherArgs &a, const u32 *cs, const u32 *ce
         f32 gz = phx_fmaxf(0.0f, phx_fmaxf(lz - pos.z, pos.z - (lz + d.cell_sizes.z)) - mz);
         if (gx * gx + gy * gy + gz * gz >= r_sq)
             continue;
-        gather_cell<DECODED>(a, cs[h], ce[h], pos, sn, r_sq, two_r_sq, nf, found);
+        gather_cell<DECODED>(a, cs[h], ce[h], pos, sn, r_sq, two_r_sq, inv_two_r_sq, nf, found);
     }
 }
 
@@ -625,8 +638,9 @@ PHX_DEV void gather_pixel(const GatherArgs &a, u32 tid, const PhxPhotonHashData 
         return;
     F4 nf = BLACK;
     f32 found = 0.0f;
-    gather_cells_lane<DECODED>(a, a.cell_start + bucket_base, a.cell_end + bucket_base, d, pos, sn, r_sq, 2.0f * r_sq,
-                               nf, found);
+    f32 two_r_sq = 2.0f * r_sq;
+    gather_cells_lane<DECODED>(a, a.cell_start + bucket_base, a.cell_end + bucket_base, d, pos, sn, r_sq, two_r_sq,
+                               1.0f / two_r_sq, nf, found);
     gather_end(a, tid, alpha, fsf, wsf, r_sq, nf, found);
 }
 

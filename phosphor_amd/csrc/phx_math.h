@@ -219,6 +219,21 @@ PHX_HD float phx_expf(float x) {
     return p * phx_ldexp1(ki);
 }
 
+/* a / b correctly rounded from y = RN(1/b) with fused multiply-adds
+ * (Markstein 1990): q0 = RN(a·y) lies within 1.5 ulp of a/b; one residual
+ * step r = a − b·q (exact in an fma when q is that close and nothing
+ * underflows), q1 = RN(q0 + r·y), brings it within one ulp; a second step is
+ * then correctly rounded by Markstein's theorem (y within half an ulp of 1/b,
+ * q1 faithful).  Valid for normal b and |a| >= 2^-100, where the residuals
+ * stay normal; callers handle smaller numerators. */
+PHX_HD float phx_div_recip(float a, float b, float y) {
+    float q = a * y;
+    float r = __builtin_fmaf(-q, b, a);
+    q = __builtin_fmaf(r, y, q);
+    r = __builtin_fmaf(-q, b, a);
+    return __builtin_fmaf(r, y, q);
+}
+
 /* phx_expf for -87 <= x <= 88 (2^k a normal float): the same operations
  * without the range branches, so the same bits. */
 PHX_HD float phx_expf_normal(float x) {
