@@ -89,6 +89,40 @@ def run_cpu_baseline(cfg, scene, passes: int, threads: int):
     return work / dt, dt, stages
 
 
+def count_rays(cfg, scene, threads: int):
+    """scene_intersect calls of pass 2 of the seed schedule, counted by the
+    oracle (its paths are the GPU's, bit for bit): (trace, emission)."""
+    import oracle
+
+    oracle.lib().ref_set_num_threads(threads)
+    ref = oracle.RefRenderer(cfg, scene, count_visits=True)
+    ref.next_seed()
+    if ref.run_pass() != 0:
+        raise RuntimeError("oracle pass failed")
+    rays = ref.rays()
+    ref.close()
+    return int(rays[0]), int(rays[1])
+
+
+def converged_rmse(P, cfg, scene, passes: int, ref_passes: int):
+    """Relative RMSE ‖L − L_ref‖₂ / ‖L_ref‖₂ of the linear estimate after
+    `passes` passes against an independent render (seed + 1) of `ref_passes`
+    passes."""
+    import dataclasses
+
+    from phosphor_amd.output import relative_rmse
+
+    r = P.Renderer(dataclasses.replace(cfg, graph=True), scene)
+    r.run(passes)
+    est = r.estimate()
+    r.close()
+    ref = P.Renderer(dataclasses.replace(cfg, seed=cfg.seed + 1, graph=True), scene)
+    ref.run(ref_passes)
+    est_ref = ref.estimate()
+    ref.close()
+    return relative_rmse(est, est_ref)
+
+
 def bench_batch(args) -> int:
     """64 scenes × `steps` passes as one batch (graph replay), then the same
     scenes rendered one by one (graph replay each); both timed host-side
@@ -248,6 +282,9 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0 = all cores of this host)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the known-size copy kernel (counter calibration) over MIB MiB")
+    ap.add_argument("--rmse-passes", type=int, default=64,
+                    help="passes of the estimate whose relative RMSE is reported (0 = skip)")
+    ap.add_argument("--rmse-ref-passes", type=int, default=1024, help="passes of the reference render")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
@@ -294,9 +331,31 @@ def main() -> int:
           f"{', '.join(f'{x:.2f}' for x in mean_stage)} ms, deposits/pass={n_photons}", file=sys.stderr, flush=True)
     r.close()
 
+    quality = None
+    if args.rmse_passes > 0:
+        t1 = time.perf_counter()
+        rm = converged_rmse(P, cfg, scene, args.rmse_passes, args.rmse_ref_passes)
+        quality = {"relative_rmse": rm, "passes": args.rmse_passes, "reference_passes": args.rmse_ref_passes,
+                   "reference": "independent render, seed + 1"}
+        print(f"[bench] relative RMSE after {args.rmse_passes} passes vs a {args.rmse_ref_passes}-pass render: "
+              f"{rm:.4f} ({time.perf_counter() - t1:.1f} s)", file=sys.stderr, flush=True)
+
     cpu = None
+    rays = None
     if args.cpu_passes > 0:
         threads = args.cpu_threads or cpu_cores()
+        try:
+            n_tr, n_em = count_rays(cfg, scene, threads)
+            rays = {"counted_by": "oracle: scene_intersect calls of pass 2 (camera, shadow and photon rays; "
+                                  "the GPU traces the same paths)",
+                    "trace_per_pass": n_tr, "emit_per_pass": n_em,
+                    "trace_rays_per_s": n_tr / (float(mean_stage[0]) * 1e-3),
+                    "emit_rays_per_s": n_em / (float(mean_stage[1]) * 1e-3),
+                    "rays_per_s": (n_tr + n_em) / (float(mean_stage[0] + mean_stage[1]) * 1e-3)}
+            print(f"[bench] rays per pass: trace {n_tr}, emission {n_em} -> "
+                  f"{rays['rays_per_s']:.3e} rays/s in the trace + emission stages", file=sys.stderr, flush=True)
+        except Exception as e:
+            print(f"[bench] ray count skipped: {e}", file=sys.stderr)
         try:
             cpu_value, cpu_dt, cpu_stages = run_cpu_baseline(cfg, scene, args.cpu_passes, threads)
             cpu = {"value": cpu_value, "unit": "photons+rays/s", "cores": threads, "kind": "port",
@@ -331,8 +390,8 @@ def main() -> int:
         "roofline": {"kernel": "k_gather", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": read_traffic("k_gather", args.workload)},
-        "traversal": {"kernel": "k_trace_rays", "camera_rays_per_s": n_px / (float(mean_stage[0]) * 1e-3),
-                      "photons_per_s": cfg.photons_per_round / (float(mean_stage[1]) * 1e-3)},
+        "rays": rays,
+        "quality": quality,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -40,6 +40,7 @@ _SIGS = {
     "ref_renderer_create": (VP, VP, VP),
     "ref_renderer_destroy": (None, VP),
     "ref_renderer_count_visits": (None, VP, INT),
+    "ref_renderer_rays": (None, VP, VP),
     "ref_trace_range": (None, VP, U32, U32, U32),
     "ref_emit_range": (U32, VP, U32, U32, U32, VP, VP, VP, U32),
     "ref_renderer_run_pass": (INT, VP),
@@ -133,6 +134,13 @@ class RefRenderer:
     def visits(self) -> np.ndarray:
         out = np.zeros(3, np.uint64)
         lib().ref_renderer_visits(self._h, _p(out))
+        return out
+
+    def rays(self) -> np.ndarray:
+        """scene_intersect calls of the trace and emission stages so far
+        (counted only with count_visits=True)."""
+        out = np.zeros(2, np.uint64)
+        lib().ref_renderer_rays(self._h, _p(out))
         return out
 
     @property

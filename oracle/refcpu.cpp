@@ -366,9 +366,15 @@ inline bool bbox_hit(const PhxBvhNode &n, V4 o, V4 d, f32 t_min, f32 t_max) {
     return true;
 }
 
+/* scene_intersect calls of this thread while visits are counted (the rays
+ * a pass traces; the caller takes differences around each work item) */
+static thread_local uint64_t tl_ray_calls = 0;
+
 /* hit.h:135-172 (the stack holds 24 entries; a push that would overflow drops the children) */
 inline bool scene_intersect(const Scene &sc, V4 origin, V4 dir, f32 t_min, f32 t_max, HitRecord *rec,
                             uint64_t *node_visits) {
+    if (node_visits)
+        tl_ray_calls++;
     int p = 0;
     u32 stack[BVH_STACK_SIZE];
     stack[0] = 0;
@@ -1492,6 +1498,7 @@ struct RefRenderer {
     std::vector<u32> ppow, pdir, spow, sdir, hashes, counts, cstart, cend;
     double stage_ms[4] = {0, 0, 0, 0};
     uint64_t visits_trace = 0, visits_emit = 0, visits_gather = 0;
+    uint64_t rays_trace = 0, rays_emit = 0;
     bool count_visits = false;
 };
 
@@ -1610,18 +1617,20 @@ static double now_ms() {
 /* Trace only the given pixel range of a pass with `seed` (for bounded CPU samples). */
 void ref_trace_range(RefRenderer *r, u32 seed, u32 first, u32 count) {
     const Scene &sc = r->sc;
-    uint64_t visits = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : visits)
+    uint64_t visits = 0, rays = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : visits, rays)
     for (int64_t i = first; i < (int64_t)first + count; i++) {
         V4 o[6];
-        uint64_t v = 0;
+        uint64_t v = 0, c0 = tl_ray_calls;
         trace_pixel(sc, &r->cam, r->args.width, r->args.height, seed, r->args.direct_samples, (u32)i, o,
                     r->count_visits ? &v : nullptr);
         visits += v;
+        rays += tl_ray_calls - c0;
         for (int k = 0; k < 6; k++)
             r->hp[k][i] = to(o[k]);
     }
     r->visits_trace += visits;
+    r->rays_trace += rays;
 }
 
 /* Emit photons [first, first+count) of a pass with `seed`; returns deposits. */
@@ -1630,14 +1639,16 @@ u32 ref_emit_range(RefRenderer *r, u32 seed, u32 first, u32 count, phx_float4 *p
     u32 B = r->args.max_photon_bounces;
     std::vector<Deposit> dep((size_t)count * B);
     std::vector<u32> cnt(count);
-    uint64_t visits = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : visits)
+    uint64_t visits = 0, rays = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : visits, rays)
     for (int64_t i = 0; i < (int64_t)count; i++) {
-        uint64_t v = 0;
+        uint64_t v = 0, c0 = tl_ray_calls;
         cnt[i] = emit_photon(r->sc, first + (u32)i, seed, B, &dep[(size_t)i * B], r->count_visits ? &v : nullptr);
         visits += v;
+        rays += tl_ray_calls - c0;
     }
     r->visits_emit += visits;
+    r->rays_emit += rays;
     u32 total = 0;
     for (u32 i = 0; i < count; i++)
         for (u32 k = 0; k < cnt[i]; k++) {
@@ -1735,6 +1746,11 @@ void ref_renderer_visits(RefRenderer *r, uint64_t *out3) {
     out3[0] = r->visits_trace;
     out3[1] = r->visits_emit;
     out3[2] = r->visits_gather;
+}
+/* scene_intersect calls (camera, shadow and photon rays) while counting is on */
+void ref_renderer_rays(RefRenderer *r, uint64_t *out2) {
+    out2[0] = r->rays_trace;
+    out2[1] = r->rays_emit;
 }
 u32 ref_renderer_last_photon_count(RefRenderer *r) { return r->last_count; }
 u32 ref_renderer_rounds_done(RefRenderer *r) { return r->rounds; }
