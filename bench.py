@@ -89,25 +89,34 @@ def run_cpu_baseline(cfg, scene, passes: int, threads: int):
     return work / dt, dt, stages
 
 
-def count_rays(cfg, scene, threads: int):
-    """scene_intersect calls of pass 2 of the seed schedule, counted by the
-    oracle (its paths are the GPU's, bit for bit): (trace, emission)."""
+def oracle_pass(P, cfg, scene, threads: int):
+    """One pass of the oracle with ray counting, against one pass of the GPU
+    renderer with the same configuration: scene_intersect calls of the pass
+    (trace, emission) and the relative RMSE ‖L − L_ref‖₂ / ‖L_ref‖₂ of the
+    GPU's linear estimate against the oracle's (0 when bit-identical)."""
     import oracle
+
+    from phosphor_amd.output import relative_rmse
 
     oracle.lib().ref_set_num_threads(threads)
     ref = oracle.RefRenderer(cfg, scene, count_visits=True)
-    ref.next_seed()
     if ref.run_pass() != 0:
         raise RuntimeError("oracle pass failed")
     rays = ref.rays()
+    est_ref = ref.estimate()
     ref.close()
-    return int(rays[0]), int(rays[1])
+    r = P.Renderer(cfg, scene)
+    r.run_pass()
+    est = r.estimate()
+    r.close()
+    same = bool(np.array_equal(np.ascontiguousarray(est).view(np.uint32), np.ascontiguousarray(est_ref).view(np.uint32)))
+    return int(rays[0]), int(rays[1]), relative_rmse(est, est_ref), same
 
 
-def converged_rmse(P, cfg, scene, passes: int, ref_passes: int):
-    """Relative RMSE ‖L − L_ref‖₂ / ‖L_ref‖₂ of the linear estimate after
-    `passes` passes against an independent render (seed + 1) of `ref_passes`
-    passes."""
+def noise_rmse(P, cfg, scene, passes: int, ref_passes: int):
+    """Relative RMSE of the linear estimate after `passes` passes against an
+    independent render (seed + 1) of `ref_passes` passes: the estimator's
+    noise at that pass count, not an error of the implementation."""
     import dataclasses
 
     from phosphor_amd.output import relative_rmse
@@ -282,9 +291,10 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0 = all cores of this host)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the known-size copy kernel (counter calibration) over MIB MiB")
-    ap.add_argument("--rmse-passes", type=int, default=64,
-                    help="passes of the estimate whose relative RMSE is reported (0 = skip)")
-    ap.add_argument("--rmse-ref-passes", type=int, default=1024, help="passes of the reference render")
+    ap.add_argument("--noise-passes", type=int, default=0,
+                    help="report the relative RMSE of an estimate after this many passes against an independent "
+                         "render (0 = skip)")
+    ap.add_argument("--noise-ref-passes", type=int, default=1024, help="passes of that independent render")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
@@ -331,22 +341,26 @@ def main() -> int:
           f"{', '.join(f'{x:.2f}' for x in mean_stage)} ms, deposits/pass={n_photons}", file=sys.stderr, flush=True)
     r.close()
 
-    quality = None
-    if args.rmse_passes > 0:
+    quality = {}
+    if args.noise_passes > 0:
         t1 = time.perf_counter()
-        rm = converged_rmse(P, cfg, scene, args.rmse_passes, args.rmse_ref_passes)
-        quality = {"relative_rmse": rm, "passes": args.rmse_passes, "reference_passes": args.rmse_ref_passes,
-                   "reference": "independent render, seed + 1"}
-        print(f"[bench] relative RMSE after {args.rmse_passes} passes vs a {args.rmse_ref_passes}-pass render: "
-              f"{rm:.4f} ({time.perf_counter() - t1:.1f} s)", file=sys.stderr, flush=True)
+        rm = noise_rmse(P, cfg, scene, args.noise_passes, args.noise_ref_passes)
+        quality["noise"] = {"relative_rmse": rm, "passes": args.noise_passes,
+                            "reference_passes": args.noise_ref_passes, "reference": "independent render, seed + 1"}
+        print(f"[bench] relative RMSE after {args.noise_passes} passes vs an independent "
+              f"{args.noise_ref_passes}-pass render: {rm:.4f} ({time.perf_counter() - t1:.1f} s)",
+              file=sys.stderr, flush=True)
 
     cpu = None
     rays = None
     if args.cpu_passes > 0:
         threads = args.cpu_threads or cpu_cores()
         try:
-            n_tr, n_em = count_rays(cfg, scene, threads)
-            rays = {"counted_by": "oracle: scene_intersect calls of pass 2 (camera, shadow and photon rays; "
+            n_tr, n_em, rm_oracle, same = oracle_pass(P, cfg, scene, threads)
+            quality["vs_oracle"] = {"relative_rmse": rm_oracle, "bit_identical": same, "passes": 1}
+            print(f"[bench] one pass against the oracle: relative RMSE {rm_oracle:.3g}, bit-identical {same}",
+                  file=sys.stderr, flush=True)
+            rays = {"counted_by": "oracle: scene_intersect calls of pass 1 (camera, shadow and photon rays; "
                                   "the GPU traces the same paths)",
                     "trace_per_pass": n_tr, "emit_per_pass": n_em,
                     "trace_rays_per_s": n_tr / (float(mean_stage[0]) * 1e-3),
@@ -391,7 +405,7 @@ def main() -> int:
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": read_traffic("k_gather", args.workload)},
         "rays": rays,
-        "quality": quality,
+        "quality": quality or None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
