@@ -399,7 +399,10 @@ int phx_renderer_read_hitpoints(PhxRenderer *r, phx_float4 *hp6);
 /* Copy the last pass's sorted photon map (n = last clamped count) and cell ranges. */
 int phx_renderer_read_photon_map(PhxRenderer *r, phx_float4 *pos, uint32_t *power, uint32_t *dir,
                                  uint32_t *cell_start, uint32_t *cell_end);
-/* Device pointers of the renderer's buffers (for the Python tooling). */
+/* Device pointers of the renderer's buffers (for the Python tooling): hp_pos …
+ * hp_emission, flux, photon_weight, radiance, sorted pos/power/dir, cell
+ * start/end, unsorted pos/power/dir, hash keys, NULL (the reference's cell
+ * counts: the pass reads cell ranges off the sorted keys instead), stream. */
 int phx_renderer_device_buffers(PhxRenderer *r, void **ptrs, uint32_t n_ptrs);
 /* Final linear estimate (output.cpp:52-83 before exposure/gamma):
  * L = radiance/rounds + flux.xyz/(π r² photons_emitted). */

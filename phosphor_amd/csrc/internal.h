@@ -89,7 +89,8 @@ hipError_t launch_emit(const SceneArgs &sc, uint32_t max_photons, uint32_t offse
 
 hipError_t launch_hash(const phx_float4 *pos, uint32_t n, const PhxPhotonHashData &d, uint32_t *hashes,
                        uint32_t *counts, hipStream_t s);
-/* Same with the photon count read on the device (clamped to max_n). */
+/* Same with the photon count read on the device (clamped to max_n); counts
+ * may be null (keys only). */
 hipError_t launch_hash_dev(const phx_float4 *pos, const uint32_t *d_n, uint32_t max_n, const PhxPhotonHashData &d,
                            uint32_t *hashes, uint32_t *counts, hipStream_t s);
 
@@ -101,7 +102,11 @@ size_t sort_workspace_bytes(uint32_t n, uint32_t n_cells);
 hipError_t launch_sort(const phx_float4 *pos_in, const uint32_t *pow_in, const uint32_t *dir_in,
                        const uint32_t *hashes, const uint32_t *cell_start, uint32_t *cell_end, uint32_t n,
                        uint32_t n_cells, phx_float4 *pos_out, uint32_t *pow_out, uint32_t *dir_out, void *ws,
-                       hipStream_t s, const uint32_t *d_n = nullptr, uint32_t *perm_out = nullptr);
+                       hipStream_t s, const uint32_t *d_n = nullptr, uint32_t *perm_out = nullptr,
+                       uint32_t *cell_start_out = nullptr);
+/* With cell_start_out the cell starts come from the sorted keys (binary
+ * search: the same numbers as the exclusive scan of the cell counts) and
+ * cell_start is ignored, so the caller needs neither counts nor scan. */
 
 hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const phx_float4 *hp_throughput,
                          const phx_float4 *hp_base_color, const phx_float4 *hp_direct, const phx_float4 *hp_emission,

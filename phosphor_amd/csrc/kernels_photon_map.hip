@@ -263,7 +263,8 @@ __global__ __launch_bounds__(256) void k_hash(const phx_float4 *__restrict__ pos
         phx_float4 p = pos[i];
         u32 h = get_hash(p.x, p.y, p.z, d);
         hashes[i] = h;
-        atomicAdd(&counts[h], 1u);
+        if (counts)
+            atomicAdd(&counts[h], 1u);
     }
 }
 
@@ -419,6 +420,8 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(
                 dir_out[dst] = dir_in[src];
                 if (perm)
                     perm[dst] = src;
+                if (final_pass > 1) /* the sorted keys are wanted too */
+                    keys_out[dst] = key[k];
             } else {
                 keys_out[dst] = key[k];
                 vals_out[dst] = val[k];
@@ -432,6 +435,24 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(
 __global__ void k_sort_hist_len(u32 n, const u32 *d_n, u32 *len) {
     n = sort_count(n, d_n);
     *len = (u32)(((size_t)n + SORT_TILE - 1) / SORT_TILE) * RADIX;
+}
+
+/* cell_start[h] = number of sorted keys below h (the exclusive scan of the
+ * cell counts, scan_small_chunks.cl, found by binary search instead) */
+__global__ __launch_bounds__(256) void k_cell_start_search(const u32 *__restrict__ keys, u32 n, const u32 *d_n,
+                                                           u32 n_cells, u32 *__restrict__ cell_start) {
+    n = sort_count(n, d_n);
+    for (u32 h = blockIdx.x * 256 + threadIdx.x; h < n_cells; h += gridDim.x * 256) {
+        u32 lo = 0, hi = n;
+        while (lo < hi) {
+            u32 mid = lo + (hi - lo) / 2;
+            if (keys[mid] < h)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        cell_start[h] = lo;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_cell_end(const u32 *__restrict__ cell_start, u32 *__restrict__ cell_end,
@@ -465,8 +486,10 @@ constexpr u32 SORT_DEV_BLOCKS = 2048;
 
 hipError_t launch_sort(const phx_float4 *pos_in, const u32 *pow_in, const u32 *dir_in, const u32 *hashes,
                        const u32 *cell_start, u32 *cell_end, u32 n, u32 n_cells, phx_float4 *pos_out, u32 *pow_out,
-                       u32 *dir_out, void *ws, hipStream_t s, const u32 *d_n, u32 *perm_out) {
+                       u32 *dir_out, void *ws, hipStream_t s, const u32 *d_n, u32 *perm_out,
+                       u32 *cell_start_out) {
     hipError_t e;
+    const u32 *sorted_keys = hashes; /* n == 0: nothing is read */
     if (n > 0) {
         u32 n_tiles = (u32)(((size_t)n + SORT_TILE - 1) / SORT_TILE);
         u32 grid = n_tiles;
@@ -508,13 +531,25 @@ hipError_t launch_sort(const phx_float4 *pos_in, const u32 *pow_in, const u32 *d
                 return e;
             u32 *kout = kbuf[pass & 1], *vout = vbuf[pass & 1];
             hipLaunchKernelGGL(k_radix_scatter, dim3(grid), dim3(SORT_BLOCK), 0, s, kin, vin, n, d_n, shift, offs,
-                               kout, vout, last ? 1 : 0, pos_in, pow_in, dir_in, pos_out, pow_out, dir_out,
+                               kout, vout, last ? (cell_start_out ? 2 : 1) : 0, pos_in, pow_in, dir_in, pos_out,
+                               pow_out, dir_out,
                                perm_out);
             if ((e = hipGetLastError()) != hipSuccess)
                 return e;
             kin = kout;
             vin = vout;
         }
+        sorted_keys = kin;
+    }
+    if (cell_start_out && n_cells > 0) {
+        u32 blocks = (n_cells + 255) / 256;
+        if (blocks > 4096)
+            blocks = 4096;
+        hipLaunchKernelGGL(k_cell_start_search, dim3(blocks), dim3(256), 0, s, sorted_keys, n, d_n, n_cells,
+                           cell_start_out);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+        cell_start = cell_start_out;
     }
     if (n_cells > 0) {
         u32 blocks = (n_cells + 255) / 256;

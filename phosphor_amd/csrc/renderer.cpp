@@ -112,7 +112,6 @@ int PhxRenderer::set_scene(const PhxSceneDesc &sc) {
     PHX_TRY(p.d_photon_dir.alloc((size_t)photon_buf_size * sizeof(uint32_t)), "alloc photon map");
     PHX_TRY(p.d_photon_hashes.alloc((size_t)photon_buf_size * sizeof(uint32_t)), "alloc photon map");
     PHX_TRY(p.d_photon_dirpow.alloc((size_t)photon_buf_size * sizeof(phx_float4)), "alloc photon map");
-    PHX_TRY(p.d_cell_counts.alloc((size_t)bucket_count * sizeof(uint32_t)), "alloc cells");
     PHX_TRY(p.d_cell_start.alloc((size_t)bucket_count * sizeof(uint32_t)), "alloc cells");
     PHX_TRY(p.d_cell_end.alloc((size_t)bucket_count * sizeof(uint32_t)), "alloc cells");
     /* --batch-size (renderer.cpp:103-111) bounds the reference's device memory;
@@ -181,18 +180,17 @@ int PhxRenderer::hash_and_sort_photons() {
     PhotonBuffers &p = buffers.photon_buf;
     const uint32_t *d_n = d_batch_size.as<uint32_t>();
     uint32_t n_max = photon_buf_size;
-    PHX_TRY(launch_fill_u32(p.d_cell_counts.as<uint32_t>(), 0u, bucket_count, stream), "clear cell counts");
+    /* keys only: the stable sort orders the records, and the cell ranges
+       (the scan of the cell counts, renderer.cpp:134-147) are read off the
+       sorted keys */
     PHX_TRY(launch_hash_dev(d_photon_pos.as<phx_float4>(), d_n, n_max, photon_hash_info,
-                            p.d_photon_hashes.as<uint32_t>(), p.d_cell_counts.as<uint32_t>(), stream),
+                            p.d_photon_hashes.as<uint32_t>(), nullptr, stream),
             "hash_photons");
-    PHX_TRY(launch_scan(p.d_cell_counts.as<uint32_t>(), bucket_count, p.d_cell_start.as<uint32_t>(),
-                        p.d_cell_end.as<uint32_t>(), nullptr, d_scan_ws.p, stream),
-            "scan_cells");
     PHX_TRY(launch_sort(d_photon_pos.as<phx_float4>(), d_photon_power.as<uint32_t>(), d_photon_dir.as<uint32_t>(),
-                        p.d_photon_hashes.as<uint32_t>(), p.d_cell_start.as<uint32_t>(),
-                        p.d_cell_end.as<uint32_t>(), n_max, bucket_count, p.d_photon_pos.as<phx_float4>(),
-                        p.d_photon_power.as<uint32_t>(), p.d_photon_dir.as<uint32_t>(), d_sort_ws.p, stream, d_n,
-                        sort_perm),
+                        p.d_photon_hashes.as<uint32_t>(), nullptr, p.d_cell_end.as<uint32_t>(), n_max, bucket_count,
+                        p.d_photon_pos.as<phx_float4>(), p.d_photon_power.as<uint32_t>(),
+                        p.d_photon_dir.as<uint32_t>(), d_sort_ws.p, stream, d_n, sort_perm,
+                        p.d_cell_start.as<uint32_t>()),
             "sort_photons");
     PHX_TRY(launch_decode_photons(p.d_photon_power.as<uint32_t>(), p.d_photon_dir.as<uint32_t>(), d_n, n_max,
                                   p.d_photon_dirpow.as<phx_float4>(), stream),
@@ -447,7 +445,7 @@ int phx_renderer_device_buffers(PhxRenderer *r, void **ptrs, uint32_t n_ptrs) {
                    i.d_hp_direct.p,    i.d_hp_emission.p,   i.d_flux.p,          i.d_photon_weight.p,
                    i.d_radiance.p,     p.d_photon_pos.p,    p.d_photon_power.p,  p.d_photon_dir.p,
                    p.d_cell_start.p,   p.d_cell_end.p,      r->d_photon_pos.p,   r->d_photon_power.p,
-                   r->d_photon_dir.p,  p.d_photon_hashes.p, p.d_cell_counts.p,   (void *)r->stream};
+                   r->d_photon_dir.p,  p.d_photon_hashes.p, nullptr,             (void *)r->stream};
     uint32_t total = (uint32_t)(sizeof(all) / sizeof(all[0]));
     for (uint32_t k = 0; k < n_ptrs && k < total; k++)
         ptrs[k] = all[k];

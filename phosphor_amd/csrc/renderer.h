@@ -15,7 +15,7 @@ namespace phx {
 struct PhotonBuffers { /* include/buffers/photon.hpp (sorted photon map) */
     DevBuf d_photon_pos, d_photon_power, d_photon_dir, d_photon_hashes;
     DevBuf d_photon_dirpow; /* each record's decoded direction, packed power in w */
-    DevBuf d_cell_counts, d_cell_start, d_cell_end;
+    DevBuf d_cell_start, d_cell_end; /* cell counts are not kept: ranges come from the sorted keys */
     uint32_t n_photons = 0;
 };
 
