@@ -101,7 +101,7 @@ __global__ __launch_bounds__(WF_SHADOW_BLOCK) void k_bshadow(BShadowArgs a) {
     const WfState &S = a.st;
     u32 count = *S.counter;
     for (u32 slot = blockIdx.x * WF_SHADOW_BLOCK + threadIdx.x; slot < count; slot += gridDim.x * WF_SHADOW_BLOCK) {
-        if (S.sh_pre[slot].w != 0.0f) { /* no shadow ray for this sample */
+        if (S.sh_ev[slot] == NO_EVENT) { /* no sample drawn for this slot */
             st(S.sh_tr + slot, BLACK);
             continue;
         }

@@ -347,12 +347,15 @@ WfState wf_layout(void *ws, u32 n_pixels, u32 n, bool with_scene) {
     p += align_up((size_t)n_pixels * sizeof(phx_float4), 256);
     S.occl = (f32 *)p;
     p += align_up((size_t)n_pixels * sizeof(f32), 256);
-    S.sh_o = (phx_float4 *)p;
-    p += align_up(cap * sizeof(phx_float4), 256);
-    S.sh_d = (phx_float4 *)p;
-    p += align_up(cap * sizeof(phx_float4), 256);
-    S.sh_pre = (phx_float4 *)p;
-    p += align_up(cap * sizeof(phx_float4), 256);
+    phx_float4 **ev[5] = {&S.ev_pos, &S.ev_n, &S.ev_view, &S.ev_col, &S.ev_misc};
+    for (phx_float4 **a : ev) {
+        *a = (phx_float4 *)p;
+        p += align_up(rec * sizeof(phx_float4), 256);
+    }
+    S.sh_rng = (u32 *)p;
+    p += align_up(cap * sizeof(u32), 256);
+    S.sh_ev = (u32 *)p;
+    p += align_up(cap * sizeof(u32), 256);
     S.sh_tr = (phx_float4 *)p;
     p += align_up(cap * sizeof(phx_float4), 256);
     S.sh_scene = nullptr;
@@ -369,7 +372,8 @@ size_t trace_wavefront_workspace_bytes(u32 n_pixels, u32 n, bool with_scene) {
     size_t cap = rec * n;
     return 256 + 2 * align_up(rec * sizeof(phx_float4), 256) + align_up(rec * sizeof(u32), 256) +
            align_up((size_t)n_pixels * sizeof(u32), 256) + align_up((size_t)n_pixels * sizeof(phx_float4), 256) +
-           align_up((size_t)n_pixels * sizeof(f32), 256) + 4 * align_up(cap * sizeof(phx_float4), 256) +
+           align_up((size_t)n_pixels * sizeof(f32), 256) + 5 * align_up(rec * sizeof(phx_float4), 256) +
+           2 * align_up(cap * sizeof(u32), 256) + align_up(cap * sizeof(phx_float4), 256) +
            (with_scene ? align_up(cap * sizeof(u32), 256) : 0);
 }
 

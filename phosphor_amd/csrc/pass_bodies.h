@@ -35,6 +35,8 @@ constexpr int EMIT_WF_BLOCK = 128;
 constexpr u32 EMIT_WF_BOUNCES = 5; /* bounce-synchronous launches before the finishing launch */
 constexpr u32 NEE_NONE = 0, NEE_DIFFUSE = 1, NEE_SPECULAR = 2;
 
+constexpr u32 NO_EVENT = 0xFFFFFFFFu;
+
 struct WfState {
     phx_float4 *rec_a, *rec_thr; /* [pixel · 5 + bounce] */
     u32 *rec_base;               /* first shadow slot of the NEE event of that bounce */
@@ -42,9 +44,12 @@ struct WfState {
     phx_float4 *env;             /* per pixel: throughput · envmap on a miss */
     f32 *occl;                   /* per pixel: occlusion at the diffuse hit */
     u32 *counter;                /* shadow slots handed out */
-    phx_float4 *sh_o;            /* origin.xyz, t_max in w */
-    phx_float4 *sh_d;            /* direction */
-    phx_float4 *sh_pre;          /* NEE weight; w = 0 valid, w = -1 no shadow ray (adds BLACK) */
+    /* NEE events [pixel · 5 + bounce]: hit position, shading normal, view
+       direction, base colour (diffuse) or F0 (specular), (roughness,
+       specular flag) */
+    phx_float4 *ev_pos, *ev_n, *ev_view, *ev_col, *ev_misc;
+    u32 *sh_rng;                 /* PCG state before the slot's light sample */
+    u32 *sh_ev;                  /* event of the slot; NO_EVENT: no sample is drawn (adds BLACK) */
     phx_float4 *sh_tr;           /* NEE weight × shadow transmittance (BLACK without a shadow ray) */
     u32 *sh_scene;               /* scene of each slot (multi-scene queues only, else null) */
     u32 cap;
@@ -150,42 +155,28 @@ PHX_DEV void camera_path(const SceneView &sc, const PhxCameraParams &cam, u32 w,
             wave_base = __shfl(wave_base, (int)first, 64);
         }
         if (nee) {
+            /* The light samples are drawn by the shadow kernel, one lane per
+               sample, from the event recorded here and the PCG state before
+               each sample; this lane only steps its stream past the draws
+               the samples take (skip_light_fragment), so the path goes on
+               with the state the reference has after direct lighting. */
             u32 base = wave_base + rank * n;
-            S.rec_base[rbase + depth] = base;
+            size_t e = rbase + depth;
+            S.rec_base[e] = base;
+            st(S.ev_pos + e, hit_pos);
+            st(S.ev_n + e, sn);
+            st(S.ev_view + e, view);
+            st(S.ev_col + e, spec ? f0 : bc);
+            st(S.ev_misc + e, mk(rough, spec ? 1.0f : 0.0f, 0.0f, 0.0f));
+            /* direct_lighting_specular returns BLACK before sampling when
+               n·v < EPS: no draws */
+            bool draws = !spec || !(dot(view, sn) < EPS);
             for (u32 d = 0; d < n; d++) {
-                LightFragment f;
-                bool valid = false;
-                F4 pre = BLACK;
-                if (!spec) {
-                    if (sample_light_fragment(rng, sc, hit_pos, sn, f)) {
-                        valid = true;
-                        pre = f.radiance * (bc / PI);
-                    }
-                } else {
-                    f32 ndv = dot(view, sn);
-                    /* direct_lighting_specular: each early return adds BLACK */
-                    if (!(ndv < EPS) && sample_light_fragment(rng, sc, hit_pos, sn, f) && !(f.n_dot_dir < EPS)) {
-                        f32 alpha = rough * rough;
-                        F4 hv = normalize(f.dir + view);
-                        f32 ndh = phx_fmaxf(dot(sn, hv), 0.0f);
-                        f32 ggx = ggx_ndf(ndh, alpha);
-                        F4 fr = fresnel4(f0, view, hv);
-                        f32 sm = smith_g1_ggx(f.n_dot_dir, alpha) * smith_g1_ggx(ndv, alpha);
-                        F4 spv = f.radiance * ggx * fr * sm / (4.0f * f.n_dot_dir * ndv);
-                        if (!(dot(spv, spv) < EPS * EPS)) {
-                            valid = true;
-                            pre = spv;
-                        }
-                    }
-                }
                 size_t slot = (size_t)base + d;
-                if (valid) {
-                    st(S.sh_o + slot, mk(hit_pos.x, hit_pos.y, hit_pos.z, f.dist - EPS));
-                    st(S.sh_d + slot, f.dir);
-                    st(S.sh_pre + slot, mk(pre.x, pre.y, pre.z, 0.0f));
-                } else {
-                    st(S.sh_pre + slot, mk(0.0f, 0.0f, 0.0f, -1.0f));
-                }
+                S.sh_rng[slot] = rng;
+                S.sh_ev[slot] = draws ? (u32)e : NO_EVENT;
+                if (draws)
+                    skip_light_fragment(rng, sc);
                 if (S.sh_scene)
                     S.sh_scene[slot] = tag;
             }
@@ -234,15 +225,49 @@ PHX_DEV void camera_path(const SceneView &sc, const PhxCameraParams &cam, u32 w,
  * ANY = any-hit traversal (exact when no material transmits and the stack
  * never overflows). */
 template <bool ANY> PHX_DEV void shadow_one(const SceneView &sc, const WfState &S, u32 slot, u32 *stk) {
-    F4 pre = ld(S.sh_pre + slot);
-    if (pre.w != 0.0f) {
+    u32 e = S.sh_ev[slot];
+    if (e == NO_EVENT) {
         st(S.sh_tr + slot, BLACK);
         return;
     }
-    F4 ot = ld(S.sh_o + slot);
-    F4 o = mk(ot.x, ot.y, ot.z, 0.0f);
-    F4 d = ld(S.sh_d + slot);
-    f32 t_min = EPS, t_max = ot.w;
+    /* the sample of trace_rays.cl's direct lighting (light_sampling.h:
+       249-262 diffuse, 336-363 specular) from the event and the PCG state
+       the camera kernel recorded */
+    u32 rng = S.sh_rng[slot];
+    F4 hit_pos = ld(S.ev_pos + e), sn = ld(S.ev_n + e), view = ld(S.ev_view + e), col = ld(S.ev_col + e);
+    F4 misc = ld(S.ev_misc + e);
+    LightFragment f;
+    bool valid = false;
+    F4 pre = BLACK;
+    if (misc.y == 0.0f) {
+        if (sample_light_fragment(rng, sc, hit_pos, sn, f)) {
+            valid = true;
+            pre = f.radiance * (col / PI);
+        }
+    } else {
+        f32 ndv = dot(view, sn); /* >= EPS: otherwise the slot has no event */
+        if (sample_light_fragment(rng, sc, hit_pos, sn, f) && !(f.n_dot_dir < EPS)) {
+            f32 alpha = misc.x * misc.x;
+            F4 hv = normalize(f.dir + view);
+            f32 ndh = phx_fmaxf(dot(sn, hv), 0.0f);
+            f32 ggx = ggx_ndf(ndh, alpha);
+            F4 fr = fresnel4(col, view, hv);
+            f32 sm = smith_g1_ggx(f.n_dot_dir, alpha) * smith_g1_ggx(ndv, alpha);
+            F4 spv = f.radiance * ggx * fr * sm / (4.0f * f.n_dot_dir * ndv);
+            if (!(dot(spv, spv) < EPS * EPS)) {
+                valid = true;
+                pre = spv;
+            }
+        }
+    }
+    if (!valid) {
+        st(S.sh_tr + slot, BLACK);
+        return;
+    }
+    pre = mk(pre.x, pre.y, pre.z, 0.0f);
+    F4 o = mk(hit_pos.x, hit_pos.y, hit_pos.z, 0.0f);
+    F4 d = f.dir;
+    f32 t_min = EPS, t_max = f.dist - EPS;
     F4 trans = WHITE;
     for (u32 depth = 0; depth < MAX_RAY_BOUNCES; depth++) {
         HitRecord rec;

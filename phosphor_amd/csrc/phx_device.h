@@ -1011,6 +1011,28 @@ PHX_DEV void sample_light(u32 &rng, const SceneView &sc, F4 &origin, F4 &dir, F4
     }
     power = power / scale;
 }
+/* The PCG draws sample_light_fragment takes, without its geometry: one to
+ * select the light (when there is light), then for an emissive-mesh light of
+ * area >= EPS one to choose the triangle (more than one triangle, area > EPS,
+ * choose_emissive_triangle) and two for the point (sample_emissive_point). */
+PHX_DEV void skip_light_fragment(u32 &rng, const SceneView &sc) {
+    if (sc.n_lights == 0 || sc.total_lum < EPS)
+        return;
+    f32 scale;
+    u32 li = select_light(rng, sc, scale);
+    const PhxLight &l = sc.lights[li];
+    u32 kind = l.kind;
+    if (kind == PHX_LIGHT_POINT || kind == PHX_LIGHT_SPOT || kind == PHX_LIGHT_DIRECTIONAL)
+        return;
+    f32 area = l.aux.w;
+    if (area < EPS)
+        return;
+    if (phx_as_uint(l.aux.z) > 1u && area > EPS)
+        (void)random_u32(rng);
+    (void)random_u32(rng);
+    (void)random_u32(rng);
+}
+
 /* light_sampling.h:231-300 */
 PHX_DEV bool sample_light_fragment(u32 &rng, const SceneView &sc, F4 pos, F4 normal, LightFragment &out) {
     if (sc.n_lights == 0 || sc.total_lum < EPS)
