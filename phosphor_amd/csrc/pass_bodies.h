@@ -577,28 +577,36 @@ PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32
  * a skipped cell never holds a photon the test would accept and the sums are
  * unchanged.  (The cells are 1.155 radii wide when their half-diagonal is the
  * radius, so about 16 of the 27 survive on average.) */
+/* Cell i of the 27-cell query: its bucket, or 0 when get_hash gives 0 or
+ * the cell lies beyond the radius (no photon in it can pass the distance
+ * test; see gather_cells_lane). */
+PHX_DEV u32 query_cell(const PhxPhotonHashData &d, F4 pos, f32 r_sq, u32 i) {
+    i32 cx = (i32)(i % 3) - 1, cy = (i32)((i / 3) % 3) - 1, cz = (i32)((i / 9) % 3) - 1;
+    f32 qx = pos.x + (f32)cx * d.cell_sizes.x;
+    f32 qy = pos.y + (f32)cy * d.cell_sizes.y;
+    f32 qz = pos.z + (f32)cz * d.cell_sizes.z;
+    u32 h = get_hash(qx, qy, qz, d);
+    if (h == 0)
+        return 0;
+    /* get_hash's cell indices, recovered from the same quotients */
+    u32 ix = (u32)((qx - d.origin.x) / d.cell_sizes.x);
+    u32 iy = (u32)((qy - d.origin.y) / d.cell_sizes.y);
+    u32 iz = (u32)((qz - d.origin.z) / d.cell_sizes.z);
+    f32 lx = d.origin.x + (f32)ix * d.cell_sizes.x, ly = d.origin.y + (f32)iy * d.cell_sizes.y,
+        lz = d.origin.z + (f32)iz * d.cell_sizes.z;
+    const f32 mx = 1e-4f * d.cell_sizes.x, my = 1e-4f * d.cell_sizes.y, mz = 1e-4f * d.cell_sizes.z;
+    f32 gx = phx_fmaxf(0.0f, phx_fmaxf(lx - pos.x, pos.x - (lx + d.cell_sizes.x)) - mx);
+    f32 gy = phx_fmaxf(0.0f, phx_fmaxf(ly - pos.y, pos.y - (ly + d.cell_sizes.y)) - my);
+    f32 gz = phx_fmaxf(0.0f, phx_fmaxf(lz - pos.z, pos.z - (lz + d.cell_sizes.z)) - mz);
+    return (gx * gx + gy * gy + gz * gz >= r_sq) ? 0u : h;
+}
+
 template <bool DECODED>
 PHX_DEV void gather_cells_lane(const GatherArgs &a, const u32 *cs, const u32 *ce, const PhxPhotonHashData &d, F4 pos,
                                F4 sn, f32 r_sq, f32 two_r_sq, f32 inv_two_r_sq, F4 &nf, f32 &found) {
-    const f32 mx = 1e-4f * d.cell_sizes.x, my = 1e-4f * d.cell_sizes.y, mz = 1e-4f * d.cell_sizes.z;
     for (u32 i = 0; i < 27; i++) {
-        i32 cx = (i32)(i % 3) - 1, cy = (i32)((i / 3) % 3) - 1, cz = (i32)((i / 9) % 3) - 1;
-        f32 qx = pos.x + (f32)cx * d.cell_sizes.x;
-        f32 qy = pos.y + (f32)cy * d.cell_sizes.y;
-        f32 qz = pos.z + (f32)cz * d.cell_sizes.z;
-        u32 h = get_hash(qx, qy, qz, d);
+        u32 h = query_cell(d, pos, r_sq, i);
         if (h == 0)
-            continue;
-        /* get_hash's cell indices, recovered from the same quotients */
-        u32 ix = (u32)((qx - d.origin.x) / d.cell_sizes.x);
-        u32 iy = (u32)((qy - d.origin.y) / d.cell_sizes.y);
-        u32 iz = (u32)((qz - d.origin.z) / d.cell_sizes.z);
-        f32 lx = d.origin.x + (f32)ix * d.cell_sizes.x, ly = d.origin.y + (f32)iy * d.cell_sizes.y,
-            lz = d.origin.z + (f32)iz * d.cell_sizes.z;
-        f32 gx = phx_fmaxf(0.0f, phx_fmaxf(lx - pos.x, pos.x - (lx + d.cell_sizes.x)) - mx);
-        f32 gy = phx_fmaxf(0.0f, phx_fmaxf(ly - pos.y, pos.y - (ly + d.cell_sizes.y)) - my);
-        f32 gz = phx_fmaxf(0.0f, phx_fmaxf(lz - pos.z, pos.z - (lz + d.cell_sizes.z)) - mz);
-        if (gx * gx + gy * gy + gz * gz >= r_sq)
             continue;
         gather_cell<DECODED>(a, cs[h], ce[h], pos, sn, r_sq, two_r_sq, inv_two_r_sq, nf, found);
     }
