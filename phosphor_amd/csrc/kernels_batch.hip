@@ -279,9 +279,8 @@ __global__ __launch_bounds__(256) void k_bgather(BGatherArgs a) {
     const BatchScene &B = a.scenes[s];
     u32 lt = t - B.tile_base;
     u32 x = (lt % B.tiles_x) * 8 + (lane & 7u), y = (lt / B.tiles_x) * 8 + (lane >> 3);
-    if (x >= B.width || y >= B.height)
-        return;
-    gather_pixel<true>(a.ga, B.px_base + y * B.width + x, B.hash, B.alpha, B.bucket_base);
+    gather_pixel_flat(a.ga, x < B.width && y < B.height, B.px_base + y * B.width + x, B.hash, B.alpha,
+                      B.bucket_base);
 }
 
 /* ---------------- pass ---------------- */

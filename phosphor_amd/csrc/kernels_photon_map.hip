@@ -612,75 +612,12 @@ hipError_t launch_decode_photons(const u32 *ppow, const u32 *pdir, const u32 *d_
 }
 
 /* The gather of the image-shaped launch, one wave per 8×8 tile as k_gather,
- * with the cell loop and the record loop of a pixel flattened into one: each
- * iteration first moves every lane whose cell is used up to its next cell
- * (query_cell, skipping empty and distant ones), then processes one batch of
- * eight records of every lane's current cell.  A lane that leaves a short
- * cell goes on at once instead of idling until the wave's longest cell is
- * done (3.45 -> 3.2 ms on the caustic workload).  Every pixel's cells and
- * records are summed in gather_pixel's order: same bits.  Handing pixels out
- * dynamically (a lane that finishes takes another pixel of its block) was
- * measured slower: the lanes of a wave then query unrelated cells and stop
- * sharing record loads. */
+ * with gather_pixel_flat's flattened loops. */
 __global__ __launch_bounds__(256) void k_gather_flat(GatherArgs a, u32 width, u32 height, u32 tiles_x) {
     u32 wave = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     u32 lane = threadIdx.x & 63u;
     u32 x = (wave % tiles_x) * 8u + (lane & 7u), y = (wave / tiles_x) * 8u + (lane >> 3);
-    const PhxPhotonHashData &d = a.d;
-    const u32 *cs = a.cell_start, *ce = a.cell_end;
-    u32 tid = y * width + x;
-    F4 pos = ZERO, sn = ZERO, fsf = ZERO, nf = BLACK;
-    f32 wsf = 0.0f, r_sq = 0.0f, found = 0.0f;
-    /* ci: next cell of the 27 (27: none left); [k, e): records of the current cell */
-    u32 ci = 27, k = 0, e = 0;
-    if (x < width && y < height && gather_begin(a, tid, d, pos, sn, fsf, wsf, r_sq))
-        ci = 0;
-    const bool px = ci == 0;
-    const f32 two_r_sq = 2.0f * r_sq, inv_two_r_sq = px ? 1.0f / two_r_sq : 0.0f, wz = 0.0f * sn.w;
-    for (;;) {
-        while (k >= e && ci < 27) {
-            u32 h = query_cell(d, pos, r_sq, ci);
-            ci++;
-            if (h != 0) {
-                k = cs[h];
-                e = ce[h];
-            }
-        }
-        if (!__any(k < e))
-            break;
-        /* one batch of the current cell (gather_cell<true>'s loop body) */
-        F4 pp[GATHER_BATCH_DECODED], pd[GATHER_BATCH_DECODED];
-        bool work = k < e;
-        u32 kl = work ? e - 1 : 0;
-#pragma unroll
-        for (u32 q = 0; q < GATHER_BATCH_DECODED; q++) {
-            u32 kk = k + q < e ? k + q : kl;
-            pp[q] = work ? ld(a.ppos + kk) : ZERO;
-            pd[q] = work ? ld(a.pdirpow + kk) : ZERO;
-        }
-#pragma unroll
-        for (u32 q = 0; q < GATHER_BATCH_DECODED; q++)
-            keep_loaded(pd[q]);
-        if (work) {
-#pragma unroll
-            for (u32 q = 0; q < GATHER_BATCH_DECODED; q++) {
-                if (k + q >= e)
-                    break;
-                F4 dd = pos - pp[q];
-                f32 dsq = dot(dd, dd);
-                if (dsq >= r_sq)
-                    continue;
-                if (((pd[q].x * sn.x + pd[q].y * sn.y) + pd[q].z * sn.z) + wz > 0.0f)
-                    continue;
-                f32 wgt = gauss_weight(dsq, two_r_sq, inv_two_r_sq);
-                nf = nf + wgt * decode_rgbe(phx_as_uint(pd[q].w));
-                found += wgt;
-            }
-            k = k + GATHER_BATCH_DECODED < e ? k + GATHER_BATCH_DECODED : e;
-        }
-    }
-    if (px)
-        gather_end(a, tid, a.alpha, fsf, wsf, r_sq, nf, found);
+    gather_pixel_flat(a, x < width && y < height, y * width + x, a.d, a.alpha, 0u);
 }
 
 hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const phx_float4 *hp_throughput,
