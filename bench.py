@@ -401,9 +401,9 @@ def main() -> int:
                    "parallelism": "single-gpu"},
         "stage_ms": {"trace": float(mean_stage[0]), "emit": float(mean_stage[1]),
                      "hash_scan_sort": float(mean_stage[2]), "gather": float(mean_stage[3])},
-        "roofline": {"kernel": "k_gather", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+        "roofline": {"kernel": "k_gather_flat", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": read_traffic("k_gather", args.workload)},
+                     "traffic": read_traffic("k_gather_flat", args.workload)},
         "rays": rays,
         "quality": quality or None,
         "cpu_baseline": cpu,
