@@ -301,8 +301,7 @@ struct WfShadowArgs {
     u32 any_hit;
 };
 
-/* six waves per SIMD (80 VGPRs, 32 bytes of spills per lane); five without
-   spills measured the same */
+/* six waves per SIMD (76 VGPRs, no spills) */
 __global__ __launch_bounds__(WF_SHADOW_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_trace_shadow(WfShadowArgs a) {
     __shared__ u32 s_stack[BVH_STACK_SIZE * WF_SHADOW_BLOCK];
     u32 *stk = s_stack + threadIdx.x;

@@ -220,25 +220,19 @@ PHX_DEV void camera_path(const SceneView &sc, const PhxCameraParams &cam, u32 w,
     S.meta[wf] = nrec | (kinds << 3) | (env_flag << 15);
 }
 
-/* shadow_transmittance (light_sampling.h:302-333) of one queued shadow ray,
- * stored multiplied by the sample's NEE weight (the term resolve_pixel adds);
- * ANY = any-hit traversal (exact when no material transmits and the stack
- * never overflows). */
-template <bool ANY> PHX_DEV void shadow_one(const SceneView &sc, const WfState &S, u32 slot, u32 *stk) {
-    u32 e = S.sh_ev[slot];
-    if (e == NO_EVENT) {
-        st(S.sh_tr + slot, BLACK);
-        return;
-    }
-    /* the sample of trace_rays.cl's direct lighting (light_sampling.h:
-       249-262 diffuse, 336-363 specular) from the event and the PCG state
-       the camera kernel recorded */
+/* The light sample of a shadow slot: draws it from the PCG state the camera
+ * kernel recorded and weights it by the BSDF of the event (trace_rays.cl's
+ * direct lighting, light_sampling.h:249-262 diffuse, 336-363 specular).
+ * False when the sample contributes nothing (no light fragment, or a
+ * specular weight below the reference's cut-off). */
+PHX_DEV bool shadow_sample(const SceneView &sc, const WfState &S, u32 slot, u32 e, F4 &pre, F4 &o, F4 &d,
+                           f32 &t_max) {
     u32 rng = S.sh_rng[slot];
     F4 hit_pos = ld(S.ev_pos + e), sn = ld(S.ev_n + e), view = ld(S.ev_view + e), col = ld(S.ev_col + e);
     F4 misc = ld(S.ev_misc + e);
     LightFragment f;
     bool valid = false;
-    F4 pre = BLACK;
+    pre = BLACK;
     if (misc.y == 0.0f) {
         if (sample_light_fragment(rng, sc, hit_pos, sn, f)) {
             valid = true;
@@ -260,38 +254,63 @@ template <bool ANY> PHX_DEV void shadow_one(const SceneView &sc, const WfState &
             }
         }
     }
-    if (!valid) {
+    if (!valid)
+        return false;
+    pre = mk(pre.x, pre.y, pre.z, 0.0f);
+    o = mk(hit_pos.x, hit_pos.y, hit_pos.z, 0.0f);
+    d = f.dir;
+    t_max = f.dist - EPS;
+    return true;
+}
+
+/* One step of shadow_transmittance's walk through transmissive surfaces
+ * (light_sampling.h:302-333): folds the hit into the transmittance.  False
+ * when the walk ends here (opaque surface, or transmittance below EPS: then
+ * trans is BLACK). */
+template <bool ANY>
+PHX_DEV bool shadow_fold(const SceneView &sc, const HitRecord &rec, F4 o, F4 d, f32 &t_min, F4 &trans) {
+    const PhxMaterial &mat = sc.materials[rec.mat_index];
+    if (ANY || mat.transmission < EPS) {
+        trans = BLACK;
+        return false;
+    }
+    F4 bcol = from(mat.base_color);
+    if (mat.diff_index != NO_TEXTURE) {
+        SurfaceHit s2 = process_hit(sc, rec, o, d);
+        bcol = bcol * srgb_to_linear(sample_texture_uv(sc, rec.mat_index, mat.diff_index, s2.uvx, s2.uvy, 0));
+    }
+    if (mat.att_dist > EPS)
+        trans = trans * beer_lambert(from(mat.att_color), mat.att_dist, rec.t - t_min);
+    trans = trans * (bcol * mat.transmission);
+    if (dot(trans, trans) < EPS) {
+        trans = BLACK;
+        return false;
+    }
+    t_min = rec.t + EPS;
+    return true;
+}
+
+/* shadow_transmittance (light_sampling.h:302-333) of one queued shadow ray,
+ * stored multiplied by the sample's NEE weight (the term resolve_pixel adds);
+ * ANY = any-hit traversal (exact when no material transmits and the stack
+ * never overflows). */
+template <bool ANY> PHX_DEV void shadow_one(const SceneView &sc, const WfState &S, u32 slot, u32 *stk) {
+    u32 e = S.sh_ev[slot];
+    F4 pre, o, d;
+    f32 t_max;
+    if (e == NO_EVENT || !shadow_sample(sc, S, slot, e, pre, o, d, t_max)) {
         st(S.sh_tr + slot, BLACK);
         return;
     }
-    pre = mk(pre.x, pre.y, pre.z, 0.0f);
-    F4 o = mk(hit_pos.x, hit_pos.y, hit_pos.z, 0.0f);
-    F4 d = f.dir;
-    f32 t_min = EPS, t_max = f.dist - EPS;
+    f32 t_min = EPS;
     F4 trans = WHITE;
     for (u32 depth = 0; depth < MAX_RAY_BOUNCES; depth++) {
         HitRecord rec;
         if (!(ANY ? scene_any_hit(sc, o, d, t_min, t_max, rec, stk, WF_SHADOW_BLOCK)
                   : scene_intersect(sc, o, d, t_min, t_max, rec, stk, WF_SHADOW_BLOCK)))
             break;
-        const PhxMaterial &mat = sc.materials[rec.mat_index];
-        if (ANY || mat.transmission < EPS) {
-            trans = BLACK;
+        if (!shadow_fold<ANY>(sc, rec, o, d, t_min, trans))
             break;
-        }
-        F4 bcol = from(mat.base_color);
-        if (mat.diff_index != NO_TEXTURE) {
-            SurfaceHit s2 = process_hit(sc, rec, o, d);
-            bcol = bcol * srgb_to_linear(sample_texture_uv(sc, rec.mat_index, mat.diff_index, s2.uvx, s2.uvy, 0));
-        }
-        if (mat.att_dist > EPS)
-            trans = trans * beer_lambert(from(mat.att_color), mat.att_dist, rec.t - t_min);
-        trans = trans * (bcol * mat.transmission);
-        if (dot(trans, trans) < EPS) {
-            trans = BLACK;
-            break;
-        }
-        t_min = rec.t + EPS;
     }
     st(S.sh_tr + slot, pre * trans);
 }
@@ -361,13 +380,11 @@ PHX_DEV void emit_start_one(const SceneView &sc, u32 seed, u32 index, u32 li, co
     (void)stride;
 }
 
-/* One bounce of emit_photons.cl:49-113 on a photon's state; returns false
- * when the path ends (escape, roulette or the bounce limit). */
-PHX_DEV bool emit_step(const SceneView &sc, const EmitStaging &g, u32 li, u32 depth, u32 max_bounces, F4 &origin,
-                       F4 &dir, F4 &power, F4 &throughput, u32 &rng, IorStack &ior, u32 &n, u32 *stk) {
-    HitRecord rec;
-    if (!scene_intersect<true>(sc, origin, dir, EPS, INF, rec, stk, EMIT_WF_BLOCK))
-        return false;
+/* The part of one bounce of emit_photons.cl:49-113 after the ray has hit
+ * (rec); returns false when the path ends (roulette or the bounce limit). */
+PHX_DEV bool emit_shade(const SceneView &sc, const EmitStaging &g, u32 li, u32 depth, u32 max_bounces,
+                        const HitRecord &rec, F4 &origin, F4 &dir, F4 &power, F4 &throughput, u32 &rng, IorStack &ior,
+                        u32 &n) {
     SurfaceHit sh = process_hit(sc, rec, origin, dir);
     const PhxMaterial &mat = sc.materials[sh.mat_index];
     f32 curr_ior = current_ior(ior);
@@ -403,19 +420,33 @@ PHX_DEV bool emit_step(const SceneView &sc, const EmitStaging &g, u32 li, u32 de
     return depth + 1 < max_bounces;
 }
 
+/* One bounce of emit_photons.cl:49-113 on a photon's state; returns false
+ * when the path ends (escape, roulette or the bounce limit). */
+PHX_DEV bool emit_step(const SceneView &sc, const EmitStaging &g, u32 li, u32 depth, u32 max_bounces, F4 &origin,
+                       F4 &dir, F4 &power, F4 &throughput, u32 &rng, IorStack &ior, u32 &n, u32 *stk) {
+    HitRecord rec;
+    if (!scene_intersect<true>(sc, origin, dir, EPS, INF, rec, stk, EMIT_WF_BLOCK))
+        return false;
+    return emit_shade(sc, g, li, depth, max_bounces, rec, origin, dir, power, throughput, rng, ior, n);
+}
+
 /* Loads photon li's state, runs bounces depth .. (depth + 1 if one_bounce,
  * else to the end of its path), stores what changed; returns whether the
  * photon is still live after a single bounce. */
 PHX_DEV bool emit_advance(const SceneView &sc, const EmitWfState &S, const EmitStaging &g, u32 li, u32 depth,
                           u32 max_bounces, bool one_bounce, u32 *stk) {
     F4 origin = ld(S.origin + li), dir = ld(S.dir + li);
-    F4 power = ld(S.power + li), throughput = ld(S.thr + li);
-    u32 rng = S.rng[li];
-    IorStack ior{S.ior + li, g.stride, S.ior_cur[li]};
-    u32 n = S.count[li], n0 = n;
-    bool keep;
     if (one_bounce) {
-        keep = emit_step(sc, g, li, depth, max_bounces, origin, dir, power, throughput, rng, ior, n, stk);
+        /* the rest of the state is loaded after the traversal, which then
+           runs with fewer live registers */
+        HitRecord rec;
+        if (!scene_intersect<true>(sc, origin, dir, EPS, INF, rec, stk, EMIT_WF_BLOCK))
+            return false;
+        F4 power = ld(S.power + li), throughput = ld(S.thr + li);
+        u32 rng = S.rng[li];
+        IorStack ior{S.ior + li, g.stride, S.ior_cur[li]};
+        u32 n = S.count[li], n0 = n;
+        bool keep = emit_shade(sc, g, li, depth, max_bounces, rec, origin, dir, power, throughput, rng, ior, n);
         if (keep) {
             st(S.origin + li, origin);
             st(S.dir + li, dir);
@@ -424,15 +455,20 @@ PHX_DEV bool emit_advance(const SceneView &sc, const EmitWfState &S, const EmitS
             S.rng[li] = rng;
             S.ior_cur[li] = ior.cur;
         }
-    } else {
-        for (u32 d = depth; d < max_bounces; d++)
-            if (!emit_step(sc, g, li, d, max_bounces, origin, dir, power, throughput, rng, ior, n, stk))
-                break;
-        keep = false;
+        if (n != n0)
+            S.count[li] = n;
+        return keep;
     }
+    F4 power = ld(S.power + li), throughput = ld(S.thr + li);
+    u32 rng = S.rng[li];
+    IorStack ior{S.ior + li, g.stride, S.ior_cur[li]};
+    u32 n = S.count[li], n0 = n;
+    for (u32 d = depth; d < max_bounces; d++)
+        if (!emit_step(sc, g, li, d, max_bounces, origin, dir, power, throughput, rng, ior, n, stk))
+            break;
     if (n != n0)
         S.count[li] = n;
-    return keep;
+    return false;
 }
 
 /* appends li to a queue when keep, one atomic per wave; every lane of the
