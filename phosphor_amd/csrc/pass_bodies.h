@@ -25,6 +25,7 @@
 #ifndef PHX_PASS_BODIES_H
 #define PHX_PASS_BODIES_H
 
+#include "phx_cull.h"
 #include "phx_device.h"
 
 namespace phx {
@@ -605,18 +606,13 @@ PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32
     }
 }
 
-/* The 27-cell query of one lane (photon_hash.h:65-100).  A cell whose box
- * lies farther than the radius from the hitpoint is skipped: no photon stored
- * in it can pass the distance test.  The box is that of the cell get_hash
- * returned; its gap to the hitpoint is shrunk by 1e-4 cell sizes per axis, far
- * more than the rounding of a photon's cell assignment or of the distance, so
- * a skipped cell never holds a photon the test would accept and the sums are
- * unchanged.  (The cells are 1.155 radii wide when their half-diagonal is the
- * radius, so about 16 of the 27 survive on average.) */
-/* Cell i of the 27-cell query: its bucket, or 0 when get_hash gives 0 or
- * the cell lies beyond the radius (no photon in it can pass the distance
- * test; see gather_cells_lane). */
-PHX_DEV u32 query_cell(const PhxPhotonHashData &d, F4 pos, f32 r_sq, u32 i) {
+/* Cell i of the 27-cell query (photon_hash.h:65-100): its bucket, or 0 when
+ * get_hash gives 0 or the cell lies provably beyond the radius, so that no
+ * photon stored in it can pass the distance test and the sums are unchanged
+ * (phx_cull.h: the gap to the cell's box less the margins m).  The cells are
+ * 1.155 radii wide when their half-diagonal is the radius, so about 16 of the
+ * 27 survive on average.  l receives the cell's low corner. */
+PHX_DEV u32 query_cell(const PhxPhotonHashData &d, F4 pos, f32 r_sq, u32 i, F4 m, F4 &l) {
     i32 cx = (i32)(i % 3) - 1, cy = (i32)((i / 3) % 3) - 1, cz = (i32)((i / 9) % 3) - 1;
     f32 qx = pos.x + (f32)cx * d.cell_sizes.x;
     f32 qy = pos.y + (f32)cy * d.cell_sizes.y;
@@ -628,20 +624,28 @@ PHX_DEV u32 query_cell(const PhxPhotonHashData &d, F4 pos, f32 r_sq, u32 i) {
     u32 ix = (u32)((qx - d.origin.x) / d.cell_sizes.x);
     u32 iy = (u32)((qy - d.origin.y) / d.cell_sizes.y);
     u32 iz = (u32)((qz - d.origin.z) / d.cell_sizes.z);
-    f32 lx = d.origin.x + (f32)ix * d.cell_sizes.x, ly = d.origin.y + (f32)iy * d.cell_sizes.y,
-        lz = d.origin.z + (f32)iz * d.cell_sizes.z;
-    const f32 mx = 1e-4f * d.cell_sizes.x, my = 1e-4f * d.cell_sizes.y, mz = 1e-4f * d.cell_sizes.z;
-    f32 gx = phx_fmaxf(0.0f, phx_fmaxf(lx - pos.x, pos.x - (lx + d.cell_sizes.x)) - mx);
-    f32 gy = phx_fmaxf(0.0f, phx_fmaxf(ly - pos.y, pos.y - (ly + d.cell_sizes.y)) - my);
-    f32 gz = phx_fmaxf(0.0f, phx_fmaxf(lz - pos.z, pos.z - (lz + d.cell_sizes.z)) - mz);
-    return (gx * gx + gy * gy + gz * gz >= r_sq) ? 0u : h;
+    l = mk(d.origin.x + (f32)ix * d.cell_sizes.x, d.origin.y + (f32)iy * d.cell_sizes.y,
+           d.origin.z + (f32)iz * d.cell_sizes.z, 0.0f);
+    f32 gx = phx_cull_gap(pos.x, l.x, l.x + d.cell_sizes.x, m.x);
+    f32 gy = phx_cull_gap(pos.y, l.y, l.y + d.cell_sizes.y, m.y);
+    f32 gz = phx_cull_gap(pos.z, l.z, l.z + d.cell_sizes.z, m.z);
+    return phx_cull_far(gx, gy, gz, r_sq) ? 0u : h;
+}
+
+/* per-axis culling margins for a hitpoint (phx_cull_margin) */
+PHX_DEV F4 cull_margins(const PhxPhotonHashData &d, F4 pos) {
+    return mk(phx_cull_margin(d.origin.x, pos.x, d.cell_sizes.x, d.grid_res),
+              phx_cull_margin(d.origin.y, pos.y, d.cell_sizes.y, d.grid_res),
+              phx_cull_margin(d.origin.z, pos.z, d.cell_sizes.z, d.grid_res), 0.0f);
 }
 
 template <bool DECODED>
 PHX_DEV void gather_cells_lane(const GatherArgs &a, const u32 *cs, const u32 *ce, const PhxPhotonHashData &d, F4 pos,
                                F4 sn, f32 r_sq, f32 two_r_sq, f32 inv_two_r_sq, F4 &nf, f32 &found) {
+    const F4 m = cull_margins(d, pos);
     for (u32 i = 0; i < 27; i++) {
-        u32 h = query_cell(d, pos, r_sq, i);
+        F4 l;
+        u32 h = query_cell(d, pos, r_sq, i, m, l);
         if (h == 0)
             continue;
         gather_cell<DECODED>(a, cs[h], ce[h], pos, sn, r_sq, two_r_sq, inv_two_r_sq, nf, found);
@@ -735,9 +739,11 @@ PHX_DEV void gather_pixel_flat(const GatherArgs &a, bool valid, u32 tid, const P
         ci = 0;
     const bool px = ci == 0;
     const f32 two_r_sq = 2.0f * r_sq, inv_two_r_sq = px ? 1.0f / two_r_sq : 0.0f, wz = 0.0f * sn.w;
+    const F4 m = cull_margins(d, pos);
     for (;;) {
         while (k >= e && ci < 27) {
-            u32 h = query_cell(d, pos, r_sq, ci);
+            F4 l;
+            u32 h = query_cell(d, pos, r_sq, ci, m, l);
             ci++;
             if (h != 0) {
                 k = cs[h];
