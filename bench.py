@@ -295,6 +295,8 @@ def main() -> int:
                     help="report the relative RMSE of an estimate after this many passes against an independent "
                          "render (0 = skip)")
     ap.add_argument("--noise-ref-passes", type=int, default=1024, help="passes of that independent render")
+    ap.add_argument("--bvh-leaf-size", type=int, default=None,
+                    help="triangles per leaf of the SAH tree (1-4; same results, default: the workload's)")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
@@ -312,6 +314,8 @@ def main() -> int:
     import phosphor_amd as P
 
     cfg, scene, _rounds = P.config_for(args.workload)
+    if args.bvh_leaf_size is not None:
+        cfg.bvh_leaf_size = args.bvh_leaf_size
     n_px = cfg.width * cfg.height
     print(f"[bench] workload={args.workload} tris={scene.n_triangles} {cfg.width}x{cfg.height} "
           f"photons/pass={cfg.photons_per_round} grid={cfg.grid_res}", file=sys.stderr, flush=True)
@@ -398,7 +402,7 @@ def main() -> int:
                    "height": cfg.height, "photons_per_pass": cfg.photons_per_round,
                    "max_bounces": cfg.max_photon_bounces, "grid_res": cfg.grid_res,
                    "initial_radius": cfg.initial_radius, "direct_samples": cfg.direct_samples,
-                   "parallelism": "single-gpu"},
+                   "bvh_leaf_size": cfg.bvh_leaf_size or 3, "parallelism": "single-gpu"},
         "stage_ms": {"trace": float(mean_stage[0]), "emit": float(mean_stage[1]),
                      "hash_scan_sort": float(mean_stage[2]), "gather": float(mean_stage[3])},
         "roofline": {"kernel": "k_gather_flat", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,

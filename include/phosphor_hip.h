@@ -64,10 +64,13 @@ typedef struct PHX_ALIGN16 PhxBvhNode { phx_float4 min, max; } PhxBvhNode;
 /* No counterpart in the reference: one interior node of a binary tree with
  * both children's boxes, 64 bytes (one cache line), so a search step tests
  * both children with one load.  lmin.w / rmin.w = as_float(child reference):
- * PHX_PAIR_LEAF | triangle index for a leaf child, else the child's pair
- * index; lmax.w = rmax.w = 0. */
+ * for a leaf child PHX_PAIR_LEAF | first << 2 | (count − 1), the leaf's
+ * triangles being positions first .. first + count − 1 of the tree's
+ * triangle order; else the child's pair index.  lmax.w = rmax.w = 0. */
 typedef struct PHX_ALIGN16 PhxBvhPair { phx_float4 lmin, lmax, rmin, rmax; } PhxBvhPair;
 #define PHX_PAIR_LEAF 0x80000000u
+#define PHX_PAIR_MAX_LEAF 4u            /* triangles per leaf */
+#define PHX_PAIR_MAX_TRIS 0x1FFFFFF0u   /* triangles a tree of pairs can order */
 
 /* include/common/material.h:7-27 — 32 / 192 bytes */
 typedef struct PHX_ALIGN16 PhxUvTransform {
@@ -284,20 +287,23 @@ int phx_device_synchronize(void);
 uint32_t phx_build_bvh_median(const PhxTriangleVertices *tri_verts, uint32_t n_tris, PhxBvhNode *out_nodes);
 
 /* No counterpart in the reference: a binned surface-area-heuristic tree over
- * the same expanded triangle boxes, one triangle per leaf, no interior node
- * deeper than `max_depth`.  Leaf max.w = as_float(leaf_key[triangle]) (the
- * tie-break key of the near-first closest-hit search: the triangle's leaf
- * index in the reference tree); interior min.w = as_float(0xFFFFFFF8 | hint),
- * hint = split axis | 4 when the left child lies above the right one.
- * `out_nodes` must hold 2*n_tris-1 nodes.  Returns the node count, 0 on error
- * or when no tree within max_depth exists (n_tris > 2^(max_depth+1)). */
-uint32_t phx_build_bvh_sah(const PhxTriangleVertices *tri_verts, uint32_t n_tris, const uint32_t *leaf_key,
-                           uint32_t max_depth, PhxBvhNode *out_nodes);
+ * the same expanded triangle boxes, at most `max_leaf` (1 .. PHX_PAIR_MAX_LEAF)
+ * triangles per leaf, no interior node deeper than `max_depth`, in the node
+ * format above (preorder, left child at index + 1).  out_order receives the
+ * tree's triangle order (n_tris entries, a permutation of the triangles);
+ * leaf min.w = as_float(PHX_PAIR_LEAF | first << 2 | (count − 1)) names the
+ * leaf's positions in that order, leaf max.w = 0; interior min.w =
+ * as_float(0xFFFFFFF8 | hint), hint = split axis | 4 when the left child lies
+ * above the right one.  `out_nodes` must hold 2*n_tris-1 nodes.  Returns the
+ * node count, 0 on error (n_tris beyond PHX_PAIR_MAX_TRIS, max_leaf out of
+ * range) or when no tree within max_depth exists. */
+uint32_t phx_build_bvh_sah(const PhxTriangleVertices *tri_verts, uint32_t n_tris, uint32_t max_depth,
+                           uint32_t max_leaf, PhxBvhNode *out_nodes, uint32_t *out_order);
 
 /* photon_hash.h:41-50 build_photon_hash_info(bbox, grid_res), written to *out. */
-/* No counterpart in the reference: the interior nodes of a one-triangle-leaf
- * tree in the node format above (n_nodes = 2·n_tris − 1, preorder, left child
- * at index + 1, interior min.w >= 0xFFFFFFF8) as pairs, in the same preorder;
+/* No counterpart in the reference: the interior nodes of a tree built by
+ * phx_build_bvh_sah (preorder, left child at index + 1, interior min.w >=
+ * 0xFFFFFFF8, leaf min.w a leaf reference) as pairs, in the same preorder;
  * out_pairs holds n_nodes / 2 entries (none for a single leaf).  *root_ref is
  * the reference of node 0.  Returns the number of pairs, or -1 on a malformed
  * tree. */
@@ -356,6 +362,8 @@ typedef struct PhxRenderArgs {
     uint32_t flags;                 /* PHX_PASS_* */
     uint32_t emit_wave_bounces;     /* bounce-synchronous emission launches before the finishing one
                                        (0 = default 5); same results for every value */
+    uint32_t bvh_leaf_size;         /* triangles per leaf of the SAH tree rays are traced on, 1 ..
+                                       PHX_PAIR_MAX_LEAF (0 = default 3); same results for every value */
 } PhxRenderArgs;
 
 /* Capture the pass into a hipGraph on the first run_pass and replay it. */

@@ -62,12 +62,13 @@ class RenderArgs(C.Structure):
         ("max_photons_per_batch", C.c_uint32), ("max_photon_bounces", C.c_uint32), ("grid_res", C.c_uint32),
         ("direct_samples", C.c_uint32), ("seed", C.c_uint32), ("sppm_alpha", C.c_float),
         ("initial_radius", C.c_float), ("trace_mode", C.c_uint32), ("flags", C.c_uint32),
-        ("emit_wave_bounces", C.c_uint32),
+        ("emit_wave_bounces", C.c_uint32), ("bvh_leaf_size", C.c_uint32),
     ]
 
 
 PHX_PASS_GRAPH = 1
 PHX_REFERENCE_BVH = 2
+PHX_PAIR_MAX_LEAF = 4
 
 
 class BatchOptions(C.Structure):
@@ -133,7 +134,7 @@ SIGNATURES = {
     "phx_device_copy": (INT, VP, VP, C.c_size_t, VP),
     "phx_device_synchronize": (INT,),
     "phx_build_bvh_median": (U32, VP, U32, VP),
-    "phx_build_bvh_sah": (U32, VP, U32, VP, U32, VP),
+    "phx_build_bvh_sah": (U32, VP, U32, U32, U32, VP, VP),
     "phx_build_bvh_pairs": (INT, VP, U32, VP, VP),
     "phx_build_photon_hash_info": (INT, VP, U32, VP),
     "phx_renderer_create": (INT, C.POINTER(RenderArgs), C.POINTER(SceneDesc), C.POINTER(VP)),

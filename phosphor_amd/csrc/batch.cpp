@@ -104,7 +104,8 @@ int PhxBatch::build(const PhxSceneDesc *descs) {
     for (uint32_t i = 0; i < S; i++) {
         const PhxRenderArgs &a = args[i];
         scenes.emplace_back(new SceneBuffers());
-        int rc = upload_scene(descs[i], s, ((opt.flags | a.flags) & PHX_REFERENCE_BVH) != 0, *scenes[i]);
+        int rc = upload_scene(descs[i], s, ((opt.flags | a.flags) & PHX_REFERENCE_BVH) != 0, a.bvh_leaf_size,
+                              *scenes[i]);
         if (rc != PHX_OK)
             return rc;
         BatchScene &B = table[i];

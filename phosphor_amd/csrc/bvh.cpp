@@ -125,15 +125,17 @@ struct Builder {
 };
 
 /* Binned surface-area-heuristic build over the same expanded triangle boxes.
- * Leaves hold one triangle, as in the median tree, so a closest-hit search
- * over this tree meets the same candidate triangles; only the boxes (and so
- * the number of nodes visited) change.  Every split keeps the tree within
- * `max_depth` interior levels: a candidate split is taken only if both sides
- * can still be built balanced within the limit, otherwise the range is split
- * at its object median, which always can. */
+ * A range of at most `max_leaf` triangles becomes a leaf whose box is the
+ * union of its triangles' boxes, so a closest-hit search over this tree meets
+ * every candidate triangle the median tree's search meets; only the boxes
+ * (and so the number of nodes visited) change.  Every split keeps the tree
+ * within `max_depth` interior levels: a candidate split is taken only if both
+ * sides can still be built balanced within the limit, otherwise the range is
+ * split at its object median, which always can. */
 struct SahNode {
     Box box;
-    int tri;        /* -1 for interior */
+    int first;      /* -1 for interior; a leaf's first position in `order` */
+    unsigned count; /* a leaf's triangle count */
     unsigned right; /* right child id (interior) */
     unsigned hint;  /* split axis | 4 when the left child lies above the right */
 };
@@ -168,18 +170,27 @@ Box empty_box() {
     return b;
 }
 
+unsigned ceil_div(unsigned a, unsigned b) { return (a + b - 1) / b; }
+
 struct SahBuilder {
     static constexpr int BINS = 32;
     const std::vector<Box> &boxes;
     const std::vector<float> *cen; /* [3] centroid per triangle */
     std::vector<unsigned> &order;
     unsigned max_depth;
+    unsigned max_leaf;
+
+    /* interior levels a range of c triangles needs below its own node */
+    unsigned levels(unsigned c) const { return ceil_log2(ceil_div(c, max_leaf)); }
 
     std::vector<SahNode> build(unsigned start, unsigned end, unsigned depth) const {
         std::vector<SahNode> out;
         unsigned count = end - start;
-        if (count == 1) {
-            out.push_back(SahNode{boxes[order[start]], (int)order[start], 0u, 0u});
+        if (count <= max_leaf) {
+            Box lb = empty_box();
+            for (unsigned i = start; i < end; i++)
+                grow(lb, boxes[order[i]]);
+            out.push_back(SahNode{lb, (int)start, count, 0u, 0u});
             return out;
         }
         Box bb = empty_box(), cb = empty_box();
@@ -228,7 +239,7 @@ struct SahBuilder {
                     grow(acc, bin_box[k]);
                 n_acc += bin_n[k];
                 unsigned nl = left_n[k - 1], nr = n_acc;
-                if (nl == 0 || nr == 0 || depth + ceil_log2(std::max(nl, nr)) > max_depth)
+                if (nl == 0 || nr == 0 || depth + levels(std::max(nl, nr)) > max_depth)
                     continue;
                 float cost = left_area[k - 1] * (float)nl + half_area(acc) * (float)nr;
                 if (best_axis < 0 || cost < best_cost) {
@@ -271,15 +282,15 @@ struct SahBuilder {
         const Box &lb = left[0].box, &rb = right[0].box;
         unsigned swap = (lb.lo[axis] + lb.hi[axis]) > (rb.lo[axis] + rb.hi[axis]) ? 4u : 0u;
         out.reserve(1 + left.size() + right.size());
-        out.push_back(SahNode{bb, -1, (unsigned)(1 + left.size()), (unsigned)axis | swap});
+        out.push_back(SahNode{bb, -1, 0u, (unsigned)(1 + left.size()), (unsigned)axis | swap});
         unsigned rofs = (unsigned)(1 + left.size());
         for (SahNode n : left) {
-            if (n.tri < 0)
+            if (n.first < 0)
                 n.right += 1;
             out.push_back(n);
         }
         for (SahNode n : right) {
-            if (n.tri < 0)
+            if (n.first < 0)
                 n.right += rofs;
             out.push_back(n);
         }
@@ -315,11 +326,12 @@ extern "C" uint32_t phx_build_bvh_median(const PhxTriangleVertices *tri_verts, u
     return (uint32_t)nodes.size();
 }
 
-extern "C" uint32_t phx_build_bvh_sah(const PhxTriangleVertices *tri_verts, uint32_t n_tris, const uint32_t *leaf_key,
-                                      uint32_t max_depth, PhxBvhNode *out_nodes) {
-    if (!tri_verts || !leaf_key || !out_nodes || n_tris == 0 || n_tris >= 0x80000000u)
+extern "C" uint32_t phx_build_bvh_sah(const PhxTriangleVertices *tri_verts, uint32_t n_tris, uint32_t max_depth,
+                                      uint32_t max_leaf, PhxBvhNode *out_nodes, uint32_t *out_order) {
+    if (!tri_verts || !out_nodes || !out_order || n_tris == 0 || n_tris > PHX_PAIR_MAX_TRIS || max_leaf < 1 ||
+        max_leaf > PHX_PAIR_MAX_LEAF)
         return 0;
-    if (ceil_log2(n_tris) > max_depth + 1)
+    if (ceil_log2(ceil_div(n_tris, max_leaf)) > max_depth + 1)
         return 0; /* not even a balanced tree fits */
     std::vector<Box> boxes(n_tris);
     std::vector<float> cen[3];
@@ -332,7 +344,7 @@ extern "C" uint32_t phx_build_bvh_sah(const PhxTriangleVertices *tri_verts, uint
     }
     std::vector<unsigned> order(n_tris);
     std::iota(order.begin(), order.end(), 0u);
-    SahBuilder b{boxes, cen, order, max_depth};
+    SahBuilder b{boxes, cen, order, max_depth, max_leaf};
     std::vector<SahNode> nodes = b.build(0, n_tris, 0);
     for (size_t i = 0; i < nodes.size(); i++) {
         const SahNode &n = nodes[i];
@@ -343,9 +355,12 @@ extern "C" uint32_t phx_build_bvh_sah(const PhxTriangleVertices *tri_verts, uint
         o.max.x = n.box.hi[0];
         o.max.y = n.box.hi[1];
         o.max.z = n.box.hi[2];
-        o.min.w = phx_as_float(n.tri < 0 ? 0xFFFFFFF8u | n.hint : (uint32_t)n.tri);
-        o.max.w = phx_as_float(n.tri < 0 ? n.right : leaf_key[n.tri]);
+        o.min.w = phx_as_float(n.first < 0 ? 0xFFFFFFF8u | n.hint
+                                           : PHX_PAIR_LEAF | ((uint32_t)n.first << 2) | (n.count - 1u));
+        o.max.w = phx_as_float(n.first < 0 ? n.right : 0u);
     }
+    for (uint32_t i = 0; i < n_tris; i++)
+        out_order[i] = order[i];
     return (uint32_t)nodes.size();
 }
 
@@ -356,24 +371,24 @@ extern "C" int phx_build_bvh_pairs(const PhxBvhNode *tree, uint32_t n_nodes, Phx
     /* pair index of every interior node, in preorder */
     std::vector<uint32_t> pair_of(n_nodes, 0xFFFFFFFFu);
     uint32_t n_pairs = 0;
-    for (uint32_t i = 0; i < n_nodes; i++)
-        if (phx_as_uint(tree[i].min.w) >= 0xFFFFFFF8u)
+    for (uint32_t i = 0; i < n_nodes; i++) {
+        uint32_t w = phx_as_uint(tree[i].min.w);
+        if (w >= 0xFFFFFFF8u)
             pair_of[i] = n_pairs++;
+        else if (!(w & PHX_PAIR_LEAF))
+            return -1; /* neither interior nor a leaf reference */
+    }
     if (n_pairs != n_nodes / 2)
         return -1;
     auto ref = [&](uint32_t i) -> uint32_t {
         uint32_t w = phx_as_uint(tree[i].min.w);
-        return w >= 0xFFFFFFF8u ? pair_of[i] : (PHX_PAIR_LEAF | w);
+        return w >= 0xFFFFFFF8u ? pair_of[i] : w;
     };
     for (uint32_t i = 0; i < n_nodes; i++) {
         if (pair_of[i] == 0xFFFFFFFFu)
             continue;
         uint32_t l = i + 1, r = phx_as_uint(tree[i].max.w);
         if (l >= n_nodes || r >= n_nodes || r <= i)
-            return -1;
-        if (phx_as_uint(tree[l].min.w) < 0xFFFFFFF8u && phx_as_uint(tree[l].min.w) >= PHX_PAIR_LEAF)
-            return -1;
-        if (phx_as_uint(tree[r].min.w) < 0xFFFFFFF8u && phx_as_uint(tree[r].min.w) >= PHX_PAIR_LEAF)
             return -1;
         PhxBvhPair &p = out_pairs[pair_of[i]];
         p.lmin = tree[l].min;

@@ -42,13 +42,17 @@ struct SceneArgs {
     const float *envmap;
     uint32_t env_w, env_h, has_env;
     /* 1: search `pairs`, the library's SAH tree (root `root_ref`) with both
-     * children's boxes in each node, nearer child first, equal-distance ties
-     * settled by tri_key (each triangle's leaf index in the reference tree);
-     * 0: search `nodes`, the reference tree, depth-first. */
+     * children's boxes in each node, nearer child first; its leaves name
+     * positions in the tree's triangle order, whose vertices are `tvo` and
+     * whose triangle indices are `tri_order`; equal-distance ties are settled
+     * by tri_key (by position: the triangle's leaf index in the reference
+     * tree).  0: search `nodes`, the reference tree, depth-first. */
     uint32_t near_first;
     uint32_t root_ref;
     const PhxBvhPair *pairs;
     const uint32_t *tri_key;
+    const uint32_t *tri_order;
+    const PhxTriangleVertices *tvo;
 };
 
 /* ---- scan (single pass, decoupled look-back) ---- */

@@ -248,9 +248,9 @@ struct HitRecord {
     u32 tri_index, mat_index;
 };
 
-/* hit.h:45-78 */
-PHX_DEV bool is_triangle_hit(const SceneView &sc, u32 tri_idx, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec) {
-    const PhxTriangleVertices *tv = sc.tv + tri_idx;
+/* hit.h:45-78 for the triangle with vertices *tv, recorded as tri_idx */
+PHX_DEV bool is_triangle_hit(const PhxTriangleVertices *tv, u32 tri_idx, F4 o, F4 d, f32 t_min, f32 t_max,
+                             HitRecord &rec) {
     F4 v0 = ld(&tv->v0), v1 = ld(&tv->v1), v2 = ld(&tv->v2);
     F4 e1 = v1 - v0;
     F4 e2 = v2 - v0;
@@ -275,6 +275,10 @@ PHX_DEV bool is_triangle_hit(const SceneView &sc, u32 tri_idx, F4 o, F4 d, f32 t
     rec.v = v;
     rec.tri_index = tri_idx;
     return true;
+}
+
+PHX_DEV bool is_triangle_hit(const SceneView &sc, u32 tri_idx, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec) {
+    return is_triangle_hit(sc.tv + tri_idx, tri_idx, o, d, t_min, t_max, rec);
 }
 
 /* hit.h:80-133, one axis of the slab test; inv = 1/dir component (the same
@@ -350,10 +354,12 @@ PHX_DEV bool traverse(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, Hit
 }
 
 /* The hit traverse<ANY> finds on the reference tree, found on the library's
- * SAH tree of the same leaves (bvh.cpp phx_build_bvh_sah), stored as pairs:
- * each step loads one 64-byte node holding both children's boxes, tests both,
- * enters the one with the nearer entry point and pushes the other.  A leaf
- * child is a triangle reference, tested when its turn comes.
+ * SAH tree over the same triangles (bvh.cpp phx_build_bvh_sah), stored as
+ * pairs: each step loads one 64-byte node holding both children's boxes,
+ * tests both, enters the one with the nearer entry point and pushes the
+ * other.  A leaf child names one to four consecutive triangles of the tree's
+ * order (vertices in sc.tvo), tested when its turn comes; a hit records the
+ * position, mapped to the triangle index (sc.tri_order) at the end.
  *
  * Why the result is the same.  The reference's depth-first search with its
  * strict box test (hi > lo) returns the hit of least t; among several hits at
@@ -361,9 +367,10 @@ PHX_DEV bool traverse(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, Hit
  * (is_triangle_hit accepts t <= t_max, so a later equal hit overwrites an
  * earlier one, and nodes are stored in preorder) -- except at t == t_min,
  * where the first one met wins, because once closest == t_min every further
- * box fails the strict test.  The leaf boxes of both trees are the triangle
- * boxes expanded by 1e-3, so a leaf holding a hit at t in [t_min, closest] has
- * entry < t < exit by that margin and passes the inclusive test below
+ * box fails the strict test.  The reference's leaf boxes are the triangle
+ * boxes expanded by 1e-3 and every leaf box of the SAH tree contains its
+ * triangles' expanded boxes, so a leaf holding a hit at t in [t_min, closest]
+ * has entry < t < exit by that margin and passes the inclusive test below
  * whenever it is tested; a subtree is dropped only when no point of its box
  * lies in [t_min, closest].  So every candidate triangle is met whatever the
  * order, and "least t; at equal t the larger reference leaf index (tri_key),
@@ -425,25 +432,26 @@ PHX_DEV bool traverse_pairs(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_ma
     u32 ref = sc.root_ref;
 #define PHX_PAIR_LEAF_TEST(BREAK_STMT)                                                                                 \
     {                                                                                                                  \
-        u32 tri = ref & ~PHX_PAIR_LEAF;                                                                                \
-        HitRecord cand;                                                                                                \
-        if (is_triangle_hit(sc, tri, o, d, t_min, closest, cand)) {                                                    \
-            if (ANY) {                                                                                                 \
-                rec = cand;                                                                                            \
-                found = true;                                                                                          \
-                BREAK_STMT;                                                                                            \
-            }                                                                                                          \
-            bool take = !found || cand.t < closest;                                                                    \
-            if (!take) { /* cand.t == closest: the reference's order decides */                                        \
-                u32 kc = sc.tri_key[tri], kb = sc.tri_key[rec.tri_index];                                              \
-                take = (cand.t == t_min) ? kc < kb : kc > kb;                                                          \
-            }                                                                                                          \
-            if (take) {                                                                                                \
-                closest = cand.t;                                                                                      \
-                rec = cand;                                                                                            \
-                found = true;                                                                                          \
+        const u32 j0 = (ref & ~PHX_PAIR_LEAF) >> 2, j1 = j0 + (ref & 3u);                                              \
+        for (u32 j = j0; j <= j1; j++) {                                                                               \
+            HitRecord cand;                                                                                            \
+            if (is_triangle_hit(sc.tvo + j, j, o, d, t_min, closest, cand)) {                                          \
+                bool take = ANY || !found || cand.t < closest;                                                         \
+                if (!take) { /* cand.t == closest: the reference's order decides */                                    \
+                    u32 kc = sc.tri_key[j], kb = sc.tri_key[rec.tri_index];                                            \
+                    take = (cand.t == t_min) ? kc < kb : kc > kb;                                                      \
+                }                                                                                                      \
+                if (take) {                                                                                            \
+                    closest = cand.t;                                                                                  \
+                    rec = cand;                                                                                        \
+                    found = true;                                                                                      \
+                }                                                                                                      \
+                if (ANY)                                                                                               \
+                    break;                                                                                             \
             }                                                                                                          \
         }                                                                                                              \
+        if (ANY && found)                                                                                              \
+            BREAK_STMT;                                                                                                \
     }
 #define PHX_PAIR_STEP(NEAR_FAR, ONE, NONE)                                                                             \
     {                                                                                                                  \
@@ -468,7 +476,8 @@ PHX_DEV bool traverse_pairs(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_ma
         }                                                                                                              \
     }
     if (GROUP_LEAVES) {
-        /* DONE has the leaf bit but names no triangle (fewer than 2^31 - 1) */
+        /* DONE has the leaf bit but names no triangle position (a tree of
+           pairs orders at most PHX_PAIR_MAX_TRIS triangles) */
         constexpr u32 DONE = 0xFFFFFFFFu;
         for (;;) {
             while (!(ref & PHX_PAIR_LEAF)) {
@@ -504,8 +513,10 @@ PHX_DEV bool traverse_pairs(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_ma
     }
 #undef PHX_PAIR_STEP
 #undef PHX_PAIR_LEAF_TEST
-    if (found)
+    if (found) {
+        rec.tri_index = sc.tri_order[rec.tri_index];
         rec.mat_index = sc.tmat[rec.tri_index];
+    }
     return found;
 }
 

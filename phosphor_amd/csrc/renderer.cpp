@@ -67,7 +67,7 @@ int check_render_args(const PhxRenderArgs &a, const char *who) {
 int PhxRenderer::set_scene(const PhxSceneDesc &sc) {
     hipStream_t s = stream;
     const PhxRenderArgs &a = args;
-    int rc = upload_scene(sc, s, (a.flags & PHX_REFERENCE_BVH) != 0, buffers.scene);
+    int rc = upload_scene(sc, s, (a.flags & PHX_REFERENCE_BVH) != 0, a.bvh_leaf_size, buffers.scene);
     if (rc != PHX_OK)
         return rc;
     shadow_any_hit = buffers.scene.shadow_any_hit;

@@ -43,9 +43,10 @@ struct GeometryBuffers { /* include/buffers/geometry.hpp */
     DevBuf d_etri_verts, d_etri_normals, d_etri_uvs, d_etri_mat_index;
     uint32_t n_e_triangles = 0;
     DevBuf d_bvh_nodes;
-    /* the SAH tree's pairs and the triangles' reference leaf indices (empty
-       when the reference tree is searched) */
-    DevBuf d_bvh_pairs, d_tri_key;
+    /* the SAH tree's pairs; its triangle order, the triangles' vertices and
+       reference leaf indices in that order (empty when the reference tree is
+       searched) */
+    DevBuf d_bvh_pairs, d_tri_key, d_tri_order, d_tri_verts_ord;
     uint32_t root_ref = 0;
 };
 
@@ -85,7 +86,7 @@ struct SceneBuffers {
  * (scene_data.cpp:14-23), texture atlas, bounding sphere (shading.cpp:45-58)
  * and envmap (shading.cpp:60-71).  Asynchronous on `s` except for the BVH
  * build; the caller synchronises before freeing host memory. */
-int upload_scene(const PhxSceneDesc &sc, hipStream_t s, bool reference_bvh, SceneBuffers &out);
+int upload_scene(const PhxSceneDesc &sc, hipStream_t s, bool reference_bvh, uint32_t leaf_size, SceneBuffers &out);
 
 /* Scene validity checks of main.cpp:40-47 shared by the drivers. */
 int check_scene_desc(const PhxSceneDesc &sc, const char *who);

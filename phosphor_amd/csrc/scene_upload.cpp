@@ -56,6 +56,8 @@ SceneArgs SceneBuffers::args() const {
     a.root_ref = g.root_ref;
     a.pairs = g.d_bvh_pairs.as<PhxBvhPair>();
     a.tri_key = g.d_tri_key.as<uint32_t>();
+    a.tri_order = g.d_tri_order.as<uint32_t>();
+    a.tvo = g.d_tri_verts_ord.as<PhxTriangleVertices>();
     return a;
 }
 
@@ -75,7 +77,7 @@ int check_scene_desc(const PhxSceneDesc &sc, const char *who) {
             return hip_status(e_, what);                                                                               \
     } while (0)
 
-int upload_scene(const PhxSceneDesc &sc, hipStream_t s, bool reference_bvh, SceneBuffers &out) {
+int upload_scene(const PhxSceneDesc &sc, hipStream_t s, bool reference_bvh, uint32_t leaf_size, SceneBuffers &out) {
     uint32_t n = sc.n_triangles, ne = sc.n_e_triangles;
 
     /* Bvh bvh(scene.triangles) */
@@ -89,8 +91,14 @@ int upload_scene(const PhxSceneDesc &sc, hipStream_t s, bool reference_bvh, Scen
             return fail(PHX_ERR_INVALID_ARG, "set_scene: BVH build failed");
     }
     out.root = nodes[0];
-    std::vector<uint32_t> key;     /* uploaded below: live until the final synchronisation */
+    /* uploaded below: live until the final synchronisation */
+    std::vector<uint32_t> key, order;
     std::vector<PhxBvhPair> pairs;
+    std::vector<PhxTriangleVertices> tv_ord;
+    if (leaf_size == 0)
+        leaf_size = 3; /* fastest on the caustic and atrium workloads */
+    if (leaf_size > PHX_PAIR_MAX_LEAF)
+        return fail(PHX_ERR_INVALID_ARG, "set_scene: bvh_leaf_size beyond PHX_PAIR_MAX_LEAF");
     {
         bool transmissive = false;
         for (uint32_t i = 0; i < sc.n_materials; i++)
@@ -121,25 +129,33 @@ int upload_scene(const PhxSceneDesc &sc, hipStream_t s, bool reference_bvh, Scen
         const PhxBvhNode &rt = nodes[0];
         float extent = std::max({std::fabs(rt.min.x), std::fabs(rt.min.y), std::fabs(rt.min.z), std::fabs(rt.max.x),
                                  std::fabs(rt.max.y), std::fabs(rt.max.z)});
-        if (!sc.bvh_nodes && !reference_bvh && max_depth <= 22 && extent <= 1024.0f) {
-            key.assign(n, 0u);
+        if (!sc.bvh_nodes && !reference_bvh && max_depth <= 22 && extent <= 1024.0f && n <= PHX_PAIR_MAX_TRIS) {
+            std::vector<uint32_t> leaf_of(n, 0u);
             for (uint32_t i = 0; i < nodes.size(); i++) {
                 uint32_t t = phx_as_uint(nodes[i].min.w);
                 if (t != 0xFFFFFFFFu)
-                    key[t] = i;
+                    leaf_of[t] = i;
             }
             std::vector<PhxBvhNode> sah(nodes.size());
-            pairs.resize(std::max<size_t>(1, nodes.size() / 2));
+            order.resize(n);
+            uint32_t got = phx_build_bvh_sah(sc.tri_verts, n, 22, leaf_size, sah.data(), order.data());
+            pairs.resize(std::max<size_t>(1, got / 2));
             uint32_t root_ref = 0;
-            if (phx_build_bvh_sah(sc.tri_verts, n, key.data(), 22, sah.data()) == nodes.size() &&
-                phx_build_bvh_pairs(sah.data(), (uint32_t)sah.size(), pairs.data(), &root_ref) ==
-                    (int)(nodes.size() / 2)) {
+            if (got && phx_build_bvh_pairs(sah.data(), got, pairs.data(), &root_ref) == (int)(got / 2)) {
+                /* triangles and their tie-break keys in the tree's order */
+                key.resize(n);
+                tv_ord.resize(n);
+                for (uint32_t i = 0; i < n; i++) {
+                    key[i] = leaf_of[order[i]];
+                    tv_ord[i] = sc.tri_verts[order[i]];
+                }
                 GeometryBuffers &g = out.geometry;
-                PHX_TRY(copy_to_dev_buf(g.d_bvh_pairs, pairs.data(), nodes.size() / 2, sizeof(PhxBvhPair), s),
-                        "upload");
+                PHX_TRY(copy_to_dev_buf(g.d_bvh_pairs, pairs.data(), pairs.size(), sizeof(PhxBvhPair), s), "upload");
                 PHX_TRY(copy_to_dev_buf(g.d_tri_key, key.data(), n, sizeof(uint32_t), s), "upload");
+                PHX_TRY(copy_to_dev_buf(g.d_tri_order, order.data(), n, sizeof(uint32_t), s), "upload");
+                PHX_TRY(copy_to_dev_buf(g.d_tri_verts_ord, tv_ord.data(), n, sizeof(PhxTriangleVertices), s),
+                        "upload");
                 g.root_ref = root_ref;
-                nodes.swap(sah);
                 out.shallow_bvh = 1;
             }
         }
@@ -215,7 +231,7 @@ int upload_scene(const PhxSceneDesc &sc, hipStream_t s, bool reference_bvh, Scen
         sh.envmap_width = sh.envmap_height = 0;
         PHX_TRY(copy_to_dev_buf(sh.d_envmap_data, nullptr, 0, sizeof(float), s), "upload");
     }
-    /* `nodes`, `key`, `pairs` and `lum` are local: their uploads finish before they go */
+    /* `nodes`, `key`, `order`, `pairs`, `tv_ord` and `lum` are local: their uploads finish before they go */
     PHX_TRY(hipStreamSynchronize(s), "upload");
     return PHX_OK;
 }

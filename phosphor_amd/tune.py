@@ -4,8 +4,10 @@ Two kinds of knobs:
 
 * exact knobs change how the pass is launched, never its results: trace form
   (wavefront or one kernel), BVH (SAH tree visited near-first, or the
-  reference's median tree visited depth-first), emission batch size (``--batch-size``,
-  cmd_args.hpp:42) and the number of bounce-synchronous emission launches.
+  reference's median tree visited depth-first), the SAH tree's leaf size (1
+  to 4 triangles; the reference's is 1, bvh.cpp:24), emission batch size
+  (``--batch-size``, cmd_args.hpp:42) and the number of bounce-synchronous
+  emission launches.
   A candidate is accepted only if its SPPM state after the measured passes is
   bit-identical to the state under the given settings.
 * the grid resolution (``--grid-res``, cmd_args.hpp:51) changes the hash
@@ -67,7 +69,7 @@ class TuneResult:
 
 
 def autotune(cfg: PassConfig, scene, passes: int = 3, warmup: int = 1, wave_bounces=(1, 2, 3, 4, 6),
-             batch_sizes=None, trace_modes=(0, 1), cell_factors=(1.0, 1.155, 1.5, 2.0),
+             batch_sizes=None, trace_modes=(0, 1), leaf_sizes=(1, 2, 3, 4), cell_factors=(1.0, 1.155, 1.5, 2.0),
              rmse_tol: float = 1e-4) -> TuneResult:
     """Returns the fastest accepted configuration and every trial
     (knob, value, ms per pass, exact, relative RMSE, accepted)."""
@@ -91,6 +93,8 @@ def autotune(cfg: PassConfig, scene, passes: int = 3, warmup: int = 1, wave_boun
 
     consider("trace_mode", trace_modes, True)
     consider("reference_bvh", (False, True), True)
+    if not best.reference_bvh:
+        consider("bvh_leaf_size", [v for v in leaf_sizes if v != (best.bvh_leaf_size or 3)], True)
     consider("emit_wave_bounces", [w for w in wave_bounces if w <= cfg.max_photon_bounces], True)
     if batch_sizes is None:
         batch_sizes = sorted({min(cfg.photons_per_round, b) for b in (1 << 18, 1 << 20, cfg.photons_per_round)})

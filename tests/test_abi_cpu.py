@@ -88,60 +88,73 @@ def test_bvh_host_builder_matches_oracle(phx_lib, oracle_lib):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-def _sah_tree(phx_lib, sc, max_depth=22):
+def _sah_tree(phx_lib, sc, max_depth=22, max_leaf=1):
     n = sc.n_triangles
     med = np.zeros((2 * n - 1, 8), np.float32)
     assert phx_lib.phx_build_bvh_median(sc.tri_verts.ctypes.data, n, med.ctypes.data) == 2 * n - 1
-    key = np.zeros(n, np.uint32)
-    leaf = med[:, 3].view(np.uint32) != 0xFFFFFFFF
-    key[med[leaf, 3].view(np.uint32)] = np.nonzero(leaf)[0]
     sah = np.zeros((2 * n - 1, 8), np.float32)
-    got = phx_lib.phx_build_bvh_sah(sc.tri_verts.ctypes.data, n, key.ctypes.data, max_depth, sah.ctypes.data)
-    return med, key, sah, got
+    order = np.zeros(n, np.uint32)
+    got = phx_lib.phx_build_bvh_sah(sc.tri_verts.ctypes.data, n, max_depth, max_leaf, sah.ctypes.data,
+                                    order.ctypes.data)
+    return med, sah[:got], order, got
 
 
-@pytest.mark.parametrize("which", ["cornell", "caustic", "room"])
-def test_sah_tree_structure(phx_lib, which):
-    """The SAH tree holds every triangle in exactly one leaf with the same
-    expanded box as the reference tree's leaf, carries the reference leaf
-    index as key, nests child boxes in parent boxes, encodes interior hints
-    as 0xFFFFFFF8 | (axis | 4·swap) and stays within the depth limit."""
+def _leaf_range(w):
+    return (int(w) & 0x7FFFFFFF) >> 2, (int(w) & 3) + 1
+
+
+@pytest.mark.parametrize("which,max_leaf", [("cornell", 1), ("caustic", 1), ("room", 1), ("caustic", 2),
+                                            ("room", 4)])
+def test_sah_tree_structure(phx_lib, which, max_leaf):
+    """The SAH tree orders the triangles (a permutation); its leaves cover
+    that order with consecutive ranges of 1 .. max_leaf triangles, each leaf
+    box the union of its triangles' expanded boxes in the reference tree;
+    child boxes nest in parent boxes, interior hints are 0xFFFFFFF8 | (axis |
+    4·swap) and the tree stays within the depth limit.  The pairs hold each
+    interior node's two child boxes and references."""
     from phosphor_amd import scenes
 
     sc = {"cornell": lambda: scenes.cornell_box(8, 8), "caustic": lambda: scenes.caustic_scene(8, 8, level=4),
           "room": lambda: scenes.sphere_room(8, 8, 20_000, np.random.default_rng(5))}[which]()
     n = sc.n_triangles
-    med, key, sah, got = _sah_tree(phx_lib, sc)
-    assert got == 2 * n - 1
+    med, sah, order, got = _sah_tree(phx_lib, sc, max_leaf=max_leaf)
+    assert got > 0 and got % 2 == 1
+    assert np.array_equal(np.sort(order), np.arange(n, dtype=np.uint32))
     w_min, w_max = sah[:, 3].view(np.uint32), sah[:, 7].view(np.uint32)
     interior = w_min >= 0xFFFFFFF8
-    assert interior.sum() == n - 1
-    hints = w_min[interior] & 7
-    assert np.all((hints & 3) <= 2)
-    tris = w_min[~interior]
-    assert np.array_equal(np.sort(tris), np.arange(n, dtype=np.uint32))
-    assert np.array_equal(w_max[~interior], key[tris])
+    assert interior.sum() == got // 2
+    assert np.all((w_min[interior] & 3) <= 2)
+    assert np.all(w_min[~interior] & 0x80000000) and np.all(w_max[~interior] == 0)
     med_leaf = med[:, 3].view(np.uint32) != 0xFFFFFFFF
     med_box = np.zeros((n, 6), np.float32)
     med_box[med[med_leaf, 3].view(np.uint32)] = med[med_leaf][:, [0, 1, 2, 4, 5, 6]]
-    assert np.array_equal(sah[~interior][:, [0, 1, 2, 4, 5, 6]], med_box[tris])
-    depth = np.zeros(2 * n - 1, np.int64)
+    covered = np.zeros(n, np.int64)
+    for i in np.nonzero(~interior)[0]:
+        first, count = _leaf_range(w_min[i])
+        assert 1 <= count <= max_leaf and first + count <= n
+        covered[first:first + count] += 1
+        boxes = med_box[order[first:first + count]]
+        assert np.array_equal(sah[i, [0, 1, 2]], boxes[:, :3].min(0))
+        assert np.array_equal(sah[i, [4, 5, 6]], boxes[:, 3:].max(0))
+    assert np.all(covered == 1)
+    depth = np.zeros(got, np.int64)
     for i in np.nonzero(interior)[0]:  # preorder: parents come first
         l, r = i + 1, int(w_max[i])
-        assert i < r < 2 * n - 1
+        assert i < r < got
         for c in (l, r):
             depth[c] = depth[i] + 1
             assert np.all(sah[c, :3] >= sah[i, :3]) and np.all(sah[c, 4:7] <= sah[i, 4:7])
     assert depth[interior].max() <= 22
     # the same tree as pairs: each interior node's two child boxes and references
-    pairs = np.zeros((n - 1, 16), np.float32)
+    n_pairs = got // 2
+    pairs = np.zeros((max(1, n_pairs), 16), np.float32)
     root = np.zeros(1, np.uint32)
-    assert phx_lib.phx_build_bvh_pairs(sah.ctypes.data, 2 * n - 1, pairs.ctypes.data, root.ctypes.data) == n - 1
-    pair_of = np.full(2 * n - 1, -1, np.int64)
-    pair_of[np.nonzero(interior)[0]] = np.arange(n - 1)
+    assert phx_lib.phx_build_bvh_pairs(sah.ctypes.data, got, pairs.ctypes.data, root.ctypes.data) == n_pairs
+    pair_of = np.full(got, -1, np.int64)
+    pair_of[np.nonzero(interior)[0]] = np.arange(n_pairs)
 
     def ref(i):
-        return int(pair_of[i]) if interior[i] else 0x80000000 | int(w_min[i])
+        return int(pair_of[i]) if interior[i] else int(w_min[i])
 
     assert root[0] == ref(0)
     leaves = []
@@ -152,24 +165,37 @@ def test_sah_tree_structure(phx_lib, which):
         assert np.array_equal(p[[8, 9, 10, 12, 13, 14]], sah[r, [0, 1, 2, 4, 5, 6]])
         refs = p[[3, 11]].view(np.uint32)
         assert refs[0] == ref(l) and refs[1] == ref(r)
-        leaves += [int(x) & 0x7FFFFFFF for x in refs if x & 0x80000000]
+        for x in refs:
+            if x & 0x80000000:
+                first, count = _leaf_range(x)
+                leaves += list(range(first, first + count))
     assert sorted(leaves) == list(range(n))
 
 
 def test_sah_tree_depth_limit(phx_lib):
-    """A limit no balanced tree meets is refused; a tight one is met."""
+    """A limit no balanced tree meets is refused; a tight one is met, also
+    with four triangles per leaf (two levels fewer)."""
     from phosphor_amd import scenes
 
     sc = scenes.caustic_scene(8, 8, level=3)
-    k = int(np.ceil(np.log2(sc.n_triangles)))  # a balanced tree's deepest interior node is at depth k - 1
-    assert _sah_tree(phx_lib, sc, max_depth=k - 2)[3] == 0
-    med, key, sah, got = _sah_tree(phx_lib, sc, max_depth=k - 1)
-    assert got == 2 * sc.n_triangles - 1
-    w_min, w_max = sah[:, 3].view(np.uint32), sah[:, 7].view(np.uint32)
-    depth = np.zeros(len(sah), np.int64)
-    for i in np.nonzero(w_min >= 0xFFFFFFF8)[0]:
-        depth[i + 1] = depth[int(w_max[i])] = depth[i] + 1
-    assert depth[w_min >= 0xFFFFFFF8].max() == k - 1
+    for max_leaf, lv in ((1, 0), (4, 2)):
+        k = int(np.ceil(np.log2(sc.n_triangles))) - lv  # a balanced tree's deepest interior node is at depth k - 1
+        assert _sah_tree(phx_lib, sc, max_depth=k - 2, max_leaf=max_leaf)[3] == 0
+        med, sah, order, got = _sah_tree(phx_lib, sc, max_depth=k - 1, max_leaf=max_leaf)
+        assert got > 0
+        w_min, w_max = sah[:, 3].view(np.uint32), sah[:, 7].view(np.uint32)
+        depth = np.zeros(len(sah), np.int64)
+        for i in np.nonzero(w_min >= 0xFFFFFFF8)[0]:
+            depth[i + 1] = depth[int(w_max[i])] = depth[i] + 1
+        assert depth[w_min >= 0xFFFFFFF8].max() == k - 1
+
+
+def test_sah_tree_rejects_bad_leaf_size(phx_lib):
+    from phosphor_amd import scenes
+
+    sc = scenes.cornell_box(8, 8)
+    for bad in (0, 5):
+        assert _sah_tree(phx_lib, sc, max_leaf=bad)[3] == 0
 
 
 def test_no_device_reported_here(phx_lib):

@@ -67,19 +67,20 @@ def _near_first_cases():
     ]
 
 
-@pytest.mark.parametrize("index", [0, 1, 2])
-def test_sah_tree_matches_reference_tree(index):
+@pytest.mark.parametrize("index,leaf", [(0, 1), (1, 1), (2, 1), (0, 2), (0, 4), (1, 4), (2, 3)])
+def test_sah_tree_matches_reference_tree(index, leaf):
     """Where the reference's median tree is no deeper than 22 interior levels
-    the library traces on its own SAH tree, visiting nodes near-first and
-    breaking equal-distance ties by the reference tree's leaf order;
-    PHX_REFERENCE_BVH traces on the reference tree depth-first.  Hitpoints,
-    photon map and SPPM state are identical."""
+    the library traces on its own SAH tree (leaves of up to `leaf`
+    triangles), visiting nodes near-first and breaking equal-distance ties by
+    the reference tree's leaf order; PHX_REFERENCE_BVH traces on the
+    reference tree depth-first.  Hitpoints, photon map and SPPM state are
+    identical."""
     name, make, w, h, grid, radius = _near_first_cases()[index]
     scene = make()
     base = PassConfig(width=w, height=h, photons_per_round=1 << 16,
                       max_photons_per_batch=1 << 16, max_photon_bounces=16, grid_res=grid, seed=11,
-                      initial_radius=radius)
-    print(f"case {name}: {scene.n_triangles} triangles", flush=True)
+                      initial_radius=radius, bvh_leaf_size=leaf)
+    print(f"case {name}, leaf size {leaf}: {scene.n_triangles} triangles", flush=True)
     near = Renderer(base, scene)
     dfs = Renderer(PassConfig(**{**base.__dict__, "reference_bvh": True}), scene)
     for p in range(2):

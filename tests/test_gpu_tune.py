@@ -16,7 +16,7 @@ def test_autotune_small_scene():
     res = autotune(cfg, scene, passes=2, warmup=1, wave_bounces=(1, 2, 4), cell_factors=(1.155, 2.0))
     for knob, value, ms, exact, rmse, ok in res.trials:
         print(knob, value, f"{ms:.3f} ms", "exact" if exact else f"rmse {rmse:.2e}", "accepted" if ok else "rejected")
-        if knob in ("trace_mode", "reference_bvh", "emit_wave_bounces", "max_photons_per_batch"):
+        if knob in ("trace_mode", "reference_bvh", "bvh_leaf_size", "emit_wave_bounces", "max_photons_per_batch"):
             assert exact, (knob, value)
         if knob == "grid_res":
             assert rmse < 1e-3, (value, rmse)  # cells >= one radius find the same photons
