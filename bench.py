@@ -355,6 +355,12 @@ def main() -> int:
               f"{args.noise_ref_passes}-pass render: {rm:.4f} ({time.perf_counter() - t1:.1f} s)",
               file=sys.stderr, flush=True)
 
+    occupancy = None
+    try:
+        occupancy = {k: {"vgprs": v["vgprs"], "waves_per_simd": v["waves_per_simd"], "scratch_bytes": v["scratch_bytes"]}
+                     for k, v in P.kernel_info().items()}
+    except Exception as e:
+        print(f"[bench] kernel occupancy not available: {e}", file=sys.stderr)
     cpu = None
     rays = None
     if args.cpu_passes > 0:
@@ -409,6 +415,7 @@ def main() -> int:
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": read_traffic("k_gather_flat", args.workload)},
         "rays": rays,
+        "occupancy": occupancy,
         "quality": quality or None,
         "cpu_baseline": cpu,
     }

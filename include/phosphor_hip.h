@@ -265,6 +265,18 @@ int phx_gather_flux(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const
  * the same HIP runtime as the kernels)
  * ====================================================================== */
 int phx_device_count(void);
+/* Register use and occupancy of the pass's main kernels on the current
+ * device, as the HIP runtime reports them: VGPRs per lane, static LDS bytes
+ * per block, scratch bytes per lane, block size, and the waves per SIMD that
+ * can be resident at once (blocks per CU from the occupancy calculator ×
+ * waves per block / 4 SIMDs). */
+typedef struct PhxKernelInfo {
+    char name[32];
+    uint32_t vgprs, lds_bytes, scratch_bytes, block, waves_per_simd;
+} PhxKernelInfo;
+/* Writes up to `max` entries and returns the number of kernels described
+ * (call with max = 0 to count them), or a negative PHX_ERR_* (no device). */
+int phx_kernel_info(PhxKernelInfo *out, uint32_t max);
 int phx_device_malloc(void **out, size_t bytes);
 int phx_device_free(void *p);
 int phx_memcpy_htod(void *dst, const void *src, size_t bytes);

@@ -77,6 +77,21 @@ class PassConfig:
                           self.emit_wave_bounces, self.bvh_leaf_size)
 
 
+def kernel_info() -> dict:
+    """Registers and resident waves per SIMD of the pass's main kernels on the
+    current device (phx_kernel_info): {name: {vgprs, lds_bytes,
+    scratch_bytes, block, waves_per_simd}}."""
+    L = lib()
+    n = L.phx_kernel_info(None, 0)
+    arr = (abi.KernelInfo * max(n, 1))()
+    if n >= 0:
+        n = L.phx_kernel_info(arr, n)
+    if n < 0:
+        raise PhosphorError(-n, "phx_kernel_info: " + L.phx_last_error().decode(errors="replace"))
+    return {k.name.decode(): {f: int(getattr(k, f)) for f in ("vgprs", "lds_bytes", "scratch_bytes", "block",
+                                                                 "waves_per_simd")} for k in arr[:n]}
+
+
 class Renderer:
     """HIP pass driver (renderer.cpp:9-161) behind the C ABI."""
 

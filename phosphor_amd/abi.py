@@ -78,6 +78,13 @@ class BatchOptions(C.Structure):
                 ("reserved", C.c_uint32)]
 
 
+class KernelInfo(C.Structure):
+    """PhxKernelInfo — registers and resident waves of one kernel."""
+
+    _fields_ = [("name", C.c_char * 32), ("vgprs", C.c_uint32), ("lds_bytes", C.c_uint32),
+                ("scratch_bytes", C.c_uint32), ("block", C.c_uint32), ("waves_per_simd", C.c_uint32)]
+
+
 class SceneDesc(C.Structure):
     """PhxSceneDesc — host pointers of one scene (SceneData after the SoA split)."""
 
@@ -126,6 +133,7 @@ SIGNATURES = {
     "phx_sort_photons": (INT, VP, VP, VP, VP, VP, VP, U32, U32, VP, VP, VP, VP, SZ, VP),
     "phx_gather_flux": (INT, VP, VP, VP, VP, VP, VP, U32, VP, VP, VP, U32, VP, VP, VP, F32, VP, VP, VP, VP),
     "phx_device_count": (INT,),
+    "phx_kernel_info": (INT, C.POINTER(KernelInfo), U32),
     "phx_device_malloc": (INT, C.POINTER(VP), SZ),
     "phx_device_free": (INT, VP),
     "phx_memcpy_htod": (INT, VP, VP, SZ),

@@ -42,6 +42,8 @@ static_assert(offsetof(PhxMaterial, metallic) == 48 && offsetof(PhxMaterial, dif
 static_assert(sizeof(PhxTextureMeta) == 16, "TextureMeta");
 static_assert(sizeof(PhxLight) == 128 && offsetof(PhxLight, kind) == 112, "Light 128 B (light.h:14-31)");
 static_assert(sizeof(PhxPhotonHashData) == 48 && offsetof(PhxPhotonHashData, grid_res) == 32, "PhotonHashData");
+static_assert(sizeof(PhxRenderArgs) == 56 && offsetof(PhxRenderArgs, bvh_leaf_size) == 52, "PhxRenderArgs");
+static_assert(sizeof(PhxKernelInfo) == 52, "PhxKernelInfo");
 
 static SceneArgs scene_args(const PhxTriangleVertices *tv, const PhxTriangleNormals *tn,
                             const PhxTriangleTangents *tt, const PhxTriangleUvs *tuv, const PhxBvhNode *nodes,
@@ -288,6 +290,19 @@ int phx_device_count(void) {
     if (hipGetDeviceCount(&n) != hipSuccess)
         return 0;
     return n;
+}
+
+int phx_kernel_info(PhxKernelInfo *out, uint32_t max) {
+    if (max && !out)
+        return -fail(PHX_ERR_INVALID_ARG, "phx_kernel_info: null output");
+    if (phx_device_count() == 0)
+        return -fail(PHX_ERR_NO_DEVICE, "phx_kernel_info: no device");
+    uint32_t n = 0;
+    bool ok = kernel_info_trace(out, max, n);
+    ok = kernel_info_map(out, max, n) && ok;
+    if (!ok)
+        return -fail(PHX_ERR_NO_DEVICE, "phx_kernel_info: the runtime could not describe a kernel");
+    return (int)n;
 }
 
 int phx_device_malloc(void **out, size_t bytes) {

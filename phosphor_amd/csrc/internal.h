@@ -124,6 +124,12 @@ hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, 
 hipError_t launch_decode_photons(const uint32_t *ppow, const uint32_t *pdir, const uint32_t *d_n, uint32_t max_n,
                                  phx_float4 *dirpow, hipStream_t s);
 
+/* phx_kernel_info's entries for the trace/emission and the photon-map
+ * kernels: appended to out[n] while n < max, every kernel counted in n;
+ * false when the runtime cannot describe one */
+bool kernel_info_trace(PhxKernelInfo *out, uint32_t max, uint32_t &n);
+bool kernel_info_map(PhxKernelInfo *out, uint32_t max, uint32_t &n);
+
 /* ---- multi-scene passes ---- */
 /* One scene of a batch: its device scene, camera, grid and its ranges in the
  * batch-wide arrays (dense pixels, 128-pixel chunks, 8×8 gather tiles,

@@ -650,4 +650,14 @@ hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, 
     return hipGetLastError();
 }
 
+bool kernel_info_map(PhxKernelInfo *out, u32 max, u32 &n) {
+    PhxKernelInfo k;
+    bool ok = true;
+    PHX_INFO("k_hash", k_hash, 256)
+    PHX_INFO("k_radix_scatter", k_radix_scatter, SORT_BLOCK)
+    PHX_INFO("k_decode_photons", k_decode_photons, 256)
+    PHX_INFO("k_gather_flat", k_gather_flat, 256)
+    return ok;
+}
+
 } // namespace phx

@@ -634,4 +634,15 @@ hipError_t launch_emit(const SceneArgs &sc, u32 max_photons, u32 offset, u32 pho
     return launch_emit_compact(L, photons, max_photons, photon_pos, photon_power, photon_dir, n_photons, s);
 }
 
+bool kernel_info_trace(PhxKernelInfo *out, u32 max, u32 &n) {
+    PhxKernelInfo k;
+    bool ok = true;
+    PHX_INFO("k_trace_camera", k_trace_camera, WF_CAM_BLOCK)
+    PHX_INFO("k_trace_shadow", k_trace_shadow, WF_SHADOW_BLOCK)
+    PHX_INFO("k_trace_resolve", k_trace_resolve, 256)
+    PHX_INFO("k_emit_wf_bounce", k_emit_wf_bounce, EMIT_WF_BLOCK)
+    PHX_INFO("k_emit_wf_finish", k_emit_wf_finish, EMIT_WF_BLOCK)
+    return ok;
+}
+
 } // namespace phx

@@ -228,6 +228,38 @@ PHX_DEV F4 decode_oct(u32 packed) {
     return normalize(n);
 }
 
+/* registers and occupancy of one kernel (phx_kernel_info) */
+template <typename K>
+inline bool kernel_info(const char *name, K kernel, u32 block, PhxKernelInfo &out) {
+    hipFuncAttributes fa;
+    int blocks = 0;
+    if (hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(kernel)) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, (int)block, 0) != hipSuccess)
+        return false;
+    for (u32 i = 0; i < sizeof(out.name); i++) {
+        out.name[i] = name[i];
+        if (!name[i])
+            break;
+    }
+    out.name[sizeof(out.name) - 1] = 0;
+    out.vgprs = (u32)fa.numRegs;
+    out.lds_bytes = (u32)fa.sharedSizeBytes;
+    out.scratch_bytes = (u32)fa.localSizeBytes;
+    out.block = block;
+    out.waves_per_simd = (u32)blocks * (block / 64u) / 4u;
+    return true;
+}
+/* appends one kernel's entry to out[n] (when n < max), counting it in n;
+ * clears ok when the runtime cannot describe it */
+#define PHX_INFO(NAME, KERNEL, BLOCK)                                                                                 \
+    if (kernel_info(NAME, KERNEL, BLOCK, k)) {                                                                         \
+        if (n < max)                                                                                                   \
+            out[n] = k;                                                                                                \
+        n++;                                                                                                           \
+    } else {                                                                                                           \
+        ok = false;                                                                                                    \
+    }
+
 /* ---------------- photon_hash.h:21-37 ---------------- */
 PHX_DEV u32 get_hash(f32 px, f32 py, f32 pz, const PhxPhotonHashData &d) {
     u32 g = d.grid_res;

@@ -49,6 +49,8 @@ def test_struct_layouts_match_reference():
     assert abi.LIGHT_DTYPE.itemsize == 128 and abi.LIGHT_DTYPE.fields["kind"][1] == 112
     assert abi.HASH_DTYPE.itemsize == 48 and abi.HASH_DTYPE.fields["grid_res"][1] == 32
     assert abi.UV_BLOCK_DTYPE.itemsize == 192 and abi.TEX_META_DTYPE.itemsize == 16
+    # the library's own structs, as static_assert'ed in csrc/abi.cpp
+    assert C.sizeof(abi.RenderArgs) == 56 and C.sizeof(abi.KernelInfo) == 52
 
 
 def test_version_and_error_paths_without_device(phx_lib):
@@ -204,6 +206,7 @@ def test_no_device_reported_here(phx_lib):
     if os.path.exists("/dev/kfd"):
         pytest.skip("a GPU is present")
     assert phx_lib.phx_device_count() == 0
+    assert phx_lib.phx_kernel_info(None, 0) == -abi.PHX_ERR_NO_DEVICE
 
 
 def test_batch_rejects_bad_arguments(phx_lib):

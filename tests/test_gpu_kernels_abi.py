@@ -132,3 +132,16 @@ def test_trace_emit_gather_entry_points():
     assert np.array_equal(from_dev(dflux, np.uint32, (n_px, 4)), rf.view(np.uint32))
     assert np.array_equal(from_dev(dw, np.uint32, (n_px,)), rw.view(np.uint32))
     assert np.array_equal(from_dev(drad, np.uint32, (n_px, 4)), rr.view(np.uint32))
+
+
+def test_kernel_info_reports_registers_and_occupancy():
+    """phx_kernel_info describes the traversal, emission and photon-map
+    kernels: VGPRs within the 512-register file, at least one resident wave
+    per SIMD, the block sizes the launches use."""
+    info = P.kernel_info()
+    print(info)
+    for name in ("k_trace_camera", "k_trace_shadow", "k_trace_resolve", "k_emit_wf_bounce", "k_emit_wf_finish",
+                 "k_hash", "k_radix_scatter", "k_decode_photons", "k_gather_flat"):
+        k = info[name]
+        assert 0 < k["vgprs"] <= 512 and k["waves_per_simd"] >= 1, (name, k)
+    assert info["k_trace_shadow"]["block"] == 256 and info["k_emit_wf_bounce"]["block"] == 128
