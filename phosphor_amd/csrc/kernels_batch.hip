@@ -95,7 +95,10 @@ struct BShadowArgs {
     WfState st;
 };
 
-__global__ __launch_bounds__(WF_SHADOW_BLOCK) void k_bshadow(BShadowArgs a) {
+/* six waves per SIMD like k_trace_shadow (80 VGPRs, some spills), and five
+   for the emission kernels: the batch64 pass 302 -> 286 ms against the
+   unconstrained 126-VGPR build */
+__global__ __launch_bounds__(WF_SHADOW_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_bshadow(BShadowArgs a) {
     __shared__ u32 s_stack[BVH_STACK_SIZE * WF_SHADOW_BLOCK];
     u32 *stk = s_stack + threadIdx.x;
     const WfState &S = a.st;
@@ -155,7 +158,7 @@ __global__ __launch_bounds__(256) void k_bemit_start(BEmitArgs a) {
 }
 
 /* bounce a.depth of every live photon of the group; survivors go to the next queue */
-__global__ __launch_bounds__(EMIT_WF_BLOCK) void k_bemit_bounce(BEmitArgs a) {
+__global__ __launch_bounds__(EMIT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_bemit_bounce(BEmitArgs a) {
     __shared__ u32 s_stack[BVH_STACK_SIZE * EMIT_WF_BLOCK];
     u32 *stk = s_stack + threadIdx.x;
     const EmitWfState &S = a.st;
@@ -184,7 +187,7 @@ __global__ __launch_bounds__(EMIT_WF_BLOCK) void k_bemit_bounce(BEmitArgs a) {
 }
 
 /* the rest of every path still live at bounce a.depth, in one launch */
-__global__ __launch_bounds__(EMIT_WF_BLOCK) void k_bemit_finish(BEmitArgs a) {
+__global__ __launch_bounds__(EMIT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_bemit_finish(BEmitArgs a) {
     __shared__ u32 s_stack[BVH_STACK_SIZE * EMIT_WF_BLOCK];
     u32 *stk = s_stack + threadIdx.x;
     const EmitWfState &S = a.st;
