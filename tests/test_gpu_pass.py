@@ -9,7 +9,7 @@ per-pixel SPPM state must agree bit for bit after every pass.
 import numpy as np
 import pytest
 
-from phosphor_amd import PassConfig, Renderer, scenes
+from phosphor_amd import PassConfig, PhosphorError, Renderer, scenes
 
 import oracle
 
@@ -217,3 +217,44 @@ def test_pass_without_deposits_is_reported():
     assert r.last_photon_count == 0
     r.synchronize()  # reported once
     r.close()
+
+
+def _floor_and_light(w, h):
+    """Four triangles: a floor quad under a quad light, so a tree with leaves
+    of four triangles is a single leaf and one of two triangles a single
+    pair."""
+    from phosphor_amd.scenes import SceneBuilder, default_material, make_camera
+
+    b = SceneBuilder()
+    white = b.material(default_material((0.75, 0.75, 0.75)))
+    light = b.material(default_material((1.0, 1.0, 1.0), emissive=(17.0, 17.0, 17.0)))
+    b.add_quad((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), (1.0, 0.0, 1.0), (1.0, 0.0, 0.0), white)  # facing up
+    b.add_quad((0.4, 0.9, 0.4), (0.6, 0.9, 0.4), (0.6, 0.9, 0.6), (0.4, 0.9, 0.6), light)  # facing down
+    cam = make_camera((0.5, 0.6, 2.0), (0.5, 0.2, 0.5), (0.0, 1.0, 0.0), 45.0, w / h)
+    return b.build(cam, name="floor_and_light")
+
+
+@pytest.mark.parametrize("leaf", [1, 2, 4])
+def test_tiny_trees_bit_exact(leaf):
+    """A scene whose SAH tree is one leaf (four triangles, leaf size 4), one
+    pair of leaves (leaf size 2) or three pairs (leaf size 1) renders the
+    oracle's bits."""
+    scene = _floor_and_light(32, 24)
+    assert scene.n_triangles == 4
+    cfg = PassConfig(width=32, height=24, photons_per_round=1 << 12, max_photons_per_batch=1 << 12,
+                     max_photon_bounces=8, grid_res=16, seed=5, initial_radius=0.05, bvh_leaf_size=leaf)
+    print(f"case floor and light, leaf size {leaf}", flush=True)
+    gpu = Renderer(cfg, scene)
+    ref = oracle.RefRenderer(cfg, scene)
+    for p in range(2):
+        gpu.run_pass()
+        gpu.synchronize()
+        assert ref.run_pass() == 0
+        _compare_pass(gpu, ref, f"floor and light, leaf {leaf}, pass {p}")
+    gpu.close()
+
+
+def test_bad_leaf_size_rejected():
+    with pytest.raises(PhosphorError, match="bvh_leaf_size"):
+        Renderer(PassConfig(width=8, height=8, photons_per_round=64, grid_res=8, bvh_leaf_size=5),
+                 scenes.cornell_box(8, 8))
