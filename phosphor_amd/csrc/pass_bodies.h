@@ -31,7 +31,7 @@
 namespace phx {
 
 constexpr int WF_CAM_BLOCK = 128;
-constexpr int WF_SHADOW_BLOCK = 256;
+constexpr int WF_SHADOW_BLOCK = 128;
 constexpr int EMIT_WF_BLOCK = 128;
 constexpr u32 EMIT_WF_BOUNCES = 5; /* bounce-synchronous launches before the finishing launch */
 constexpr u32 NEE_NONE = 0, NEE_DIFFUSE = 1, NEE_SPECULAR = 2;

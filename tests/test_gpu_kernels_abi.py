@@ -144,4 +144,4 @@ def test_kernel_info_reports_registers_and_occupancy():
                  "k_hash", "k_radix_scatter", "k_decode_photons", "k_gather_flat"):
         k = info[name]
         assert 0 < k["vgprs"] <= 512 and k["waves_per_simd"] >= 1, (name, k)
-    assert info["k_trace_shadow"]["block"] == 256 and info["k_emit_wf_bounce"]["block"] == 128
+    assert info["k_trace_shadow"]["block"] == 128 and info["k_emit_wf_bounce"]["block"] == 128
