@@ -258,3 +258,27 @@ def test_bad_leaf_size_rejected():
     with pytest.raises(PhosphorError, match="bvh_leaf_size"):
         Renderer(PassConfig(width=8, height=8, photons_per_round=64, grid_res=8, bvh_leaf_size=5),
                  scenes.cornell_box(8, 8))
+
+
+@pytest.mark.parametrize("leaf", [1, 3])
+def test_textures_media_and_light_kinds_bit_exact(leaf):
+    """A textured floor (bilinear texture with sRGB decoding), an attenuating
+    glass box with thickness (Beer-Lambert on shadow rays, camera and photon
+    paths) and point, spot and directional lights: the features whose oracle
+    parts tests/test_oracle_lights_media.py pins by closed forms, rendered on
+    the GPU with the oracle's bits."""
+    from tests.test_oracle_lights_media import scene_with_everything
+
+    scene = scene_with_everything()
+    cfg = PassConfig(width=48, height=40, photons_per_round=1 << 13, max_photons_per_batch=1 << 13,
+                     max_photon_bounces=8, grid_res=24, seed=4, initial_radius=0.04, direct_samples=5,
+                     bvh_leaf_size=leaf)
+    print(f"case textures/media/lights, leaf size {leaf}", flush=True)
+    gpu = Renderer(cfg, scene)
+    ref = oracle.RefRenderer(cfg, scene)
+    for p in range(2):
+        gpu.run_pass()
+        gpu.synchronize()
+        assert ref.run_pass() == 0
+        _compare_pass(gpu, ref, f"textures/media/lights leaf {leaf} pass {p}")
+    gpu.close()
