@@ -25,7 +25,14 @@
  *
  * Parity pins: PCG streams, RGBE/octahedral codecs, get_hash, scan, scatter,
  * BVH layout, closest hit and the SPPM update are checked against closed
- * forms, brute force and numpy restatements in tests/.
+ * forms, brute force and numpy restatements in tests/; BSDF sampling, quad,
+ * point, spot and directional light sampling, NEE, photon emission,
+ * Beer-Lambert attenuation of shadow rays and bilinear texture sampling
+ * against closed forms and sampling statistics (tests/test_oracle_transport.py,
+ * tests/test_oracle_lights_media.py).
+ * Parity unpinned: the uv transform (rotation, scale, offset), the normal,
+ * emission, metal/roughness, transmission and occlusion texture slots and
+ * textured emissive lights (no closed-form test; no workload uses them).
  */
 #include "../include/phosphor_hip.h"
 #include "../phosphor_amd/csrc/phx_math.h"
