@@ -159,8 +159,8 @@ int PhxRenderer::trace_rays() {
 
 /* renderer.cpp:97-121; the count stays on the device (clamped to the buffer
  * by its readers, as renderer.cpp:115 clamps it) */
-int PhxRenderer::emit_photons() {
-    PHX_TRY(launch_fill_u32(d_batch_size.as<uint32_t>(), 0u, 1, stream), "emit_photons: reset counter");
+int PhxRenderer::emit_photons(hipStream_t s) {
+    PHX_TRY(launch_fill_u32(d_batch_size.as<uint32_t>(), 0u, 1, s), "emit_photons: reset counter");
     uint32_t ppr = args.photons_per_round;
     uint32_t per_batch = emit_chunk;
     SceneArgs sc = buffers.scene_args();
@@ -168,15 +168,15 @@ int PhxRenderer::emit_photons() {
         uint32_t to_emit = std::min(per_batch, ppr - off);
         PHX_TRY(launch_emit(sc, photon_buf_size, off, to_emit, args.max_photon_bounces, 0u,
                             d_photon_pos.as<phx_float4>(), d_photon_power.as<uint32_t>(), d_photon_dir.as<uint32_t>(),
-                            d_batch_size.as<uint32_t>(), d_emit_ws.p, stream, seed_ptr(), args.emit_wave_bounces),
+                            d_batch_size.as<uint32_t>(), d_emit_ws.p, s, seed_ptr(), args.emit_wave_bounces),
                 "emit_photons");
     }
-    PHX_TRY(launch_note_count(d_batch_size.as<uint32_t>(), d_info.as<uint32_t>(), stream), "emit_photons: count");
+    PHX_TRY(launch_note_count(d_batch_size.as<uint32_t>(), d_info.as<uint32_t>(), s), "emit_photons: count");
     return PHX_OK;
 }
 
 /* renderer.cpp:123-153, sized by the device-side count */
-int PhxRenderer::hash_and_sort_photons() {
+int PhxRenderer::hash_and_sort_photons(hipStream_t s) {
     PhotonBuffers &p = buffers.photon_buf;
     const uint32_t *d_n = d_batch_size.as<uint32_t>();
     uint32_t n_max = photon_buf_size;
@@ -184,16 +184,16 @@ int PhxRenderer::hash_and_sort_photons() {
        (the scan of the cell counts, renderer.cpp:134-147) are read off the
        sorted keys */
     PHX_TRY(launch_hash_dev(d_photon_pos.as<phx_float4>(), d_n, n_max, photon_hash_info,
-                            p.d_photon_hashes.as<uint32_t>(), nullptr, stream),
+                            p.d_photon_hashes.as<uint32_t>(), nullptr, s),
             "hash_photons");
     PHX_TRY(launch_sort(d_photon_pos.as<phx_float4>(), d_photon_power.as<uint32_t>(), d_photon_dir.as<uint32_t>(),
                         p.d_photon_hashes.as<uint32_t>(), nullptr, p.d_cell_end.as<uint32_t>(), n_max, bucket_count,
                         p.d_photon_pos.as<phx_float4>(), p.d_photon_power.as<uint32_t>(),
-                        p.d_photon_dir.as<uint32_t>(), d_sort_ws.p, stream, d_n, sort_perm,
+                        p.d_photon_dir.as<uint32_t>(), d_sort_ws.p, s, d_n, sort_perm,
                         p.d_cell_start.as<uint32_t>()),
             "sort_photons");
     PHX_TRY(launch_decode_photons(p.d_photon_power.as<uint32_t>(), p.d_photon_dir.as<uint32_t>(), d_n, n_max,
-                                  p.d_photon_dirpow.as<phx_float4>(), stream),
+                                  p.d_photon_dirpow.as<phx_float4>(), s),
             "decode_photons");
     return PHX_OK;
 }
@@ -219,18 +219,28 @@ int PhxRenderer::gather_flux() {
 int PhxRenderer::enqueue_pass(bool events) {
     int rc;
     PHX_TRY(launch_next_round_seed(d_rng.as<uint32_t>(), seed_ptr(), 1, stream), "round seed");
+    /* emission and photon map on the side stream, from the round seed on;
+       the camera trace meanwhile on the main stream; the gather after both */
+    PHX_TRY(hipEventRecord(ev_fork, stream), "event");
+    PHX_TRY(hipStreamWaitEvent(side, ev_fork, 0), "event");
+    if (events)
+        PHX_TRY(hipEventRecord(ev_emit, side), "event");
+    if ((rc = emit_photons(side)) != PHX_OK)
+        return rc;
+    if (events)
+        PHX_TRY(hipEventRecord(ev[2], side), "event");
+    if ((rc = hash_and_sort_photons(side)) != PHX_OK)
+        return rc;
+    if (events)
+        PHX_TRY(hipEventRecord(ev[3], side), "event");
+    PHX_TRY(hipEventRecord(ev_join, side), "event");
     if ((rc = trace_rays()) != PHX_OK)
         return rc;
     if (events)
         PHX_TRY(hipEventRecord(ev[1], stream), "event");
-    if ((rc = emit_photons()) != PHX_OK)
-        return rc;
+    PHX_TRY(hipStreamWaitEvent(stream, ev_join, 0), "event");
     if (events)
-        PHX_TRY(hipEventRecord(ev[2], stream), "event");
-    if ((rc = hash_and_sort_photons()) != PHX_OK)
-        return rc;
-    if (events)
-        PHX_TRY(hipEventRecord(ev[3], stream), "event");
+        PHX_TRY(hipEventRecord(ev_gather, stream), "event");
     return gather_flux();
 }
 
@@ -311,13 +321,19 @@ int phx_renderer_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, Ph
     std::unique_ptr<PhxRenderer> r(new PhxRenderer());
     r->args = *args;
     hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking);
     if (e != hipSuccess)
         return hip_status(e, "phx_renderer_create: stream");
     for (hipEvent_t &ev : r->ev)
         if ((e = hipEventCreate(&ev)) != hipSuccess)
             return hip_status(e, "phx_renderer_create: event");
-    if ((e = hipEventCreateWithFlags(&r->ev_info, hipEventDisableTiming)) != hipSuccess)
-        return hip_status(e, "phx_renderer_create: event");
+    for (hipEvent_t *ev : {&r->ev_emit, &r->ev_gather})
+        if ((e = hipEventCreate(ev)) != hipSuccess)
+            return hip_status(e, "phx_renderer_create: event");
+    for (hipEvent_t *ev : {&r->ev_info, &r->ev_fork, &r->ev_join})
+        if ((e = hipEventCreateWithFlags(ev, hipEventDisableTiming)) != hipSuccess)
+            return hip_status(e, "phx_renderer_create: event");
     if ((e = hipHostMalloc((void **)&r->h_info, 2 * sizeof(uint32_t), hipHostMallocDefault)) != hipSuccess)
         return hip_status(e, "phx_renderer_create: pinned info");
     r->h_info[0] = r->h_info[1] = 0;
@@ -364,8 +380,11 @@ int phx_renderer_last_stage_ms(PhxRenderer *r, float out_ms[4]) {
     hipError_t e = hipEventSynchronize(r->ev[4]);
     if (e != hipSuccess)
         return hip_status(e, "stage timing");
+    /* the trace and the emission overlap: each is timed on its own stream */
+    const hipEvent_t from[4] = {r->ev[0], r->ev_emit, r->ev[2], r->ev_gather};
+    const hipEvent_t to[4] = {r->ev[1], r->ev[2], r->ev[3], r->ev[4]};
     for (int k = 0; k < 4; k++)
-        if ((e = hipEventElapsedTime(&out_ms[k], r->ev[k], r->ev[k + 1])) != hipSuccess)
+        if ((e = hipEventElapsedTime(&out_ms[k], from[k], to[k])) != hipSuccess)
             return hip_status(e, "stage timing");
     return PHX_OK;
 }

@@ -42,7 +42,15 @@ struct PhxRenderer {
     typedef phx::DevBuf DevBuf;
     PhxRenderArgs args{};
     hipStream_t stream = nullptr;
+    /* the emission and the photon map run on `side` while the camera trace
+       runs on `stream` (the two are independent until the gather) */
+    hipStream_t side = nullptr;
+    /* pass timing: ev[0] start, ev[1] trace done, ev[2] emission done,
+       ev[3] photon map done, ev[4] end; ev_emit / ev_gather the starts of
+       the emission and of the gather */
     hipEvent_t ev[5] = {};
+    hipEvent_t ev_emit = nullptr, ev_gather = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr; /* no timing */
     hipEvent_t ev_info = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -78,10 +86,13 @@ struct PhxRenderer {
         for (hipEvent_t &e : ev)
             if (e)
                 (void)hipEventDestroy(e);
-        if (ev_info)
-            (void)hipEventDestroy(ev_info);
+        for (hipEvent_t e : {ev_emit, ev_gather, ev_fork, ev_join, ev_info})
+            if (e)
+                (void)hipEventDestroy(e);
         if (h_info)
             (void)hipHostFree(h_info);
+        if (side)
+            (void)hipStreamDestroy(side);
         if (stream)
             (void)hipStreamDestroy(stream);
     }
@@ -94,8 +105,10 @@ struct PhxRenderer {
     int collect();
     int check_passes();
     int trace_rays();
-    int emit_photons();
-    int hash_and_sort_photons();
+    int emit_photons() { return emit_photons(stream); }
+    int emit_photons(hipStream_t s);
+    int hash_and_sort_photons() { return hash_and_sort_photons(stream); }
+    int hash_and_sort_photons(hipStream_t s);
     int gather_flux();
 };
 

@@ -282,3 +282,23 @@ def test_textures_media_and_light_kinds_bit_exact(leaf):
         assert ref.run_pass() == 0
         _compare_pass(gpu, ref, f"textures/media/lights leaf {leaf} pass {p}")
     gpu.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_back_to_back_passes_bit_exact(graph):
+    """Passes enqueued with no host synchronisation between them (the camera
+    trace of a pass and its emission run on two streams, joined before the
+    gather; the next pass's emission starts after that gather): the state
+    after three passes is the oracle's."""
+    scene = scenes.caustic_scene(64, 48, level=3)
+    cfg = PassConfig(width=64, height=48, photons_per_round=1 << 14, max_photons_per_batch=1 << 14,
+                     max_photon_bounces=12, grid_res=60, seed=13, initial_radius=0.015, graph=graph)
+    print(f"case back-to-back, graph {graph}", flush=True)
+    gpu = Renderer(cfg, scene)
+    ref = oracle.RefRenderer(cfg, scene)
+    gpu.run(3)
+    gpu.synchronize()
+    for _ in range(3):
+        assert ref.run_pass() == 0
+    _compare_pass(gpu, ref, f"back-to-back graph={graph}")
+    gpu.close()
