@@ -31,7 +31,9 @@ struct PhxBatch {
     std::vector<PhxRenderArgs> args;
     PhxBatchOptions opt{};
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr; /* emission and photon map, beside the camera trace */
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_info = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<std::unique_ptr<SceneBuffers>> scenes;
@@ -55,11 +57,13 @@ struct PhxBatch {
             (void)hipGraphExecDestroy(graph_exec);
         if (graph)
             (void)hipGraphDestroy(graph);
-        for (hipEvent_t e : {ev0, ev1, ev_info})
+        for (hipEvent_t e : {ev0, ev1, ev_info, ev_fork, ev_join})
             if (e)
                 (void)hipEventDestroy(e);
         if (h_info)
             (void)hipHostFree(h_info);
+        if (side)
+            (void)hipStreamDestroy(side);
         if (stream)
             (void)hipStreamDestroy(stream);
     }
@@ -282,6 +286,9 @@ int phx_batch_create(const PhxRenderArgs *args, const PhxSceneDesc *scenes, uint
     if (options)
         b->opt = *options;
     PHX_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "phx_batch_create: stream");
+    PHX_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "phx_batch_create: stream");
+    PHX_TRY(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming), "phx_batch_create: event");
+    PHX_TRY(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming), "phx_batch_create: event");
     PHX_TRY(hipEventCreate(&b->ev0), "phx_batch_create: event");
     PHX_TRY(hipEventCreate(&b->ev1), "phx_batch_create: event");
     PHX_TRY(hipEventCreateWithFlags(&b->ev_info, hipEventDisableTiming), "phx_batch_create: event");
@@ -303,7 +310,7 @@ int phx_batch_run_pass(PhxBatch *b) {
     bool use_graph = (b->opt.flags & PHX_PASS_GRAPH) != 0;
     if (use_graph && !b->graph_exec) {
         PHX_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "batch graph capture");
-        hipError_t le = launch_batch_pass(b->plan, b->stream);
+        hipError_t le = launch_batch_pass(b->plan, b->stream, b->side, b->ev_fork, b->ev_join);
         hipGraph_t g = nullptr;
         hipError_t ce = hipStreamEndCapture(b->stream, &g);
         if (le != hipSuccess || ce != hipSuccess) {
@@ -318,7 +325,7 @@ int phx_batch_run_pass(PhxBatch *b) {
     if (use_graph)
         PHX_TRY(hipGraphLaunch(b->graph_exec, b->stream), "batch graph launch");
     else
-        PHX_TRY(launch_batch_pass(b->plan, b->stream), "batch pass");
+        PHX_TRY(launch_batch_pass(b->plan, b->stream, b->side, b->ev_fork, b->ev_join), "batch pass");
     PHX_TRY(hipEventRecord(b->ev1, b->stream), "event");
     size_t S = b->n();
     PHX_TRY(hipMemcpyAsync(b->h_info, b->d_counts.p, S * 4, hipMemcpyDeviceToHost, b->stream), "batch info");

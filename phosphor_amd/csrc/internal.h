@@ -180,7 +180,11 @@ size_t batch_emit_workspace_bytes(uint32_t max_photons, uint32_t max_bounces);
 /* persistent camera-kernel blocks that fit the device at once */
 uint32_t batch_camera_blocks();
 /* one pass over every scene of the plan (kernel launches only) */
-hipError_t launch_batch_pass(const BatchPlan &P, hipStream_t s);
+/* one batched pass: the camera trace on s, the emission and the photon map
+ * on `side` (forked after the round seeds with ev_fork, joined before the
+ * gather with ev_join) */
+hipError_t launch_batch_pass(const BatchPlan &P, hipStream_t s, hipStream_t side, hipEvent_t ev_fork,
+                             hipEvent_t ev_join);
 
 /* ---- adjoint ---- */
 /* what the gradient needs per material (host-filled, device table) */

@@ -313,10 +313,72 @@ uint32_t batch_camera_blocks() {
     } while (0)
 #define BLAUNCH() BTRY(hipGetLastError())
 
-hipError_t launch_batch_pass(const BatchPlan &P, hipStream_t s) {
+hipError_t launch_batch_pass(const BatchPlan &P, hipStream_t s, hipStream_t side, hipEvent_t ev_fork,
+                             hipEvent_t ev_join) {
     u32 S = P.n_scenes;
     BTRY(launch_next_round_seed(P.rng, P.round_seed, S, s));
     BTRY(launch_fill_u32(P.tickets, 0u, P.trace_groups.size(), s));
+    /* the emission and the photon map need only the round seeds: they run on
+       the side stream while the camera trace runs on s */
+    BTRY(hipEventRecord(ev_fork, s));
+    BTRY(hipStreamWaitEvent(side, ev_fork, 0));
+    {
+        hipStream_t main = s;
+        s = side;
+        /* emit_photons, group by group, packed scene by scene */
+        BTRY(launch_fill_u32(P.dep_counter, 0u, 1, s));
+        for (const BatchGroup &G : P.emit_groups) {
+            EmitLayout L = emit_layout(P.emit_ws, G.count, P.max_bounces);
+            BTRY(launch_fill_u32(L.st.qcount, 0u, (size_t)P.max_bounces + 2, s));
+            BEmitArgs a{P.scenes, P.ph_base, P.round_seed, G.s0, G.s1, G.first, G.count, 0u, L.st, L.g};
+            hipLaunchKernelGGL(k_bemit_start, dim3((G.count + 255) / 256), dim3(256), 0, s, a);
+            BLAUNCH();
+            u32 blocks = (G.count + EMIT_WF_BLOCK - 1) / EMIT_WF_BLOCK;
+            if (blocks > 4096)
+                blocks = 4096;
+            u32 wave_bounces = P.max_bounces < EMIT_WF_BOUNCES ? P.max_bounces : EMIT_WF_BOUNCES;
+            for (u32 d = 0; d < wave_bounces; d++) {
+                a.depth = d;
+                hipLaunchKernelGGL(k_bemit_bounce, dim3(blocks), dim3(EMIT_WF_BLOCK), 0, s, a);
+                BLAUNCH();
+            }
+            if (wave_bounces < P.max_bounces) {
+                a.depth = wave_bounces;
+                hipLaunchKernelGGL(k_bemit_finish, dim3(blocks), dim3(EMIT_WF_BLOCK), 0, s, a);
+                BLAUNCH();
+            }
+            BTRY(launch_scan(L.st.count, G.count, L.offs, nullptr, L.total, L.scan_ws, s));
+            hipLaunchKernelGGL(k_bdep_base, dim3(1), dim3(64), 0, s, P.ph_base, G.s0, G.s1, G.first, L.offs,
+                               P.dep_counter, P.dep_base);
+            BLAUNCH();
+            hipLaunchKernelGGL(k_bemit_compact, dim3((G.count + 255) / 256), dim3(256), 0, s, L.g, L.st.count, L.offs,
+                               G.count, P.dep_counter, P.dep_pos, P.dep_power, P.dep_dir);
+            BLAUNCH();
+            hipLaunchKernelGGL(k_bcounter_add, dim3(1), dim3(1), 0, s, P.dep_counter, L.total);
+            BLAUNCH();
+        }
+        hipLaunchKernelGGL(k_bdep_end, dim3(1), dim3(256), 0, s, S, P.dep_counter, P.dep_base, P.counts, P.zero_passes);
+        BLAUNCH();
+
+        /* hash, scan, stable sort over every scene's buckets */
+        BTRY(launch_fill_u32(P.cell_counts, 0u, P.total_buckets, s));
+        {
+            u32 blocks = (P.dep_cap + 255) / 256;
+            if (blocks > 8192)
+                blocks = 8192;
+            if (blocks == 0)
+                blocks = 1;
+            hipLaunchKernelGGL(k_bhash, dim3(blocks), dim3(256), 0, s, P.scenes, S, P.dep_base, P.dep_pos,
+                               P.dep_counter, P.dep_cap, P.hashes, P.cell_counts);
+            BLAUNCH();
+        }
+        BTRY(launch_scan(P.cell_counts, P.total_buckets, P.cell_start, P.cell_end, nullptr, P.scan_ws, s));
+        BTRY(launch_sort(P.dep_pos, P.dep_power, P.dep_dir, P.hashes, P.cell_start, P.cell_end, P.dep_cap,
+                         P.total_buckets, P.map_pos, P.map_power, P.map_dir, P.sort_ws, s, P.dep_counter));
+        BTRY(launch_decode_photons(P.map_power, P.map_dir, P.dep_counter, P.dep_cap, P.map_dirpow, s));
+        BTRY(hipEventRecord(ev_join, side));
+        s = main;
+    }
 
     /* trace_rays, group by group */
     WfState W = wf_layout(P.trace_ws, P.trace_max_pixels, P.max_direct_samples, true);
@@ -341,57 +403,7 @@ hipError_t launch_batch_pass(const BatchPlan &P, hipStream_t s) {
         BLAUNCH();
     }
 
-    /* emit_photons, group by group, packed scene by scene */
-    BTRY(launch_fill_u32(P.dep_counter, 0u, 1, s));
-    for (const BatchGroup &G : P.emit_groups) {
-        EmitLayout L = emit_layout(P.emit_ws, G.count, P.max_bounces);
-        BTRY(launch_fill_u32(L.st.qcount, 0u, (size_t)P.max_bounces + 2, s));
-        BEmitArgs a{P.scenes, P.ph_base, P.round_seed, G.s0, G.s1, G.first, G.count, 0u, L.st, L.g};
-        hipLaunchKernelGGL(k_bemit_start, dim3((G.count + 255) / 256), dim3(256), 0, s, a);
-        BLAUNCH();
-        u32 blocks = (G.count + EMIT_WF_BLOCK - 1) / EMIT_WF_BLOCK;
-        if (blocks > 4096)
-            blocks = 4096;
-        u32 wave_bounces = P.max_bounces < EMIT_WF_BOUNCES ? P.max_bounces : EMIT_WF_BOUNCES;
-        for (u32 d = 0; d < wave_bounces; d++) {
-            a.depth = d;
-            hipLaunchKernelGGL(k_bemit_bounce, dim3(blocks), dim3(EMIT_WF_BLOCK), 0, s, a);
-            BLAUNCH();
-        }
-        if (wave_bounces < P.max_bounces) {
-            a.depth = wave_bounces;
-            hipLaunchKernelGGL(k_bemit_finish, dim3(blocks), dim3(EMIT_WF_BLOCK), 0, s, a);
-            BLAUNCH();
-        }
-        BTRY(launch_scan(L.st.count, G.count, L.offs, nullptr, L.total, L.scan_ws, s));
-        hipLaunchKernelGGL(k_bdep_base, dim3(1), dim3(64), 0, s, P.ph_base, G.s0, G.s1, G.first, L.offs,
-                           P.dep_counter, P.dep_base);
-        BLAUNCH();
-        hipLaunchKernelGGL(k_bemit_compact, dim3((G.count + 255) / 256), dim3(256), 0, s, L.g, L.st.count, L.offs,
-                           G.count, P.dep_counter, P.dep_pos, P.dep_power, P.dep_dir);
-        BLAUNCH();
-        hipLaunchKernelGGL(k_bcounter_add, dim3(1), dim3(1), 0, s, P.dep_counter, L.total);
-        BLAUNCH();
-    }
-    hipLaunchKernelGGL(k_bdep_end, dim3(1), dim3(256), 0, s, S, P.dep_counter, P.dep_base, P.counts, P.zero_passes);
-    BLAUNCH();
-
-    /* hash, scan, stable sort over every scene's buckets */
-    BTRY(launch_fill_u32(P.cell_counts, 0u, P.total_buckets, s));
-    {
-        u32 blocks = (P.dep_cap + 255) / 256;
-        if (blocks > 8192)
-            blocks = 8192;
-        if (blocks == 0)
-            blocks = 1;
-        hipLaunchKernelGGL(k_bhash, dim3(blocks), dim3(256), 0, s, P.scenes, S, P.dep_base, P.dep_pos,
-                           P.dep_counter, P.dep_cap, P.hashes, P.cell_counts);
-        BLAUNCH();
-    }
-    BTRY(launch_scan(P.cell_counts, P.total_buckets, P.cell_start, P.cell_end, nullptr, P.scan_ws, s));
-    BTRY(launch_sort(P.dep_pos, P.dep_power, P.dep_dir, P.hashes, P.cell_start, P.cell_end, P.dep_cap,
-                     P.total_buckets, P.map_pos, P.map_power, P.map_dir, P.sort_ws, s, P.dep_counter));
-    BTRY(launch_decode_photons(P.map_power, P.map_dir, P.dep_counter, P.dep_cap, P.map_dirpow, s));
+    BTRY(hipStreamWaitEvent(s, ev_join, 0));
 
     /* gather_flux over every tile of every scene */
     GatherArgs ga{P.hp[0], P.hp[1], P.hp[2], P.hp[3], P.hp[4], P.hp[5], 0u, P.map_pos, P.map_power, P.map_dir,
