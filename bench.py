@@ -297,6 +297,10 @@ def main() -> int:
     ap.add_argument("--noise-ref-passes", type=int, default=1024, help="passes of that independent render")
     ap.add_argument("--bvh-leaf-size", type=int, default=None,
                     help="triangles per leaf of the SAH tree (1-4; same results, default: the workload's)")
+    ap.add_argument("--serial", action="store_true",
+                    help="each pass's gather ends before the next pass starts (default: pipelined; same results)")
+    ap.add_argument("--stage-passes", type=int, default=4,
+                    help="passes after the timed region, each waited for, that give the stage timings")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
@@ -316,6 +320,7 @@ def main() -> int:
     cfg, scene, _rounds = P.config_for(args.workload)
     if args.bvh_leaf_size is not None:
         cfg.bvh_leaf_size = args.bvh_leaf_size
+    cfg.serial = args.serial
     n_px = cfg.width * cfg.height
     print(f"[bench] workload={args.workload} tris={scene.n_triangles} {cfg.width}x{cfg.height} "
           f"photons/pass={cfg.photons_per_round} grid={cfg.grid_res}", file=sys.stderr, flush=True)
@@ -323,15 +328,20 @@ def main() -> int:
     for _ in range(args.warmup):
         r.run_pass()
     r.synchronize()
-    stage = []
-    counts = []
+    # timed region: passes enqueued back to back, no host synchronisation
     t0 = time.perf_counter()
     for i in range(args.steps):
+        r.run_pass()
+    r.synchronize()
+    dt = time.perf_counter() - t0
+    # stage timings: further passes, each waited for before the next is enqueued
+    stage = []
+    counts = []
+    for i in range(args.stage_passes):
         r.run_pass()
         stage.append(r.stage_ms())
         counts.append(r.last_photon_count)
     r.synchronize()
-    dt = time.perf_counter() - t0
     stage = np.array(stage)
     ms_per_step = dt * 1e3 / args.steps
     value = (n_px + cfg.photons_per_round) * args.steps / dt
@@ -373,8 +383,10 @@ def main() -> int:
             rays = {"counted_by": "oracle: scene_intersect calls of pass 1 (camera, shadow and photon rays; "
                                   "the GPU traces the same paths)",
                     "trace_per_pass": n_tr, "emit_per_pass": n_em,
-                    "note": "the trace and emission stages run concurrently on two streams; each stage's rate "
-                            "uses its own stream's time, rays_per_s the pass time before the gather",
+                    "note": "the trace and emission stages run concurrently on two streams (and, pipelined, "
+                            "beside the previous pass's gather); each stage's rate uses its own stream's time, "
+                            "measured on passes run one at a time; rays_per_s uses the timed pass rate less "
+                            "the gather's stage time",
                     "trace_rays_per_s": n_tr / (float(mean_stage[0]) * 1e-3),
                     "emit_rays_per_s": n_em / (float(mean_stage[1]) * 1e-3),
                     "rays_per_s": (n_tr + n_em) / ((ms_per_step - float(mean_stage[3])) * 1e-3)}
@@ -410,7 +422,8 @@ def main() -> int:
                    "height": cfg.height, "photons_per_pass": cfg.photons_per_round,
                    "max_bounces": cfg.max_photon_bounces, "grid_res": cfg.grid_res,
                    "initial_radius": cfg.initial_radius, "direct_samples": cfg.direct_samples,
-                   "bvh_leaf_size": cfg.bvh_leaf_size or 3, "parallelism": "single-gpu"},
+                   "bvh_leaf_size": cfg.bvh_leaf_size or 3, "passes": "serial" if cfg.serial else "pipelined",
+                   "parallelism": "single-gpu"},
         "stage_ms": {"trace": float(mean_stage[0]), "emit": float(mean_stage[1]),
                      "hash_scan_sort": float(mean_stage[2]), "gather": float(mean_stage[3])},
         "roofline": {"kernel": "k_gather_flat", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,

@@ -385,6 +385,12 @@ typedef struct PhxRenderArgs {
  * tree with near-first order (reference tree no deeper than 22 interior
  * levels, built by the library); results are identical either way. */
 #define PHX_REFERENCE_BVH 2u
+/* Run each pass's gather before the next pass starts.  Without this flag
+ * (and without PHX_PASS_GRAPH) the gather of pass N runs on a stream of its
+ * own beside the camera trace and the emission of pass N+1; the hitpoints and
+ * the sorted photon map then alternate between two sets of buffers (twice
+ * their device memory).  Results are identical either way. */
+#define PHX_PASS_SERIAL 4u
 
 typedef struct PhxRenderer PhxRenderer;
 
@@ -419,7 +425,9 @@ int phx_renderer_read_hitpoints(PhxRenderer *r, phx_float4 *hp6);
 /* Copy the last pass's sorted photon map (n = last clamped count) and cell ranges. */
 int phx_renderer_read_photon_map(PhxRenderer *r, phx_float4 *pos, uint32_t *power, uint32_t *dir,
                                  uint32_t *cell_start, uint32_t *cell_end);
-/* Device pointers of the renderer's buffers (for the Python tooling): hp_pos …
+/* Device pointers of the renderer's buffers (for the Python tooling), after
+ * waiting for every enqueued pass; the hitpoint and photon-map pointers are
+ * the last pass's set (pipelined passes alternate between two): hp_pos …
  * hp_emission, flux, photon_weight, radiance, sorted pos/power/dir, cell
  * start/end, unsorted pos/power/dir, hash keys, NULL (the reference's cell
  * counts: the pass reads cell ranges off the sorted keys instead), stream. */

@@ -68,12 +68,14 @@ class PassConfig:
     emit_wave_bounces: int = 0  # bounce-synchronous emission launches (0 = library default); same results
     reference_bvh: bool = False  # reference median BVH + depth-first order instead of SAH + near-first; same results
     bvh_leaf_size: int = 0  # triangles per leaf of the SAH tree (0 = library default 3); same results
+    serial: bool = False  # each gather ends before the next pass starts (no second buffer set); same results
 
     def to_c(self) -> RenderArgs:
         return RenderArgs(self.width, self.height, self.photons_per_round, self.max_photons_per_batch,
                           self.max_photon_bounces, self.grid_res, self.direct_samples, self.seed,
                           self.sppm_alpha, self.initial_radius, self.trace_mode,
-                          (abi.PHX_PASS_GRAPH if self.graph else 0) | (abi.PHX_REFERENCE_BVH if self.reference_bvh else 0),
+                          (abi.PHX_PASS_GRAPH if self.graph else 0) | (abi.PHX_REFERENCE_BVH if self.reference_bvh else 0)
+                          | (abi.PHX_PASS_SERIAL if self.serial else 0),
                           self.emit_wave_bounces, self.bvh_leaf_size)
 
 

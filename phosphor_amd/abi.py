@@ -68,6 +68,7 @@ class RenderArgs(C.Structure):
 
 PHX_PASS_GRAPH = 1
 PHX_REFERENCE_BVH = 2
+PHX_PASS_SERIAL = 4
 PHX_PAIR_MAX_LEAF = 4
 
 

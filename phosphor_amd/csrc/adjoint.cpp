@@ -119,7 +119,7 @@ int PhxAdjoint::apply(const float *th) {
 int PhxAdjoint::forward(const float *th, uint32_t n_rounds, float *image) {
     if (n_rounds == 0 || n_rounds > max_rounds)
         return fail(PHX_ERR_INVALID_ARG, "phx_adjoint_forward: rounds outside 1..max_rounds");
-    PHX_TRY(hipStreamSynchronize(r->stream), "adjoint");
+    PHX_TRY(r->sync_all(), "adjoint");
     int rc = apply(th);
     if (rc != PHX_OK)
         return rc;
@@ -158,6 +158,7 @@ int PhxAdjoint::forward(const float *th, uint32_t n_rounds, float *image) {
 int PhxAdjoint::backward(const float *dl, float *grad) {
     if (rounds == 0)
         return fail(PHX_ERR_INVALID_ARG, "phx_adjoint_backward: no forward run");
+    PHX_TRY(r->sync_all(), "adjoint");
     hipStream_t s = r->stream;
     ImageBuffers &im = r->buffers.image_buf;
     /* λ of the two terms of output.cpp:52-83: L = radiance/rounds + τ·inv_ph/(π r²) */

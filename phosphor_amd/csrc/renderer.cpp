@@ -55,7 +55,7 @@ int check_render_args(const PhxRenderArgs &a, const char *who) {
         return fail(PHX_ERR_INVALID_ARG, w + ": trace_mode must be 0 or 1");
     if (a.direct_samples == 0 || a.direct_samples > 128)
         return fail(PHX_ERR_INVALID_ARG, w + ": direct_samples outside 1..128 (cmd_args.hpp:59)");
-    if (a.flags & ~(uint32_t)(PHX_PASS_GRAPH | PHX_REFERENCE_BVH))
+    if (a.flags & ~(uint32_t)(PHX_PASS_GRAPH | PHX_REFERENCE_BVH | PHX_PASS_SERIAL))
         return fail(PHX_ERR_INVALID_ARG, w + ": unknown flags");
     if (a.emit_wave_bounces > 32)
         return fail(PHX_ERR_INVALID_ARG, w + ": emit_wave_bounces outside 0..32");
@@ -139,21 +139,40 @@ int PhxRenderer::set_scene(const PhxSceneDesc &sc) {
     return PHX_OK;
 }
 
+/* the six hitpoint buffers of a set (0: in `buffers`, 1: alt_hp) */
+void PhxRenderer::hp_bufs(int set, DevBuf *out[6]) {
+    ImageBuffers &i = buffers.image_buf;
+    DevBuf *b[6] = {&i.d_hp_pos, &i.d_hp_normal, &i.d_hp_throughput, &i.d_hp_base_color, &i.d_hp_direct,
+                    &i.d_hp_emission};
+    for (int k = 0; k < 6; k++)
+        out[k] = set ? &alt_hp[k] : b[k];
+}
+
+/* the sorted photon map of a set: pos, power, dir, dirpow, cell_start, cell_end */
+void PhxRenderer::map_bufs(int set, DevBuf *out[6]) {
+    PhotonBuffers &p = buffers.photon_buf;
+    DevBuf *b[6] = {&p.d_photon_pos, &p.d_photon_power, &p.d_photon_dir, &p.d_photon_dirpow, &p.d_cell_start,
+                    &p.d_cell_end};
+    for (int k = 0; k < 6; k++)
+        out[k] = set ? &alt_map[k] : b[k];
+}
+
 /* renderer.cpp:90-95 */
-int PhxRenderer::trace_rays() {
+int PhxRenderer::trace_rays(int set) {
     const ImageBuffers &i = buffers.image_buf;
+    DevBuf *h[6];
+    hp_bufs(set, h);
     if (args.trace_mode == 0)
         return hip_status(launch_trace_wavefront(i.camera, i.image_width, i.image_height, 0u, args.direct_samples,
-                                                 buffers.scene_args(), shadow_any_hit, i.d_hp_pos.as<phx_float4>(),
-                                                 i.d_hp_normal.as<phx_float4>(), i.d_hp_throughput.as<phx_float4>(),
-                                                 i.d_hp_base_color.as<phx_float4>(), i.d_hp_direct.as<phx_float4>(),
-                                                 i.d_hp_emission.as<phx_float4>(), d_trace_ws.p, stream, seed_ptr()),
+                                                 buffers.scene_args(), shadow_any_hit, h[0]->as<phx_float4>(),
+                                                 h[1]->as<phx_float4>(), h[2]->as<phx_float4>(),
+                                                 h[3]->as<phx_float4>(), h[4]->as<phx_float4>(),
+                                                 h[5]->as<phx_float4>(), d_trace_ws.p, stream, seed_ptr()),
                           "trace_rays");
     return hip_status(launch_trace(i.camera, i.image_width, i.image_height, 0u, args.direct_samples,
-                                   buffers.scene_args(), i.d_hp_pos.as<phx_float4>(), i.d_hp_normal.as<phx_float4>(),
-                                   i.d_hp_throughput.as<phx_float4>(), i.d_hp_base_color.as<phx_float4>(),
-                                   i.d_hp_direct.as<phx_float4>(), i.d_hp_emission.as<phx_float4>(), stream,
-                                   seed_ptr()),
+                                   buffers.scene_args(), h[0]->as<phx_float4>(), h[1]->as<phx_float4>(),
+                                   h[2]->as<phx_float4>(), h[3]->as<phx_float4>(), h[4]->as<phx_float4>(),
+                                   h[5]->as<phx_float4>(), stream, seed_ptr()),
                       "trace_rays");
 }
 
@@ -176,8 +195,10 @@ int PhxRenderer::emit_photons(hipStream_t s) {
 }
 
 /* renderer.cpp:123-153, sized by the device-side count */
-int PhxRenderer::hash_and_sort_photons(hipStream_t s) {
+int PhxRenderer::hash_and_sort_photons(hipStream_t s, int set) {
     PhotonBuffers &p = buffers.photon_buf;
+    DevBuf *m[6];
+    map_bufs(set, m);
     const uint32_t *d_n = d_batch_size.as<uint32_t>();
     uint32_t n_max = photon_buf_size;
     /* keys only: the stable sort orders the records, and the cell ranges
@@ -187,40 +208,101 @@ int PhxRenderer::hash_and_sort_photons(hipStream_t s) {
                             p.d_photon_hashes.as<uint32_t>(), nullptr, s),
             "hash_photons");
     PHX_TRY(launch_sort(d_photon_pos.as<phx_float4>(), d_photon_power.as<uint32_t>(), d_photon_dir.as<uint32_t>(),
-                        p.d_photon_hashes.as<uint32_t>(), nullptr, p.d_cell_end.as<uint32_t>(), n_max, bucket_count,
-                        p.d_photon_pos.as<phx_float4>(), p.d_photon_power.as<uint32_t>(),
-                        p.d_photon_dir.as<uint32_t>(), d_sort_ws.p, s, d_n, sort_perm,
-                        p.d_cell_start.as<uint32_t>()),
+                        p.d_photon_hashes.as<uint32_t>(), nullptr, m[5]->as<uint32_t>(), n_max, bucket_count,
+                        m[0]->as<phx_float4>(), m[1]->as<uint32_t>(), m[2]->as<uint32_t>(), d_sort_ws.p, s, d_n,
+                        sort_perm, m[4]->as<uint32_t>()),
             "sort_photons");
-    PHX_TRY(launch_decode_photons(p.d_photon_power.as<uint32_t>(), p.d_photon_dir.as<uint32_t>(), d_n, n_max,
-                                  p.d_photon_dirpow.as<phx_float4>(), s),
+    PHX_TRY(launch_decode_photons(m[1]->as<uint32_t>(), m[2]->as<uint32_t>(), d_n, n_max, m[3]->as<phx_float4>(), s),
             "decode_photons");
     return PHX_OK;
 }
 
 /* renderer.cpp:155-161 */
-int PhxRenderer::gather_flux() {
+int PhxRenderer::gather_flux(hipStream_t s, int set) {
     const ImageBuffers &i = buffers.image_buf;
-    const PhotonBuffers &p = buffers.photon_buf;
-    return hip_status(launch_gather(i.d_hp_pos.as<phx_float4>(), i.d_hp_normal.as<phx_float4>(),
-                                    i.d_hp_throughput.as<phx_float4>(), i.d_hp_base_color.as<phx_float4>(),
-                                    i.d_hp_direct.as<phx_float4>(), i.d_hp_emission.as<phx_float4>(), i.n_pixels,
-                                    p.d_photon_pos.as<phx_float4>(), p.d_photon_power.as<uint32_t>(),
-                                    p.d_photon_dir.as<uint32_t>(), p.d_cell_start.as<uint32_t>(),
-                                    p.d_cell_end.as<uint32_t>(), photon_hash_info, args.sppm_alpha,
+    DevBuf *h[6], *m[6];
+    hp_bufs(set, h);
+    map_bufs(set, m);
+    return hip_status(launch_gather(h[0]->as<phx_float4>(), h[1]->as<phx_float4>(), h[2]->as<phx_float4>(),
+                                    h[3]->as<phx_float4>(), h[4]->as<phx_float4>(), h[5]->as<phx_float4>(),
+                                    i.n_pixels, m[0]->as<phx_float4>(), m[1]->as<uint32_t>(), m[2]->as<uint32_t>(),
+                                    m[4]->as<uint32_t>(), m[5]->as<uint32_t>(), photon_hash_info, args.sppm_alpha,
                                     i.d_flux.as<phx_float4>(), i.d_photon_weight.as<float>(),
-                                    i.d_radiance.as<phx_float4>(), i.image_width, stream, rec_ratio, rec_r2,
-                                    p.d_photon_dirpow.as<phx_float4>()),
+                                    i.d_radiance.as<phx_float4>(), i.image_width, s, rec_ratio, rec_r2,
+                                    m[3]->as<phx_float4>()),
                       "gather_flux");
 }
 
+/* the second set of hitpoint and photon-map buffers, on the first pipelined
+ * pass (cleared, like the first set) */
+int PhxRenderer::alloc_alt() {
+    if (alt_ready)
+        return PHX_OK;
+    DevBuf *h[6], *m[6];
+    hp_bufs(0, h);
+    map_bufs(0, m);
+    for (int k = 0; k < 6; k++) {
+        PHX_TRY(alt_hp[k].alloc(h[k]->bytes), "alloc second hitpoint set");
+        PHX_TRY(hipMemsetAsync(alt_hp[k].p, 0, alt_hp[k].bytes, stream), "clear second hitpoint set");
+        PHX_TRY(alt_map[k].alloc(m[k]->bytes), "alloc second photon map");
+        PHX_TRY(hipMemsetAsync(alt_map[k].p, 0, alt_map[k].bytes, stream), "clear second photon map");
+    }
+    alt_ready = true;
+    return PHX_OK;
+}
+
+/* after a pipelined pass: its set becomes the one in `buffers` */
+void PhxRenderer::swap_sets() {
+    DevBuf *h[6], *m[6];
+    hp_bufs(0, h);
+    map_bufs(0, m);
+    for (int k = 0; k < 6; k++) {
+        h[k]->swap(alt_hp[k]);
+        m[k]->swap(alt_map[k]);
+    }
+    std::swap(ev_read[0], ev_read[1]);
+    std::swap(read_pending[0], read_pending[1]);
+}
+
+/* waits for every stream of the renderer */
+hipError_t PhxRenderer::sync_all() {
+    for (hipStream_t s : {stream, side, gstream}) {
+        hipError_t e = s ? hipStreamSynchronize(s) : hipSuccess;
+        if (e != hipSuccess)
+            return e;
+    }
+    read_pending[0] = read_pending[1] = false;
+    emit_pending = false;
+    return hipSuccess;
+}
+
 /* renderer.cpp:64-78, one round, enqueued without host synchronisation;
- * stage events are recorded between the stages when `events` is set */
-int PhxRenderer::enqueue_pass(bool events) {
+ * stage events are recorded between the stages when `events` is set.
+ *
+ * Serial form: round seed, then the emission and the photon map on `side`
+ * beside the camera trace on `stream`, then the gather on `stream` after both.
+ * Pipelined form (`pipe`): the same, with the trace and the photon map
+ * written into the second set and the gather on `gstream`, so that the next
+ * pass's seed, trace and emission do not wait for it.  Ordering:
+ *   - the round seed waits for the previous emission (which reads the seed);
+ *   - the trace and the photon map overwrite the set the gather before last
+ *     read: `stream` waits for that gather before the seed, and `side` follows
+ *     `stream` from the seed on;
+ *   - the gather waits for the trace and the photon map; gathers run in order
+ *     on `gstream`, and a serial pass first waits for every pending gather. */
+int PhxRenderer::enqueue_pass(bool events, bool pipe) {
     int rc;
+    const int set = pipe ? 1 : 0;
+    if (emit_pending) {
+        PHX_TRY(hipStreamWaitEvent(stream, ev_emit_done, 0), "event");
+        emit_pending = false;
+    }
+    for (int k = pipe ? 1 : 0; k < 2; k++)
+        if (read_pending[k]) {
+            PHX_TRY(hipStreamWaitEvent(stream, ev_read[k], 0), "event");
+            read_pending[k] = false;
+        }
     PHX_TRY(launch_next_round_seed(d_rng.as<uint32_t>(), seed_ptr(), 1, stream), "round seed");
-    /* emission and photon map on the side stream, from the round seed on;
-       the camera trace meanwhile on the main stream; the gather after both */
     PHX_TRY(hipEventRecord(ev_fork, stream), "event");
     PHX_TRY(hipStreamWaitEvent(side, ev_fork, 0), "event");
     if (events)
@@ -229,26 +311,49 @@ int PhxRenderer::enqueue_pass(bool events) {
         return rc;
     if (events)
         PHX_TRY(hipEventRecord(ev[2], side), "event");
-    if ((rc = hash_and_sort_photons(side)) != PHX_OK)
+    if (pipe) {
+        PHX_TRY(hipMemcpyAsync(h_info, d_info.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, side), "pass info");
+        PHX_TRY(hipEventRecord(ev_info, side), "event");
+        PHX_TRY(hipEventRecord(ev_emit_done, side), "event");
+        emit_pending = true;
+    }
+    if ((rc = hash_and_sort_photons(side, set)) != PHX_OK)
         return rc;
     if (events)
         PHX_TRY(hipEventRecord(ev[3], side), "event");
     PHX_TRY(hipEventRecord(ev_join, side), "event");
-    if ((rc = trace_rays()) != PHX_OK)
+    if (events)
+        PHX_TRY(hipEventRecord(ev_trace, stream), "event");
+    if ((rc = trace_rays(set)) != PHX_OK)
         return rc;
     if (events)
         PHX_TRY(hipEventRecord(ev[1], stream), "event");
-    PHX_TRY(hipStreamWaitEvent(stream, ev_join, 0), "event");
+    if (!pipe) {
+        PHX_TRY(hipStreamWaitEvent(stream, ev_join, 0), "event");
+        if (events)
+            PHX_TRY(hipEventRecord(ev_gather, stream), "event");
+        return gather_flux(stream, 0);
+    }
+    PHX_TRY(hipEventRecord(ev_trace_done, stream), "event");
+    PHX_TRY(hipStreamWaitEvent(gstream, ev_trace_done, 0), "event");
+    PHX_TRY(hipStreamWaitEvent(gstream, ev_join, 0), "event");
     if (events)
-        PHX_TRY(hipEventRecord(ev_gather, stream), "event");
-    return gather_flux();
+        PHX_TRY(hipEventRecord(ev_gather, gstream), "event");
+    if ((rc = gather_flux(gstream, 1)) != PHX_OK)
+        return rc;
+    if (events)
+        PHX_TRY(hipEventRecord(ev[4], gstream), "event");
+    PHX_TRY(hipEventRecord(ev_read[1], gstream), "event");
+    read_pending[1] = true;
+    swap_sets();
+    return PHX_OK;
 }
 
 /* records one pass into a graph (kernel nodes only: counters and flags are
  * cleared by fill kernels, never by memset nodes) */
 int PhxRenderer::capture_pass() {
     PHX_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "graph capture");
-    int rc = enqueue_pass(false);
+    int rc = enqueue_pass(false, false);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(stream, &g);
     if (rc != PHX_OK) {
@@ -265,17 +370,24 @@ int PhxRenderer::capture_pass() {
 int PhxRenderer::run_pass() {
     int rc;
     bool use_graph = (args.flags & PHX_PASS_GRAPH) != 0;
+    /* the adjoint driver's recording passes stay serial */
+    bool pipe = !use_graph && !(args.flags & PHX_PASS_SERIAL) && !rec_ratio && !sort_perm;
     if (use_graph && !graph_exec && (rc = capture_pass()) != PHX_OK)
+        return rc;
+    if (pipe && (rc = alloc_alt()) != PHX_OK)
         return rc;
     PHX_TRY(hipEventRecord(ev[0], stream), "event");
     if (use_graph) {
         PHX_TRY(hipGraphLaunch(graph_exec, stream), "graph launch");
-    } else if ((rc = enqueue_pass(true)) != PHX_OK) {
+    } else if ((rc = enqueue_pass(true, pipe)) != PHX_OK) {
         return rc;
     }
-    PHX_TRY(hipEventRecord(ev[4], stream), "event");
-    PHX_TRY(hipMemcpyAsync(h_info, d_info.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream), "pass info");
-    PHX_TRY(hipEventRecord(ev_info, stream), "event");
+    if (!pipe) {
+        PHX_TRY(hipEventRecord(ev[4], stream), "event");
+        PHX_TRY(hipMemcpyAsync(h_info, d_info.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
+                "pass info");
+        PHX_TRY(hipEventRecord(ev_info, stream), "event");
+    }
     info_pending = true;
     timed = !use_graph;
     rounds++;
@@ -323,15 +435,18 @@ int phx_renderer_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, Ph
     hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
     if (e == hipSuccess)
         e = hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&r->gstream, hipStreamNonBlocking);
     if (e != hipSuccess)
         return hip_status(e, "phx_renderer_create: stream");
     for (hipEvent_t &ev : r->ev)
         if ((e = hipEventCreate(&ev)) != hipSuccess)
             return hip_status(e, "phx_renderer_create: event");
-    for (hipEvent_t *ev : {&r->ev_emit, &r->ev_gather})
+    for (hipEvent_t *ev : {&r->ev_emit, &r->ev_gather, &r->ev_trace})
         if ((e = hipEventCreate(ev)) != hipSuccess)
             return hip_status(e, "phx_renderer_create: event");
-    for (hipEvent_t *ev : {&r->ev_info, &r->ev_fork, &r->ev_join})
+    for (hipEvent_t *ev : {&r->ev_info, &r->ev_fork, &r->ev_join, &r->ev_trace_done, &r->ev_emit_done,
+                           &r->ev_read[0], &r->ev_read[1]})
         if ((e = hipEventCreateWithFlags(ev, hipEventDisableTiming)) != hipSuccess)
             return hip_status(e, "phx_renderer_create: event");
     if ((e = hipHostMalloc((void **)&r->h_info, 2 * sizeof(uint32_t), hipHostMallocDefault)) != hipSuccess)
@@ -355,7 +470,7 @@ int phx_renderer_run_pass(PhxRenderer *r) {
 int phx_renderer_synchronize(PhxRenderer *r) {
     if (!r)
         return fail(PHX_ERR_INVALID_ARG, "null renderer");
-    PHX_TRY(hipStreamSynchronize(r->stream), "phx_renderer_synchronize");
+    PHX_TRY(r->sync_all(), "phx_renderer_synchronize");
     return r->check_passes();
 }
 
@@ -380,8 +495,8 @@ int phx_renderer_last_stage_ms(PhxRenderer *r, float out_ms[4]) {
     hipError_t e = hipEventSynchronize(r->ev[4]);
     if (e != hipSuccess)
         return hip_status(e, "stage timing");
-    /* the trace and the emission overlap: each is timed on its own stream */
-    const hipEvent_t from[4] = {r->ev[0], r->ev_emit, r->ev[2], r->ev_gather};
+    /* the stages overlap: each is timed on its own stream */
+    const hipEvent_t from[4] = {r->ev_trace, r->ev_emit, r->ev[2], r->ev_gather};
     const hipEvent_t to[4] = {r->ev[1], r->ev[2], r->ev[3], r->ev[4]};
     for (int k = 0; k < 4; k++)
         if ((e = hipEventElapsedTime(&out_ms[k], from[k], to[k])) != hipSuccess)
@@ -400,6 +515,7 @@ uint32_t phx_renderer_rounds_done(PhxRenderer *r) { return r ? r->rounds : 0u; }
 int phx_renderer_read_state(PhxRenderer *r, phx_float4 *flux, float *photon_weight, phx_float4 *radiance) {
     if (!r)
         return fail(PHX_ERR_INVALID_ARG, "null renderer");
+    PHX_TRY(r->sync_all(), "synchronize");
     const ImageBuffers &i = r->buffers.image_buf;
     size_t np = i.n_pixels;
     hipStream_t s = r->stream;
@@ -417,6 +533,7 @@ int phx_renderer_read_state(PhxRenderer *r, phx_float4 *flux, float *photon_weig
 int phx_renderer_read_hitpoints(PhxRenderer *r, phx_float4 *hp6) {
     if (!r || !hp6)
         return fail(PHX_ERR_INVALID_ARG, "null argument");
+    PHX_TRY(r->sync_all(), "synchronize");
     const ImageBuffers &i = r->buffers.image_buf;
     size_t np = i.n_pixels;
     std::vector<phx_float4> tmp(np);
@@ -436,6 +553,7 @@ int phx_renderer_read_photon_map(PhxRenderer *r, phx_float4 *pos, uint32_t *powe
                                  uint32_t *cell_start, uint32_t *cell_end) {
     if (!r)
         return fail(PHX_ERR_INVALID_ARG, "null renderer");
+    PHX_TRY(r->sync_all(), "synchronize");
     (void)r->collect();
     const PhotonBuffers &p = r->buffers.photon_buf;
     size_t n = p.n_photons, nb = r->bucket_count;
@@ -458,6 +576,7 @@ int phx_renderer_read_photon_map(PhxRenderer *r, phx_float4 *pos, uint32_t *powe
 int phx_renderer_device_buffers(PhxRenderer *r, void **ptrs, uint32_t n_ptrs) {
     if (!r || !ptrs)
         return fail(PHX_ERR_INVALID_ARG, "null argument");
+    PHX_TRY(r->sync_all(), "synchronize");
     const ImageBuffers &i = r->buffers.image_buf;
     const PhotonBuffers &p = r->buffers.photon_buf;
     void *all[] = {i.d_hp_pos.p,       i.d_hp_normal.p,     i.d_hp_throughput.p, i.d_hp_base_color.p,
@@ -492,7 +611,7 @@ int phx_renderer_save_checkpoint(PhxRenderer *r, void *buf, size_t bytes) {
         return fail(PHX_ERR_INVALID_ARG, "null argument");
     if (bytes < phx_renderer_checkpoint_bytes(r))
         return fail(PHX_ERR_WORKSPACE, "phx_renderer_save_checkpoint: buffer too small");
-    PHX_TRY(hipStreamSynchronize(r->stream), "checkpoint");
+    PHX_TRY(r->sync_all(), "checkpoint");
     const ImageBuffers &i = r->buffers.image_buf;
     size_t np = i.n_pixels;
     PhxCheckpointHeader h{};
@@ -528,7 +647,7 @@ int phx_renderer_load_checkpoint(PhxRenderer *r, const void *buf, size_t bytes) 
         return fail(PHX_ERR_INVALID_ARG, "phx_renderer_load_checkpoint: not a checkpoint");
     if (h.width != i.image_width || h.height != i.image_height || bytes < phx_renderer_checkpoint_bytes(r))
         return fail(PHX_ERR_INVALID_ARG, "phx_renderer_load_checkpoint: image size differs");
-    PHX_TRY(hipStreamSynchronize(r->stream), "checkpoint");
+    PHX_TRY(r->sync_all(), "checkpoint");
     const char *p = (const char *)buf + sizeof(h);
     PHX_TRY(hipMemcpy(i.d_flux.p, p, np * sizeof(phx_float4), hipMemcpyHostToDevice), "checkpoint");
     p += np * sizeof(phx_float4);

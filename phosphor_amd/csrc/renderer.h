@@ -46,12 +46,27 @@ struct PhxRenderer {
        runs on `stream` (the two are independent until the gather) */
     hipStream_t side = nullptr;
     /* pass timing: ev[0] start, ev[1] trace done, ev[2] emission done,
-       ev[3] photon map done, ev[4] end; ev_emit / ev_gather the starts of
-       the emission and of the gather */
+       ev[3] photon map done, ev[4] end; ev_trace / ev_emit / ev_gather the
+       starts of the trace, the emission and the gather */
     hipEvent_t ev[5] = {};
-    hipEvent_t ev_emit = nullptr, ev_gather = nullptr;
+    hipEvent_t ev_emit = nullptr, ev_gather = nullptr, ev_trace = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr; /* no timing */
     hipEvent_t ev_info = nullptr;
+    /* Pipelined passes (without PHX_PASS_GRAPH or PHX_PASS_SERIAL): a
+       pass's gather runs on `gstream` while the next pass's trace and
+       emission run.  The hitpoints and the sorted photon map alternate
+       between two sets: the one in `buffers` (the last pass's, which readers
+       see) and `alt_hp` / `alt_map` (the one the next pass writes), each with
+       the end of the last gather that read it (ev_read, [0] for the set in
+       `buffers`).  A pass's round seed waits for the previous pass's emission
+       (ev_emit_done), which reads the seed it replaces. */
+    hipStream_t gstream = nullptr;
+    hipEvent_t ev_trace_done = nullptr, ev_emit_done = nullptr, ev_read[2] = {};
+    bool read_pending[2] = {false, false};
+    bool emit_pending = false;
+    bool alt_ready = false;
+    DevBuf alt_hp[6];  /* pos, normal, throughput, base colour, direct, emission */
+    DevBuf alt_map[6]; /* pos, power, dir, dirpow, cell_start, cell_end */
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     uint32_t *h_info = nullptr; /* pinned: last deposit count, passes without deposits */
@@ -77,8 +92,7 @@ struct PhxRenderer {
     uint32_t *sort_perm = nullptr;
 
     ~PhxRenderer() {
-        if (stream)
-            (void)hipStreamSynchronize(stream);
+        (void)sync_all();
         if (graph_exec)
             (void)hipGraphExecDestroy(graph_exec);
         if (graph)
@@ -86,11 +100,14 @@ struct PhxRenderer {
         for (hipEvent_t &e : ev)
             if (e)
                 (void)hipEventDestroy(e);
-        for (hipEvent_t e : {ev_emit, ev_gather, ev_fork, ev_join, ev_info})
+        for (hipEvent_t e : {ev_emit, ev_gather, ev_trace, ev_fork, ev_join, ev_info, ev_trace_done, ev_emit_done, ev_read[0],
+                             ev_read[1]})
             if (e)
                 (void)hipEventDestroy(e);
         if (h_info)
             (void)hipHostFree(h_info);
+        if (gstream)
+            (void)hipStreamDestroy(gstream);
         if (side)
             (void)hipStreamDestroy(side);
         if (stream)
@@ -100,16 +117,23 @@ struct PhxRenderer {
     uint32_t *seed_ptr() const { return d_rng.as<uint32_t>() + 1; }
     int set_scene(const PhxSceneDesc &scene);
     int run_pass();
-    int enqueue_pass(bool events);
+    int enqueue_pass(bool events, bool pipe);
+    int alloc_alt();
+    void hp_bufs(int set, DevBuf *out[6]);
+    void map_bufs(int set, DevBuf *out[6]);
+    void swap_sets();
+    hipError_t sync_all();
     int capture_pass();
     int collect();
     int check_passes();
-    int trace_rays();
+    /* set 0: the buffers in `buffers`; set 1: alt_hp / alt_map */
+    int trace_rays(int set = 0);
     int emit_photons() { return emit_photons(stream); }
     int emit_photons(hipStream_t s);
-    int hash_and_sort_photons() { return hash_and_sort_photons(stream); }
-    int hash_and_sort_photons(hipStream_t s);
-    int gather_flux();
+    int hash_and_sort_photons() { return hash_and_sort_photons(stream, 0); }
+    int hash_and_sort_photons(hipStream_t s, int set);
+    int gather_flux() { return gather_flux(stream, 0); }
+    int gather_flux(hipStream_t s, int set);
 };
 
 

@@ -32,6 +32,10 @@ struct DevBuf {
         return hipMalloc(&p, bytes);
     }
     template <class T> T *as() const { return (T *)p; }
+    void swap(DevBuf &o) {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+    }
 };
 
 /* rendering.cpp:3-13: zero items still allocate one item */

@@ -284,21 +284,25 @@ def test_textures_media_and_light_kinds_bit_exact(leaf):
     gpu.close()
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_back_to_back_passes_bit_exact(graph):
-    """Passes enqueued with no host synchronisation between them (the camera
-    trace of a pass and its emission run on two streams, joined before the
-    gather; the next pass's emission starts after that gather): the state
-    after three passes is the oracle's."""
+@pytest.mark.parametrize("mode", ["pipelined", "serial", "graph"])
+def test_back_to_back_passes_bit_exact(mode):
+    """Passes enqueued with no host synchronisation between them.  Serial and
+    graph passes: the camera trace and the emission run on two streams, joined
+    before the gather, and the next pass starts after that gather.  Pipelined
+    passes (the default): each gather runs on a third stream beside the next
+    pass's trace and emission, on alternating hitpoint / photon-map sets.  The
+    state after five passes, and again after two more, is the oracle's."""
     scene = scenes.caustic_scene(64, 48, level=3)
     cfg = PassConfig(width=64, height=48, photons_per_round=1 << 14, max_photons_per_batch=1 << 14,
-                     max_photon_bounces=12, grid_res=60, seed=13, initial_radius=0.015, graph=graph)
-    print(f"case back-to-back, graph {graph}", flush=True)
+                     max_photon_bounces=12, grid_res=60, seed=13, initial_radius=0.015, graph=mode == "graph",
+                     serial=mode == "serial")
+    print(f"case back-to-back, {mode}", flush=True)
     gpu = Renderer(cfg, scene)
     ref = oracle.RefRenderer(cfg, scene)
-    gpu.run(3)
-    gpu.synchronize()
-    for _ in range(3):
-        assert ref.run_pass() == 0
-    _compare_pass(gpu, ref, f"back-to-back graph={graph}")
+    for rounds in (5, 2):
+        gpu.run(rounds)
+        gpu.synchronize()
+        for _ in range(rounds):
+            assert ref.run_pass() == 0
+        _compare_pass(gpu, ref, f"back-to-back {mode}")
     gpu.close()
