@@ -22,6 +22,7 @@ Prints one JSON line.  With --gpus > 1 it exits: this build measures one GPU.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import platform
@@ -117,7 +118,6 @@ def noise_rmse(P, cfg, scene, passes: int, ref_passes: int):
     """Relative RMSE of the linear estimate after `passes` passes against an
     independent render (seed + 1) of `ref_passes` passes: the estimator's
     noise at that pass count, not an error of the implementation."""
-    import dataclasses
 
     from phosphor_amd.output import relative_rmse
 
@@ -133,9 +133,11 @@ def noise_rmse(P, cfg, scene, passes: int, ref_passes: int):
 
 
 def bench_batch(args) -> int:
-    """64 scenes × `steps` passes as one batch (graph replay), then the same
-    scenes rendered one by one (graph replay each); both timed host-side
-    between stream synchronisations, scene upload excluded."""
+    """64 scenes × `steps` passes as one batch, then the same scenes rendered
+    one by one, each way both with graph replay and with pipelined passes
+    (a pass's gather beside the next pass's trace and emission); all timed
+    host-side between stream synchronisations, scene upload excluded.  The
+    batch's value and the one-by-one baseline are each the faster mode."""
     import phosphor_amd as P
     from phosphor_amd import scenes
 
@@ -143,41 +145,47 @@ def bench_batch(args) -> int:
     work = scenes.batch_workload(64, seed=1)
     cfgs = [c for c, _ in work]
     scs = [s for _, s in work]
-    for c in cfgs:
-        c.graph = True
     n_px = sum(c.width * c.height for c in cfgs)
     n_ph = sum(c.photons_per_round for c in cfgs)
     print(f"[bench] batch64: {sum(s.n_triangles for s in scs)} triangles, {n_px} pixels, {n_ph} photons per pass "
           f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    b = P.Batch(cfgs, scs, graph=True)
-    print(f"[bench] batch created in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        b.run_pass()
-    b.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        b.run_pass()
-    b.synchronize()
-    dt_batch = time.perf_counter() - t0
-    last_ms = b.pass_ms()
-    b.close()
-    print(f"[bench] batch: {dt_batch * 1e3 / args.steps:.1f} ms/pass (last {last_ms:.1f} ms)", file=sys.stderr,
-          flush=True)
-    dt_plain = 0.0
-    for i, (c, s) in enumerate(work):
-        r = P.Renderer(c, s)
+    batch_s = {}
+    for mode in ("graph", "pipelined"):
+        t0 = time.perf_counter()
+        b = P.Batch(cfgs, scs, graph=mode == "graph")
+        print(f"[bench] batch ({mode}) created in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
         for _ in range(args.warmup):
-            r.run_pass()
-        r.synchronize()
+            b.run_pass()
+        b.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            r.run_pass()
-        r.synchronize()
-        dt_plain += time.perf_counter() - t0
-        r.close()
-        if i % 16 == 15:
-            print(f"[bench] one-by-one: {i + 1} scenes, {dt_plain:.2f} s so far", file=sys.stderr, flush=True)
+            b.run_pass()
+        b.synchronize()
+        batch_s[mode] = time.perf_counter() - t0
+        b.close()
+        print(f"[bench] batch ({mode}): {batch_s[mode] * 1e3 / args.steps:.1f} ms/pass", file=sys.stderr, flush=True)
+    plain_s = {}
+    for mode in ("graph", "pipelined"):
+        dt = 0.0
+        for i, (c, s) in enumerate(work):
+            r = P.Renderer(dataclasses.replace(c, graph=mode == "graph"), s)
+            for _ in range(args.warmup):
+                r.run_pass()
+            r.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                r.run_pass()
+            r.synchronize()
+            dt += time.perf_counter() - t0
+            r.close()
+            if i % 16 == 15:
+                print(f"[bench] one-by-one ({mode}): {i + 1} scenes, {dt:.2f} s so far", file=sys.stderr,
+                      flush=True)
+        plain_s[mode] = dt
+    batch_mode = min(batch_s, key=batch_s.get)
+    plain_mode = min(plain_s, key=plain_s.get)
+    dt_batch = batch_s[batch_mode]
+    dt_plain = plain_s[plain_mode]
     value = 64 * args.steps / dt_batch
     line = {
         "metric": "batched SPPM scene-passes/s",
@@ -195,8 +203,11 @@ def bench_batch(args) -> int:
         "config": {"workload": "batch64", "scenes": 64, "triangles": sum(s.n_triangles for s in scs),
                    "pixels_per_pass": n_px, "photons_per_pass": n_ph, "parallelism": "single-gpu"},
         "photons_rays_per_s": (n_px + n_ph) * args.steps / dt_batch,
+        "batch_mode": batch_mode,
         "one_by_one": {"value": 64 * args.steps / dt_plain, "unit": "scene-passes/s",
-                       "ms_per_step": dt_plain * 1e3 / args.steps},
+                       "ms_per_step": dt_plain * 1e3 / args.steps, "mode": plain_mode},
+        "modes_ms_per_step": {**{f"batch_{k}": v * 1e3 / args.steps for k, v in batch_s.items()},
+                              **{f"one_by_one_{k}": v * 1e3 / args.steps for k, v in plain_s.items()}},
         "speedup_vs_one_by_one": dt_plain / dt_batch,
     }
     print(json.dumps(line))

@@ -445,9 +445,9 @@ int phx_renderer_estimate(PhxRenderer *r, float *rgb_out);
  * with the same arguments. */
 typedef struct PhxBatch PhxBatch;
 typedef struct PhxBatchOptions {
-    uint32_t trace_group_pixels; /* pixels traced per group (0 → 2 Mi) */
-    uint32_t emit_group_photons; /* photons emitted per group (0 → 8 Mi) */
-    uint32_t flags;              /* PHX_PASS_GRAPH | PHX_REFERENCE_BVH */
+    uint32_t trace_group_pixels; /* pixels traced per group (0 → 1 Mi) */
+    uint32_t emit_group_photons; /* photons emitted per group (0 → 2 Mi) */
+    uint32_t flags;              /* PHX_PASS_GRAPH | PHX_PASS_SERIAL | PHX_REFERENCE_BVH */
     uint32_t reserved;
 } PhxBatchOptions;
 

@@ -211,7 +211,7 @@ class Batch:
     each scene's results bit-identical to a ``Renderer`` with its config."""
 
     def __init__(self, configs: list[PassConfig], scenes: list, trace_group_pixels: int = 0,
-                 emit_group_photons: int = 0, graph: bool = True):
+                 emit_group_photons: int = 0, graph: bool = True, serial: bool = False):
         if len(configs) != len(scenes) or not scenes:
             raise ValueError("one PassConfig per scene, at least one scene")
         self.configs = list(configs)
@@ -220,7 +220,8 @@ class Batch:
         self._args = (RenderArgs * n)(*(c.to_c() for c in configs))
         self._descs = [sc.desc() for sc in scenes]  # keeps the host arrays referenced
         self._desc_arr = (abi.SceneDesc * n)(*self._descs)
-        opt = abi.BatchOptions(trace_group_pixels, emit_group_photons, abi.PHX_PASS_GRAPH if graph else 0, 0)
+        flags = (abi.PHX_PASS_GRAPH if graph else 0) | (abi.PHX_PASS_SERIAL if serial else 0)
+        opt = abi.BatchOptions(trace_group_pixels, emit_group_photons, flags, 0)
         h = C.c_void_p()
         check(lib().phx_batch_create(C.cast(self._args, C.c_void_p), C.cast(self._desc_arr, C.c_void_p), n,
                                      C.byref(opt), C.byref(h)), "batch_create")

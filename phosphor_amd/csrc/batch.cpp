@@ -34,6 +34,20 @@ struct PhxBatch {
     hipStream_t side = nullptr; /* emission and photon map, beside the camera trace */
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_info = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    /* Pipelined passes (without PHX_PASS_GRAPH or PHX_PASS_SERIAL), as for
+       one scene: the gather on `gstream` beside the next pass's trace and
+       emission; the hitpoints and the photon map alternate between the set
+       in d_hp / d_map_* (the last pass's; `plan` points at it) and the alt
+       set (`plan_alt`), swapped after each pass; ev_read[k] ends the last
+       gather that read set k, ev_emit_done the last emission (which reads
+       the round seeds). */
+    hipStream_t gstream = nullptr;
+    hipEvent_t ev_trace_done = nullptr, ev_emit_done = nullptr, ev_read[2] = {};
+    bool read_pending[2] = {false, false};
+    bool emit_pending = false;
+    bool alt_ready = false;
+    DevBuf alt_hp[6], alt_map[6]; /* map: pos, power, dir, dirpow, cell_start, cell_end */
+    BatchPlan plan_alt;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<std::unique_ptr<SceneBuffers>> scenes;
@@ -51,17 +65,18 @@ struct PhxBatch {
     bool info_pending = false;
 
     ~PhxBatch() {
-        if (stream)
-            (void)hipStreamSynchronize(stream);
+        (void)sync_all();
         if (graph_exec)
             (void)hipGraphExecDestroy(graph_exec);
         if (graph)
             (void)hipGraphDestroy(graph);
-        for (hipEvent_t e : {ev0, ev1, ev_info, ev_fork, ev_join})
+        for (hipEvent_t e : {ev0, ev1, ev_info, ev_fork, ev_join, ev_trace_done, ev_emit_done, ev_read[0], ev_read[1]})
             if (e)
                 (void)hipEventDestroy(e);
         if (h_info)
             (void)hipHostFree(h_info);
+        if (gstream)
+            (void)hipStreamDestroy(gstream);
         if (side)
             (void)hipStreamDestroy(side);
         if (stream)
@@ -71,7 +86,133 @@ struct PhxBatch {
     uint32_t n() const { return (uint32_t)args.size(); }
     int build(const PhxSceneDesc *descs);
     int collect();
+    int run_pass();
+    int alloc_alt();
+    void map_bufs(DevBuf *out[6]) {
+        DevBuf *m[6] = {&d_map_pos, &d_map_power, &d_map_dir, &d_map_dirpow, &d_cell_start, &d_cell_end};
+        for (int k = 0; k < 6; k++)
+            out[k] = m[k];
+    }
+    /* waits for every stream of the batch */
+    hipError_t sync_all() {
+        for (hipStream_t q : {stream, side, gstream}) {
+            hipError_t e = q ? hipStreamSynchronize(q) : hipSuccess;
+            if (e != hipSuccess)
+                return e;
+        }
+        read_pending[0] = read_pending[1] = false;
+        emit_pending = false;
+        return hipSuccess;
+    }
 };
+
+/* the second set of hitpoint and photon-map buffers (cleared), and the plan
+ * that writes it */
+int PhxBatch::alloc_alt() {
+    if (alt_ready)
+        return PHX_OK;
+    DevBuf *m[6];
+    map_bufs(m);
+    for (int k = 0; k < 6; k++) {
+        PHX_TRY(alt_hp[k].alloc(d_hp[k].bytes), "batch second hitpoint set");
+        PHX_TRY(hipMemsetAsync(alt_hp[k].p, 0, alt_hp[k].bytes, stream), "batch second hitpoint set");
+        PHX_TRY(alt_map[k].alloc(m[k]->bytes), "batch second photon map");
+        PHX_TRY(hipMemsetAsync(alt_map[k].p, 0, alt_map[k].bytes, stream), "batch second photon map");
+    }
+    plan_alt = plan;
+    for (int k = 0; k < 6; k++)
+        plan_alt.hp[k] = alt_hp[k].as<phx_float4>();
+    plan_alt.map_pos = alt_map[0].as<phx_float4>();
+    plan_alt.map_power = alt_map[1].as<uint32_t>();
+    plan_alt.map_dir = alt_map[2].as<uint32_t>();
+    plan_alt.map_dirpow = alt_map[3].as<phx_float4>();
+    plan_alt.cell_start = alt_map[4].as<uint32_t>();
+    plan_alt.cell_end = alt_map[5].as<uint32_t>();
+    alt_ready = true;
+    return PHX_OK;
+}
+
+/* one pass: serial (graph replay, PHX_PASS_SERIAL) or pipelined.  Ordering
+ * of a pipelined pass: the round seeds wait for the previous emission; the
+ * trace and the photon map, which overwrite the set the gather before last
+ * read, wait for that gather (on `stream`, before the seeds; `side` follows
+ * `stream` from the seeds on); the gather waits for the trace and the photon
+ * map, and gathers run in order on `gstream`.  A serial pass first waits for
+ * every pending gather. */
+int PhxBatch::run_pass() {
+    bool use_graph = (opt.flags & PHX_PASS_GRAPH) != 0;
+    bool pipe = !use_graph && !(opt.flags & PHX_PASS_SERIAL);
+    BatchStreams B;
+    B.s = stream;
+    B.side = side;
+    B.ev_fork = ev_fork;
+    B.ev_join = ev_join;
+    if (use_graph && !graph_exec) {
+        PHX_TRY(sync_all(), "batch graph capture");
+        PHX_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "batch graph capture");
+        hipError_t le = launch_batch_pass(plan, B);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(stream, &g);
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return hip_status(le != hipSuccess ? le : ce, "batch graph capture");
+        }
+        graph = g;
+        PHX_TRY(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0), "batch graph instantiate");
+    }
+    if (pipe) {
+        int rc = alloc_alt();
+        if (rc != PHX_OK)
+            return rc;
+    }
+    if (emit_pending) {
+        PHX_TRY(hipStreamWaitEvent(stream, ev_emit_done, 0), "event");
+        emit_pending = false;
+    }
+    for (int k = pipe ? 1 : 0; k < 2; k++)
+        if (read_pending[k]) {
+            PHX_TRY(hipStreamWaitEvent(stream, ev_read[k], 0), "event");
+            read_pending[k] = false;
+        }
+    PHX_TRY(hipEventRecord(ev0, stream), "event");
+    size_t S = n();
+    if (use_graph) {
+        PHX_TRY(hipGraphLaunch(graph_exec, stream), "batch graph launch");
+    } else if (!pipe) {
+        PHX_TRY(launch_batch_pass(plan, B), "batch pass");
+    } else {
+        B.gather = gstream;
+        B.ev_trace = ev_trace_done;
+        B.ev_emit = ev_emit_done;
+        B.h_info = h_info;
+        PHX_TRY(launch_batch_pass(plan_alt, B), "batch pass");
+        PHX_TRY(hipEventRecord(ev_info, side), "event");
+        PHX_TRY(hipEventRecord(ev1, gstream), "event");
+        PHX_TRY(hipEventRecord(ev_read[1], gstream), "event");
+        read_pending[1] = true;
+        emit_pending = true;
+        /* the set just written becomes the one in d_hp / d_map_* */
+        DevBuf *m[6];
+        map_bufs(m);
+        for (int k = 0; k < 6; k++) {
+            d_hp[k].swap(alt_hp[k]);
+            m[k]->swap(alt_map[k]);
+        }
+        std::swap(plan, plan_alt);
+        std::swap(ev_read[0], ev_read[1]);
+        std::swap(read_pending[0], read_pending[1]);
+    }
+    if (!pipe) {
+        PHX_TRY(hipEventRecord(ev1, stream), "event");
+        PHX_TRY(hipMemcpyAsync(h_info, d_counts.p, S * 4, hipMemcpyDeviceToHost, stream), "batch info");
+        PHX_TRY(hipMemcpyAsync(h_info + S, d_zero.p, S * 4, hipMemcpyDeviceToHost, stream), "batch info");
+        PHX_TRY(hipEventRecord(ev_info, stream), "event");
+    }
+    info_pending = true;
+    rounds++;
+    return PHX_OK;
+}
 
 /* groups of whole scenes with at most `budget` items each (a scene larger
  * than the budget forms its own group) */
@@ -157,8 +298,10 @@ int PhxBatch::build(const PhxSceneDesc *descs) {
         return fail(PHX_ERR_TOO_LARGE, "phx_batch_create: batch totals exceed 32-bit indices");
     ph_base[S] = (uint32_t)photons;
 
-    uint64_t trace_budget = opt.trace_group_pixels ? opt.trace_group_pixels : (1u << 21);
-    uint64_t emit_budget = opt.emit_group_photons ? opt.emit_group_photons : (1u << 23);
+    /* batch64 workload (64 scenes, 34.7 M pixels, 33.5 M photons per pass):
+       1 Mi / 2 Mi 245 ms per pass, 2 Mi / 8 Mi 257 ms, 8 Mi / 16 Mi 270 ms */
+    uint64_t trace_budget = opt.trace_group_pixels ? opt.trace_group_pixels : (1u << 20);
+    uint64_t emit_budget = opt.emit_group_photons ? opt.emit_group_photons : (1u << 21);
     plan.trace_groups = make_groups(px_items, px_first, chunk_first, chunk_count, trace_budget);
     plan.emit_groups = make_groups(ph_items, ph_first, {}, {}, emit_budget);
     uint32_t max_tp = 0, max_ep = 0;
@@ -276,7 +419,7 @@ int phx_batch_create(const PhxRenderArgs *args, const PhxSceneDesc *scenes, uint
             return fail(PHX_ERR_INVALID_ARG, "phx_batch_create: scene " + std::to_string(i) + ": " +
                                                  std::string(phx_last_error()));
     }
-    if (options && (options->flags & ~(uint32_t)(PHX_PASS_GRAPH | PHX_REFERENCE_BVH)))
+    if (options && (options->flags & ~(uint32_t)(PHX_PASS_GRAPH | PHX_REFERENCE_BVH | PHX_PASS_SERIAL)))
         return fail(PHX_ERR_INVALID_ARG, "phx_batch_create: unknown flags");
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
@@ -287,6 +430,9 @@ int phx_batch_create(const PhxRenderArgs *args, const PhxSceneDesc *scenes, uint
         b->opt = *options;
     PHX_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "phx_batch_create: stream");
     PHX_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "phx_batch_create: stream");
+    PHX_TRY(hipStreamCreateWithFlags(&b->gstream, hipStreamNonBlocking), "phx_batch_create: stream");
+    for (hipEvent_t *e : {&b->ev_trace_done, &b->ev_emit_done, &b->ev_read[0], &b->ev_read[1]})
+        PHX_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming), "phx_batch_create: event");
     PHX_TRY(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming), "phx_batch_create: event");
     PHX_TRY(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming), "phx_batch_create: event");
     PHX_TRY(hipEventCreate(&b->ev0), "phx_batch_create: event");
@@ -307,39 +453,13 @@ void phx_batch_destroy(PhxBatch *b) { delete b; }
 int phx_batch_run_pass(PhxBatch *b) {
     if (!b)
         return fail(PHX_ERR_INVALID_ARG, "phx_batch_run_pass: null batch");
-    bool use_graph = (b->opt.flags & PHX_PASS_GRAPH) != 0;
-    if (use_graph && !b->graph_exec) {
-        PHX_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "batch graph capture");
-        hipError_t le = launch_batch_pass(b->plan, b->stream, b->side, b->ev_fork, b->ev_join);
-        hipGraph_t g = nullptr;
-        hipError_t ce = hipStreamEndCapture(b->stream, &g);
-        if (le != hipSuccess || ce != hipSuccess) {
-            if (g)
-                (void)hipGraphDestroy(g);
-            return hip_status(le != hipSuccess ? le : ce, "batch graph capture");
-        }
-        b->graph = g;
-        PHX_TRY(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0), "batch graph instantiate");
-    }
-    PHX_TRY(hipEventRecord(b->ev0, b->stream), "event");
-    if (use_graph)
-        PHX_TRY(hipGraphLaunch(b->graph_exec, b->stream), "batch graph launch");
-    else
-        PHX_TRY(launch_batch_pass(b->plan, b->stream, b->side, b->ev_fork, b->ev_join), "batch pass");
-    PHX_TRY(hipEventRecord(b->ev1, b->stream), "event");
-    size_t S = b->n();
-    PHX_TRY(hipMemcpyAsync(b->h_info, b->d_counts.p, S * 4, hipMemcpyDeviceToHost, b->stream), "batch info");
-    PHX_TRY(hipMemcpyAsync(b->h_info + S, b->d_zero.p, S * 4, hipMemcpyDeviceToHost, b->stream), "batch info");
-    PHX_TRY(hipEventRecord(b->ev_info, b->stream), "event");
-    b->info_pending = true;
-    b->rounds++;
-    return PHX_OK;
+    return b->run_pass();
 }
 
 int phx_batch_synchronize(PhxBatch *b) {
     if (!b)
         return fail(PHX_ERR_INVALID_ARG, "null batch");
-    PHX_TRY(hipStreamSynchronize(b->stream), "phx_batch_synchronize");
+    PHX_TRY(b->sync_all(), "phx_batch_synchronize");
     int rc = b->collect();
     if (rc != PHX_OK)
         return rc;
@@ -388,6 +508,7 @@ int phx_batch_read_state(PhxBatch *b, uint32_t scene, phx_float4 *flux, float *p
     int rc = batch_scene(b, scene, "phx_batch_read_state");
     if (rc != PHX_OK)
         return rc;
+    PHX_TRY(b->sync_all(), "read");
     const BatchScene &B = b->table[scene];
     hipStream_t s = b->stream;
     size_t np = B.n_pixels, off = B.px_base;
@@ -408,6 +529,7 @@ int phx_batch_read_hitpoints(PhxBatch *b, uint32_t scene, phx_float4 *hp6) {
         return rc;
     if (!hp6)
         return fail(PHX_ERR_INVALID_ARG, "null argument");
+    PHX_TRY(b->sync_all(), "read");
     const BatchScene &B = b->table[scene];
     size_t np = B.n_pixels;
     std::vector<phx_float4> tmp(np);
@@ -429,6 +551,7 @@ int phx_batch_read_photon_map(PhxBatch *b, uint32_t scene, phx_float4 *pos, uint
     int rc = batch_scene(b, scene, "phx_batch_read_photon_map");
     if (rc != PHX_OK)
         return rc;
+    PHX_TRY(b->sync_all(), "read");
     const BatchScene &B = b->table[scene];
     hipStream_t s = b->stream;
     uint32_t base[2];

@@ -179,12 +179,20 @@ size_t batch_trace_workspace_bytes(uint32_t max_pixels, uint32_t max_direct_samp
 size_t batch_emit_workspace_bytes(uint32_t max_photons, uint32_t max_bounces);
 /* persistent camera-kernel blocks that fit the device at once */
 uint32_t batch_camera_blocks();
-/* one pass over every scene of the plan (kernel launches only) */
-/* one batched pass: the camera trace on s, the emission and the photon map
- * on `side` (forked after the round seeds with ev_fork, joined before the
- * gather with ev_join) */
-hipError_t launch_batch_pass(const BatchPlan &P, hipStream_t s, hipStream_t side, hipEvent_t ev_fork,
-                             hipEvent_t ev_join);
+/* the streams of one batched pass: the camera trace on `s`, the emission and
+ * the photon map on `side` (forked after the round seeds with ev_fork, joined
+ * before the gather with ev_join); the gather on `s`, or, when `gather` is
+ * set, on that stream after the trace (ev_trace).  ev_emit, when set, is
+ * recorded on `side` after the emission, and h_info, when set, receives the
+ * per-scene deposit counts and passes without deposits there. */
+struct BatchStreams {
+    hipStream_t s = nullptr, side = nullptr, gather = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_trace = nullptr, ev_emit = nullptr;
+    uint32_t *h_info = nullptr;
+};
+/* one batched pass over every scene of the plan (launches and event
+ * records only) */
+hipError_t launch_batch_pass(const BatchPlan &P, const BatchStreams &B);
 
 /* ---- adjoint ---- */
 /* what the gradient needs per material (host-filled, device table) */

@@ -313,9 +313,10 @@ uint32_t batch_camera_blocks() {
     } while (0)
 #define BLAUNCH() BTRY(hipGetLastError())
 
-hipError_t launch_batch_pass(const BatchPlan &P, hipStream_t s, hipStream_t side, hipEvent_t ev_fork,
-                             hipEvent_t ev_join) {
+hipError_t launch_batch_pass(const BatchPlan &P, const BatchStreams &B) {
     u32 S = P.n_scenes;
+    hipStream_t s = B.s, side = B.side;
+    hipEvent_t ev_fork = B.ev_fork, ev_join = B.ev_join;
     BTRY(launch_next_round_seed(P.rng, P.round_seed, S, s));
     BTRY(launch_fill_u32(P.tickets, 0u, P.trace_groups.size(), s));
     /* the emission and the photon map need only the round seeds: they run on
@@ -359,6 +360,12 @@ hipError_t launch_batch_pass(const BatchPlan &P, hipStream_t s, hipStream_t side
         }
         hipLaunchKernelGGL(k_bdep_end, dim3(1), dim3(256), 0, s, S, P.dep_counter, P.dep_base, P.counts, P.zero_passes);
         BLAUNCH();
+        if (B.h_info) { /* the pass's counts, before the next emission overwrites them */
+            BTRY(hipMemcpyAsync(B.h_info, P.counts, S * sizeof(u32), hipMemcpyDeviceToHost, s));
+            BTRY(hipMemcpyAsync(B.h_info + S, P.zero_passes, S * sizeof(u32), hipMemcpyDeviceToHost, s));
+        }
+        if (B.ev_emit)
+            BTRY(hipEventRecord(B.ev_emit, s));
 
         /* hash, scan, stable sort over every scene's buckets */
         BTRY(launch_fill_u32(P.cell_counts, 0u, P.total_buckets, s));
@@ -403,6 +410,11 @@ hipError_t launch_batch_pass(const BatchPlan &P, hipStream_t s, hipStream_t side
         BLAUNCH();
     }
 
+    if (B.gather) {
+        BTRY(hipEventRecord(B.ev_trace, s));
+        BTRY(hipStreamWaitEvent(B.gather, B.ev_trace, 0));
+        s = B.gather;
+    }
     BTRY(hipStreamWaitEvent(s, ev_join, 0));
 
     /* gather_flux over every tile of every scene */

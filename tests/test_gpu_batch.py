@@ -38,11 +38,12 @@ def _cases():
     return out
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_batch_matches_single_scene_passes(graph):
+@pytest.mark.parametrize("mode", ["graph", "pipelined", "serial"])
+def test_batch_matches_single_scene_passes(mode):
     cases = _cases()
+    print(f"case batch {mode}", flush=True)
     batch = Batch([c for _, c in cases], [s for s, _ in cases], trace_group_pixels=2500,
-                  emit_group_photons=7000, graph=graph)
+                  emit_group_photons=7000, graph=mode == "graph", serial=mode == "serial")
     singles = [Renderer(c, s) for s, c in cases]
     for p in range(3):
         batch.run_pass()
@@ -67,3 +68,31 @@ def test_batch_matches_single_scene_passes(graph):
     assert batch.pass_ms() > 0.0
     batch.close()
 
+
+@pytest.mark.parametrize("mode", ["graph", "pipelined"])
+def test_batch_back_to_back_passes(mode):
+    """Batched passes enqueued with no host synchronisation between them
+    (pipelined: each gather beside the next pass's trace and emission, on
+    alternating hitpoint / photon-map sets): after four passes every scene's
+    hitpoints, photon map and state are those of its own renderer."""
+    cases = _cases()
+    print(f"case batch back-to-back {mode}", flush=True)
+    batch = Batch([c for _, c in cases], [s for s, _ in cases], trace_group_pixels=2500,
+                  emit_group_photons=7000, graph=mode == "graph")
+    for _ in range(4):
+        batch.run_pass()
+    batch.synchronize()
+    counts = batch.photon_counts()
+    for i, (s, c) in enumerate(cases):
+        r = Renderer(c, s)
+        r.run(4)
+        tag = f"scene {i}"
+        assert counts[i] == r.last_photon_count, tag
+        assert np.array_equal(_bits(batch.read_hitpoints(i)), _bits(r.read_hitpoints())), f"{tag}: hitpoints"
+        for a, b, name in zip(batch.read_photon_map(i), r.read_photon_map(), ("pos", "power", "dir", "cs", "ce")):
+            assert np.array_equal(_bits(a), _bits(b)), f"{tag}: photon map {name} differs"
+        for a, b, name in zip(batch.read_state(i), r.read_state(), ("flux", "weight", "radiance")):
+            assert np.array_equal(_bits(a), _bits(b)), f"{tag}: {name} differs"
+        r.close()
+    assert batch.rounds_done == 4
+    batch.close()
