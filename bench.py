@@ -317,8 +317,8 @@ def main() -> int:
         return copy_calibration(args.copy_calibration)
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the pass is single-GPU; sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(one process drives one GPU; a multi-GPU run would split the photons of a pass across GPUs)", file=sys.stderr)
         return 2
 
     if args.workload == "batch64":

@@ -1,5 +1,5 @@
 """The autotuner on a small scene: every exact knob keeps the SPPM state
-bit-identical, grid candidates are judged by relative RMSE, and the chosen
+bit-identical, grid candidates are compared by relative RMSE, and the chosen
 configuration is one of the accepted trials."""
 import pytest
 
