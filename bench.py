@@ -17,7 +17,7 @@ to 40 material parameters, reported as gradient evaluations per second.
 reported as scene-passes/s (scenes × passes / s) beside the same scenes
 rendered one after another with the one-scene driver.
 
-Prints one JSON line.  With --gpus > 1 it exits: this build measures one GPU.
+Prints one JSON line.  With --gpus > 1 it exits: the pass runs on one GPU.
 """
 from __future__ import annotations
 
