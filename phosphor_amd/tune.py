@@ -26,6 +26,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import time
 
 import numpy as np
 
@@ -34,17 +35,22 @@ from .output import relative_rmse
 
 
 def _run(cfg: PassConfig, scene, warmup: int, passes: int):
+    """Milliseconds per pass of `passes` passes enqueued back to back (as a
+    render runs them: pipelined unless the config says otherwise), after
+    `warmup` passes; the state and the estimate at the end."""
     r = Renderer(cfg, scene)
-    times = []
-    for i in range(warmup + passes):
+    for _ in range(warmup):
         r.run_pass()
-        r.synchronize()
-        if i >= warmup:
-            times.append(r.pass_ms())
+    r.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        r.run_pass()
+    r.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / max(passes, 1)
     flux, weight, rad = r.read_state()
     est = r.estimate()
     r.close()
-    return float(np.median(times)), (flux, weight, rad), est
+    return ms, (flux, weight, rad), est
 
 
 def _same_state(a, b) -> bool:
@@ -116,7 +122,7 @@ def main(argv=None) -> int:
 
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("--workload", default="caustic")
-    ap.add_argument("--passes", type=int, default=4)
+    ap.add_argument("--passes", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
     args = ap.parse_args(argv)
     cfg, scene, _ = config_for(args.workload)
