@@ -226,3 +226,30 @@ def test_batch_rejects_bad_arguments(phx_lib):
     opt = abi.BatchOptions(0, 0, 0x80, 0)
     assert phx_lib.phx_batch_create(C.cast(args, C.c_void_p), C.cast(descs, C.c_void_p), 1, C.byref(opt),
                                     C.byref(h)) == abi.PHX_ERR_INVALID_ARG
+
+
+def test_renderer_flags_checked_before_the_device(phx_lib):
+    """PHX_PASS_GRAPH, PHX_REFERENCE_BVH and PHX_PASS_SERIAL pass the argument
+    check (creation then stops at the missing device); any other bit is
+    refused first."""
+    import os
+
+    from phosphor_amd import PassConfig, scenes
+
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    sc = scenes.cornell_box(8, 8)
+    desc = sc.desc()
+    h = C.c_void_p()
+    for flags, want in ((abi.PHX_PASS_SERIAL, abi.PHX_ERR_NO_DEVICE),
+                        (abi.PHX_PASS_GRAPH | abi.PHX_REFERENCE_BVH | abi.PHX_PASS_SERIAL, abi.PHX_ERR_NO_DEVICE),
+                        (0x10, abi.PHX_ERR_INVALID_ARG)):
+        a = PassConfig(width=8, height=8, photons_per_round=16, grid_res=8).to_c()
+        a.flags = flags
+        assert phx_lib.phx_renderer_create(C.byref(a), C.byref(desc), C.byref(h)) == want, flags
+        assert not h.value
+    opt = abi.BatchOptions(0, 0, abi.PHX_PASS_SERIAL, 0)
+    args = (abi.RenderArgs * 1)(PassConfig(width=8, height=8, photons_per_round=16, grid_res=8).to_c())
+    descs = (abi.SceneDesc * 1)(desc)
+    assert phx_lib.phx_batch_create(C.cast(args, C.c_void_p), C.cast(descs, C.c_void_p), 1, C.byref(opt),
+                                    C.byref(h)) == abi.PHX_ERR_NO_DEVICE
