@@ -130,7 +130,7 @@ int PhxRenderer::set_scene(const PhxSceneDesc &sc) {
                 "alloc trace workspace");
 
     /* renderer.cpp:54: the round-seed generator, kept on the device */
-    uint32_t rng0[2] = {host_pcg_seed(a.seed), 0u};
+    uint32_t rng0[3] = {host_pcg_seed(a.seed), 0u, 0u};
     PHX_TRY(d_rng.alloc(sizeof(rng0)), "alloc seeds");
     PHX_TRY(hipMemcpyAsync(d_rng.p, rng0, sizeof(rng0), hipMemcpyHostToDevice, s), "upload seeds");
     PHX_TRY(d_info.alloc(2 * sizeof(uint32_t)), "alloc pass info");
@@ -272,7 +272,7 @@ hipError_t PhxRenderer::sync_all() {
             return e;
     }
     read_pending[0] = read_pending[1] = false;
-    emit_pending = false;
+    emit_pending[0] = emit_pending[1] = false;
     return hipSuccess;
 }
 
@@ -284,7 +284,9 @@ hipError_t PhxRenderer::sync_all() {
  * Pipelined form (`pipe`): the same, with the trace and the photon map
  * written into the second set and the gather on `gstream`, so that the next
  * pass's seed, trace and emission do not wait for it.  Ordering:
- *   - the round seed waits for the previous emission (which reads the seed);
+ *   - pipelined passes alternate between two round-seed slots: the round
+ *     seed waits for the emission of the pass before last, the last reader
+ *     of its slot (serial passes use slot 0 and wait for every emission);
  *   - the trace and the photon map overwrite the set the gather before last
  *     read: `stream` waits for that gather before the seed, and `side` follows
  *     `stream` from the seed on;
@@ -293,10 +295,13 @@ hipError_t PhxRenderer::sync_all() {
 int PhxRenderer::enqueue_pass(bool events, bool pipe) {
     int rc;
     const int set = pipe ? 1 : 0;
-    if (emit_pending) {
-        PHX_TRY(hipStreamWaitEvent(stream, ev_emit_done, 0), "event");
-        emit_pending = false;
-    }
+    /* serial passes use seed slot 0 */
+    seed_slot = pipe ? rounds & 1u : 0u;
+    for (uint32_t k = 0; k < 2; k++)
+        if (emit_pending[k] && (!pipe || k == seed_slot)) {
+            PHX_TRY(hipStreamWaitEvent(stream, ev_emit_done[k], 0), "event");
+            emit_pending[k] = false;
+        }
     for (int k = pipe ? 1 : 0; k < 2; k++)
         if (read_pending[k]) {
             PHX_TRY(hipStreamWaitEvent(stream, ev_read[k], 0), "event");
@@ -314,8 +319,8 @@ int PhxRenderer::enqueue_pass(bool events, bool pipe) {
     if (pipe) {
         PHX_TRY(hipMemcpyAsync(h_info, d_info.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, side), "pass info");
         PHX_TRY(hipEventRecord(ev_info, side), "event");
-        PHX_TRY(hipEventRecord(ev_emit_done, side), "event");
-        emit_pending = true;
+        PHX_TRY(hipEventRecord(ev_emit_done[seed_slot], side), "event");
+        emit_pending[seed_slot] = true;
     }
     if ((rc = hash_and_sort_photons(side, set)) != PHX_OK)
         return rc;
@@ -445,7 +450,7 @@ int phx_renderer_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, Ph
     for (hipEvent_t *ev : {&r->ev_emit, &r->ev_gather, &r->ev_trace})
         if ((e = hipEventCreate(ev)) != hipSuccess)
             return hip_status(e, "phx_renderer_create: event");
-    for (hipEvent_t *ev : {&r->ev_info, &r->ev_fork, &r->ev_join, &r->ev_trace_done, &r->ev_emit_done,
+    for (hipEvent_t *ev : {&r->ev_info, &r->ev_fork, &r->ev_join, &r->ev_trace_done, &r->ev_emit_done[0], &r->ev_emit_done[1],
                            &r->ev_read[0], &r->ev_read[1]})
         if ((e = hipEventCreateWithFlags(ev, hipEventDisableTiming)) != hipSuccess)
             return hip_status(e, "phx_renderer_create: event");

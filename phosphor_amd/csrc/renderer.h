@@ -58,12 +58,15 @@ struct PhxRenderer {
        between two sets: the one in `buffers` (the last pass's, which readers
        see) and `alt_hp` / `alt_map` (the one the next pass writes), each with
        the end of the last gather that read it (ev_read, [0] for the set in
-       `buffers`).  A pass's round seed waits for the previous pass's emission
-       (ev_emit_done), which reads the seed it replaces. */
+       `buffers`).  Pipelined passes also alternate between two round-seed
+       slots (seed_slot); a pass's round seed waits for the emission of the
+       pass before last (ev_emit_done[slot]), the last one that read that
+       slot. */
     hipStream_t gstream = nullptr;
-    hipEvent_t ev_trace_done = nullptr, ev_emit_done = nullptr, ev_read[2] = {};
+    hipEvent_t ev_trace_done = nullptr, ev_emit_done[2] = {}, ev_read[2] = {};
     bool read_pending[2] = {false, false};
-    bool emit_pending = false;
+    bool emit_pending[2] = {false, false};
+    uint32_t seed_slot = 0;
     bool alt_ready = false;
     DevBuf alt_hp[6];  /* pos, normal, throughput, base colour, direct, emission */
     DevBuf alt_map[6]; /* pos, power, dir, dirpow, cell_start, cell_end */
@@ -77,7 +80,7 @@ struct PhxRenderer {
     /* Renderer's own unsorted photon buffers (renderer.hpp:48-51) */
     DevBuf d_photon_pos, d_photon_power, d_photon_dir, d_batch_size;
     DevBuf d_emit_ws, d_scan_ws, d_sort_ws, d_trace_ws;
-    DevBuf d_rng;  /* [0] PCG state of the round seeds, [1] the current round seed */
+    DevBuf d_rng;  /* [0] PCG state of the round seeds, [1] and [2] the round-seed slots */
     DevBuf d_info; /* [0] last deposit count, [1] passes without deposits */
     uint32_t shadow_any_hit = 0;
     uint32_t emit_chunk = 1; /* photons per emission launch sequence */
@@ -100,7 +103,8 @@ struct PhxRenderer {
         for (hipEvent_t &e : ev)
             if (e)
                 (void)hipEventDestroy(e);
-        for (hipEvent_t e : {ev_emit, ev_gather, ev_trace, ev_fork, ev_join, ev_info, ev_trace_done, ev_emit_done, ev_read[0],
+        for (hipEvent_t e : {ev_emit, ev_gather, ev_trace, ev_fork, ev_join, ev_info, ev_trace_done, ev_emit_done[0],
+                             ev_emit_done[1], ev_read[0],
                              ev_read[1]})
             if (e)
                 (void)hipEventDestroy(e);
@@ -114,7 +118,7 @@ struct PhxRenderer {
             (void)hipStreamDestroy(stream);
     }
 
-    uint32_t *seed_ptr() const { return d_rng.as<uint32_t>() + 1; }
+    uint32_t *seed_ptr() const { return d_rng.as<uint32_t>() + 1 + seed_slot; }
     int set_scene(const PhxSceneDesc &scene);
     int run_pass();
     int enqueue_pass(bool events, bool pipe);
