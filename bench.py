@@ -295,7 +295,7 @@ def copy_calibration(mib: int) -> int:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=12)
+    ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="caustic", choices=["caustic", "cornell", "atrium", "batch64", "adjoint512"])
     ap.add_argument("--cpu-passes", type=int, default=1, help="oracle passes timed for the CPU baseline (0 = skip)")
