@@ -5,8 +5,11 @@ A step is one full SPPM pass (trace → emit → hash/scan/sort → gather) of t
 workload, with the scene already resident in HBM.  The metric is
 photons+rays/s per pass = (pixels·1 camera ray + photons per pass) / T_pass.
 
-Default workload: the caustic stress scene (Cornell box + two 20480-triangle
-GGX spheres, 1024², 2^21 photons/pass, r0 = 0.01).
+Default workload: the 300k-triangle atrium (jittered columns and arches, GGX
+roughness 0.05-0.9, 2048x1024 sky envmap with a 5e4 sun disk plus a matching
+directional light for photons; 1920x1080, 2^22 photons/pass, grid 64).
+--workload caustic runs the caustic stress scene (Cornell box + two
+20480-triangle GGX spheres, 1024², 2^21 photons/pass, r0 = 0.01).
 
 --workload adjoint512 measures gradients: forward + backward of a 32-pass
 render of the eight-material room (512², 2^20 photons per pass) with respect
@@ -37,6 +40,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
+# oracle passes timed for the CPU baseline per workload: the first passes of the
+# seed schedule after pass 1 (heavy workloads time a stated subset)
+CPU_PASSES = {"atrium": 2, "caustic": 4, "cornell": 16}
 
 
 def cpu_model() -> str:
@@ -297,8 +303,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="caustic", choices=["caustic", "cornell", "atrium", "batch64", "adjoint512"])
-    ap.add_argument("--cpu-passes", type=int, default=1, help="oracle passes timed for the CPU baseline (0 = skip)")
+    ap.add_argument("--workload", default="atrium", choices=["caustic", "cornell", "atrium", "batch64", "adjoint512"])
+    ap.add_argument("--cpu-passes", type=int, default=-1,
+                    help="oracle passes timed for the CPU baseline (0 = skip; default: 2 for the atrium, "
+                         "4 for the caustic scene, 16 for the Cornell box)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0 = all cores of this host)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the known-size copy kernel (counter calibration) over MIB MiB")
@@ -384,6 +392,8 @@ def main() -> int:
         print(f"[bench] kernel occupancy not available: {e}", file=sys.stderr)
     cpu = None
     rays = None
+    if args.cpu_passes < 0:
+        args.cpu_passes = CPU_PASSES.get(args.workload, 1)
     if args.cpu_passes > 0:
         threads = args.cpu_threads or cpu_cores()
         try:

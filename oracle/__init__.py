@@ -7,12 +7,15 @@ CPU-baseline leg use this package; the product (phosphor_amd) never does.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
 ORACLE_DIR = Path(__file__).resolve().parent
-LIB_PATH = ORACLE_DIR / "build" / "librefcpu.so"
+# PHOSPHOR_ORACLE_LIB selects another build of refcpu.cpp (the planted-mistake
+# test loads mutated copies this way)
+LIB_PATH = Path(os.environ.get("PHOSPHOR_ORACLE_LIB", ORACLE_DIR / "build" / "librefcpu.so"))
 _LIB = None
 
 VP, U32, F32, INT = C.c_void_p, C.c_uint32, C.c_float, C.c_int
@@ -32,6 +35,9 @@ _SIGS = {
     "ref_build_bvh": (U32, VP, U32, VP),
     "ref_scene_intersect": (INT, VP, VP, VP, U32, VP, VP, F32, F32, VP),
     "ref_sample_bsdf": (U32, VP, VP, VP, VP, INT, VP, VP),
+    "ref_sample_bsdf_ior": (U32, VP, VP, VP, VP, INT, VP, VP, VP, VP),
+    "ref_renderer_camera_ray": (U32, VP, U32, U32, VP, VP),
+    "ref_emit_photons_counted": (U32, VP, U32, U32, U32, VP, VP, VP, VP, U32),
     "ref_build_photon_hash_info": (None, VP, U32, VP),
     "ref_hash_photons": (None, VP, U32, VP, VP, VP),
     "ref_scan_cells": (None, VP, U32, VP, VP),
@@ -120,6 +126,23 @@ class RefRenderer:
 
     def emit_range(self, seed: int, first: int, count: int) -> int:
         return int(lib().ref_emit_range(self._h, seed, first, count, None, None, None, 0))
+
+    def camera_ray(self, seed: int, tid: int):
+        """Origin and direction of pixel `tid`'s camera ray in a pass with `seed`."""
+        o, d = np.zeros(4, np.float32), np.zeros(4, np.float32)
+        lib().ref_renderer_camera_ray(self._h, seed, tid, _p(o), _p(d))
+        return o, d
+
+    def emit_counted(self, seed: int, first: int, count: int):
+        """Deposits of photons [first, first+count) in (photon, bounce) order and
+        the number each photon made: (counts, pos, power, dir)."""
+        counts = np.zeros(count, np.uint32)
+        cap = count * self.config.max_photon_bounces
+        pos = np.zeros((cap, 4), np.float32)
+        pw = np.zeros(cap, np.uint32)
+        dr = np.zeros(cap, np.uint32)
+        n = lib().ref_emit_photons_counted(self._h, seed, first, count, _p(counts), _p(pos), _p(pw), _p(dr), cap)
+        return counts, pos[:n], pw[:n], dr[:n]
 
     def sample_light(self, seed: int, tid: int):
         o, d, p = (np.zeros(4, np.float32) for _ in range(3))

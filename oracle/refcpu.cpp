@@ -1419,6 +1419,36 @@ u32 ref_sample_bsdf(u32 *state, const PhxMaterial *mat, const f32 *normal, const
     return b.event;
 }
 
+/* shading.h:150-213 with an explicit IOR stack (ior[8], depth cur), read and
+ * written back, so that entering, leaving and nested media can be checked. */
+u32 ref_sample_bsdf_ior(u32 *state, const PhxMaterial *mat, const f32 *normal, const f32 *view, int front_face,
+                        f32 *ior_stack, u32 *ior_cur, f32 *out_dir, f32 *out_thr) {
+    Rng rng{*state};
+    Ctx c;
+    c.base_color = from(mat->base_color);
+    c.emissive = from(mat->emissive);
+    c.metallic = mat->metallic;
+    c.roughness = phx_clampf(mat->roughness, MIN_ROUGHNESS, 1.0f);
+    c.transmission = mat->transmission;
+    c.occlusion = 1.0f;
+    c.ior = mat->ior;
+    V4 n = mk(normal[0], normal[1], normal[2], 0.0f);
+    c.shading_normal = n;
+    IorStack ior;
+    for (u32 k = 0; k < IOR_STACK_SIZE; k++)
+        ior.ior[k] = ior_stack[k];
+    ior.cur = std::min(*ior_cur, IOR_STACK_SIZE - 1);
+    Bsdf b = sample_bsdf(&rng, c, n, n, mk(view[0], view[1], view[2], 0.0f), &ior, front_face != 0);
+    *state = rng.state;
+    for (u32 k = 0; k < IOR_STACK_SIZE; k++)
+        ior_stack[k] = ior.ior[k];
+    *ior_cur = ior.cur;
+    out_dir[0] = b.dir.x, out_dir[1] = b.dir.y, out_dir[2] = b.dir.z, out_dir[3] = b.dir.w;
+    out_thr[0] = b.throughput.x, out_thr[1] = b.throughput.y, out_thr[2] = b.throughput.z,
+    out_thr[3] = b.throughput.w;
+    return b.event;
+}
+
 /* photon_hash.h:41-50 */
 void ref_build_photon_hash_info(const PhxBvhNode *root, u32 grid_res, PhxPhotonHashData *out) {
     PhxPhotonHashData d;
@@ -1740,6 +1770,37 @@ u32 ref_renderer_sample_light(RefRenderer *r, u32 seed, u32 tid, f32 *o4, f32 *d
     std::memcpy(d4, &d, 16);
     std::memcpy(p4, &p, 16);
     return rng.state;
+}
+
+/* camera.h:28-47 for pixel `tid` of a pass with `seed` (trace_rays.cl:49):
+ * the camera ray and the PCG state after it. */
+u32 ref_renderer_camera_ray(RefRenderer *r, u32 seed, u32 tid, f32 *o4, f32 *d4) {
+    Rng rng = pcg_seed(seed + tid);
+    V4 o, d;
+    make_ray(&r->cam, tid % r->args.width, tid / r->args.width, r->args.width, r->args.height, &rng, &o, &d);
+    std::memcpy(o4, &o, 16);
+    std::memcpy(d4, &d, 16);
+    return rng.state;
+}
+
+/* emit_photons.cl:34-113 for photons [first, first+count) of a pass with
+ * `seed`: deposits of each photon in bounce order, and per-photon counts. */
+u32 ref_emit_photons_counted(RefRenderer *r, u32 seed, u32 first, u32 count, u32 *counts, phx_float4 *pos_out,
+                             u32 *pow_out, u32 *dir_out, u32 max_out) {
+    u32 B = r->args.max_photon_bounces;
+    std::vector<Deposit> dep((size_t)count * B);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)count; i++)
+        counts[i] = emit_photon(r->sc, first + (u32)i, seed, B, &dep[(size_t)i * B], nullptr);
+    u32 total = 0;
+    for (u32 i = 0; i < count; i++)
+        for (u32 k = 0; k < counts[i]; k++, total++)
+            if (total < max_out) {
+                pos_out[total] = to(dep[(size_t)i * B + k].pos);
+                pow_out[total] = dep[(size_t)i * B + k].power;
+                dir_out[total] = dep[(size_t)i * B + k].dir;
+            }
+    return total;
 }
 
 /* Skip a round's seed without rendering (keeps the seed schedule aligned). */

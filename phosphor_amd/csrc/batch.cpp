@@ -309,6 +309,11 @@ int PhxBatch::build(const PhxSceneDesc *descs) {
         max_tp = std::max(max_tp, g.count);
     for (const BatchGroup &g : plan.emit_groups)
         max_ep = std::max(max_ep, g.count);
+    /* a trace group's shadow queue holds 5 NEE events per pixel, each with up to
+       max_ds light samples, indexed by a u32 slot counter */
+    if ((uint64_t)max_tp * 5u * max_ds > 0xFFFFFFFFull)
+        return fail(PHX_ERR_TOO_LARGE, "phx_batch_create: a trace group's shadow queue exceeds 2^32 slots "
+                                       "(lower trace_group_pixels)");
 
     PHX_TRY(copy_to_dev_buf(d_table, table.data(), S, sizeof(BatchScene), s), "batch table");
     PHX_TRY(copy_to_dev_buf(d_chunk_scene, chunk_scene.data(), chunk_scene.size(), 4, s), "batch table");
@@ -414,10 +419,11 @@ int phx_batch_create(const PhxRenderArgs *args, const PhxSceneDesc *scenes, uint
         return fail(PHX_ERR_INVALID_ARG, "phx_batch_create: null argument or no scene");
     *out = nullptr;
     for (uint32_t i = 0; i < n_scenes; i++) {
-        if (check_scene_desc(scenes[i], "phx_batch_create") != PHX_OK ||
-            check_render_args(args[i], "phx_batch_create") != PHX_OK)
-            return fail(PHX_ERR_INVALID_ARG, "phx_batch_create: scene " + std::to_string(i) + ": " +
-                                                 std::string(phx_last_error()));
+        int rc = check_scene_desc(scenes[i], "phx_batch_create");
+        if (rc == PHX_OK)
+            rc = check_render_args(args[i], "phx_batch_create");
+        if (rc != PHX_OK)
+            return fail(rc, "phx_batch_create: scene " + std::to_string(i) + ": " + std::string(phx_last_error()));
     }
     if (options && (options->flags & ~(uint32_t)(PHX_PASS_GRAPH | PHX_REFERENCE_BVH | PHX_PASS_SERIAL)))
         return fail(PHX_ERR_INVALID_ARG, "phx_batch_create: unknown flags");

@@ -59,6 +59,15 @@ int check_render_args(const PhxRenderArgs &a, const char *who) {
         return fail(PHX_ERR_INVALID_ARG, w + ": unknown flags");
     if (a.emit_wave_bounces > 32)
         return fail(PHX_ERR_INVALID_ARG, w + ": emit_wave_bounces outside 0..32");
+    /* pixel indices, shadow-queue slots (5 NEE events per pixel at most, each
+       with direct_samples light samples) and deposit slots are u32 on the device */
+    uint64_t px = (uint64_t)a.width * a.height;
+    if (px > 0xFFFFFFFFull)
+        return fail(PHX_ERR_TOO_LARGE, w + ": width * height exceeds 2^32 pixels");
+    if (px * 5u * a.direct_samples > 0xFFFFFFFFull)
+        return fail(PHX_ERR_TOO_LARGE, w + ": width * height * 5 * direct_samples exceeds 2^32 shadow slots");
+    if ((uint64_t)a.photons_per_round * a.max_photon_bounces > 0xFFFFFFFFull)
+        return fail(PHX_ERR_TOO_LARGE, w + ": photons_per_round * max_bounces exceeds 2^32 deposit slots");
     return PHX_OK;
 }
 } // namespace phx
@@ -430,8 +439,9 @@ int phx_renderer_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, Ph
     *out = nullptr;
     if (check_scene_desc(*scene, "phx_renderer_create") != PHX_OK)
         return PHX_ERR_INVALID_ARG;
-    if (check_render_args(*args, "phx_renderer_create") != PHX_OK)
-        return PHX_ERR_INVALID_ARG;
+    int rc = check_render_args(*args, "phx_renderer_create");
+    if (rc != PHX_OK)
+        return rc;
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
         return fail(PHX_ERR_NO_DEVICE, "phx_renderer_create: no HIP device");
@@ -457,7 +467,7 @@ int phx_renderer_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, Ph
     if ((e = hipHostMalloc((void **)&r->h_info, 2 * sizeof(uint32_t), hipHostMallocDefault)) != hipSuccess)
         return hip_status(e, "phx_renderer_create: pinned info");
     r->h_info[0] = r->h_info[1] = 0;
-    int rc = r->set_scene(*scene);
+    rc = r->set_scene(*scene);
     if (rc != PHX_OK)
         return rc;
     *out = r.release();

@@ -253,3 +253,34 @@ def test_renderer_flags_checked_before_the_device(phx_lib):
     descs = (abi.SceneDesc * 1)(desc)
     assert phx_lib.phx_batch_create(C.cast(args, C.c_void_p), C.cast(descs, C.c_void_p), 1, C.byref(opt),
                                     C.byref(h)) == abi.PHX_ERR_NO_DEVICE
+
+
+def test_u32_index_overflow_refused(phx_lib):
+    """Pixel indices, shadow-queue slots (5 NEE events per pixel, each with
+    direct_samples light samples) and deposit slots are u32 on the device:
+    configurations beyond them are refused with PHX_ERR_TOO_LARGE before any
+    allocation, by the renderer and by the batch (whose trace group sizes the
+    shadow queue as well)."""
+    from phosphor_amd import PassConfig, scenes
+
+    sc = scenes.cornell_box(8, 8)
+    desc = sc.desc()
+    h = C.c_void_p()
+    cases = (dict(width=3840, height=2160, direct_samples=128),     # 5.3 G shadow slots
+             dict(width=70000, height=70000, direct_samples=1),     # 4.9 G pixels
+             dict(width=8, height=8, photons_per_round=1 << 28, max_photon_bounces=32))
+    for kw in cases:
+        cfg = PassConfig(**{"photons_per_round": 16, "grid_res": 8, **kw})
+        a = cfg.to_c()
+        assert phx_lib.phx_renderer_create(C.byref(a), C.byref(desc), C.byref(h)) == abi.PHX_ERR_TOO_LARGE, kw
+        assert not h.value
+        args = (abi.RenderArgs * 1)(a)
+        descs = (abi.SceneDesc * 1)(desc)
+        assert phx_lib.phx_batch_create(C.cast(args, C.c_void_p), C.cast(descs, C.c_void_p), 1, None,
+                                        C.byref(h)) == abi.PHX_ERR_TOO_LARGE, kw
+    # the largest image the shadow queue allows at 128 samples passes the check
+    ok = PassConfig(width=2560, height=1310, direct_samples=128, photons_per_round=16, grid_res=8).to_c()
+    want = abi.PHX_ERR_NO_DEVICE if phx_lib.phx_device_count() == 0 else None
+    if want is not None:
+        assert phx_lib.phx_renderer_create(C.byref(ok), C.byref(desc), C.byref(h)) == want
+
