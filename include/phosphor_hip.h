@@ -226,6 +226,23 @@ size_t phx_scan_cells_workspace_size(uint32_t n_cells);
 int phx_scan_cells(const uint32_t *cell_counts, uint32_t n_cells, uint32_t *cell_start, uint32_t *cell_end,
                    void *workspace, size_t workspace_bytes, void *stream);
 
+/* Largest n_cells phx_scan_cells_noalloc accepts (8160 tiles of 4096 cells:
+ * above the reference's 4096² scan limit and the 255³ + 1 buckets of its
+ * largest grid). */
+#define PHX_SCAN_NOALLOC_MAX_CELLS 33423360u
+
+/* No counterpart in the reference.  phx_scan_cells for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same scan and postcondition, with its
+ * workspace in a 64 KiB file-scope device array instead of a caller buffer;
+ * the call allocates nothing.  n_cells above PHX_SCAN_NOALLOC_MAX_CELLS is
+ * refused with PHX_ERR_TOO_LARGE.  All calls on one GPU share that GPU's
+ * array: calls that overlap on one GPU (concurrent streams or threads) race
+ * and must be serialised by the caller; calls in order on one stream are
+ * safe (each clears the words it uses). */
+int phx_scan_cells_noalloc(const uint32_t *cell_counts, uint32_t n_cells, uint32_t *cell_start, uint32_t *cell_end,
+                           void *stream);
+
 /* Replaces kernels/scatter_photons.cl:4-7 with its exact semantics:
  * dst = atomic_inc(&cell_end[h]); records copied to dst.  Order inside a cell
  * follows atomic arrival (nondeterministic, as in the reference). */

@@ -129,6 +129,7 @@ SIGNATURES = {
     "phx_hash_photons": (INT, VP, U32, VP, VP, VP, VP),
     "phx_scan_cells_workspace_size": (SZ, U32),
     "phx_scan_cells": (INT, VP, U32, VP, VP, VP, SZ, VP),
+    "phx_scan_cells_noalloc": (INT, VP, U32, VP, VP, VP),
     "phx_scatter_photons": (INT, VP, VP, VP, VP, VP, U32, VP, VP, VP, VP),
     "phx_sort_photons_workspace_size": (SZ, U32, U32),
     "phx_sort_photons": (INT, VP, VP, VP, VP, VP, VP, U32, U32, VP, VP, VP, VP, SZ, VP),

@@ -30,6 +30,9 @@ int hip_status(hipError_t e, const char *what) {
 
 /* Layout checks: the boundary structs must match the reference byte for byte. */
 static_assert(sizeof(phx_float4) == 16, "float4");
+static_assert(256 + 8 * ((PHX_SCAN_NOALLOC_MAX_CELLS + phx::SCAN_TILE - 1) / phx::SCAN_TILE) <=
+                  phx::NOALLOC_SCAN_SCRATCH_BYTES,
+              "the allocation-free scan's largest input fits its scratch array");
 static_assert(sizeof(PhxCameraParams) == 112, "CameraParams 112 B (camera.h:7-22)");
 static_assert(offsetof(PhxCameraParams, defocus_angle) == 96, "CameraParams.defocus_angle");
 static_assert(sizeof(PhxTriangleVertices) == 48 && sizeof(PhxTriangleNormals) == 48 &&
@@ -232,6 +235,21 @@ int phx_scan_cells(const uint32_t *cell_counts, uint32_t n_cells, uint32_t *cell
         return fail(PHX_ERR_WORKSPACE, "phx_scan_cells: workspace smaller than phx_scan_cells_workspace_size");
     return hip_status(launch_scan(cell_counts, n_cells, cell_start, cell_end, nullptr, workspace, (hipStream_t)stream),
                       "phx_scan_cells");
+}
+
+int phx_scan_cells_noalloc(const uint32_t *cell_counts, uint32_t n_cells, uint32_t *cell_start, uint32_t *cell_end,
+                           void *stream) {
+    if (n_cells > 0 && (!cell_counts || !cell_start))
+        return fail(PHX_ERR_INVALID_ARG, "phx_scan_cells_noalloc: null pointer argument");
+    if (n_cells > PHX_SCAN_NOALLOC_MAX_CELLS || scan_workspace_bytes(n_cells) > NOALLOC_SCAN_SCRATCH_BYTES)
+        return fail(PHX_ERR_TOO_LARGE, "phx_scan_cells_noalloc: n_cells above PHX_SCAN_NOALLOC_MAX_CELLS (" +
+                                           std::to_string(PHX_SCAN_NOALLOC_MAX_CELLS) + ")");
+    void *ws = nullptr;
+    hipError_t e = noalloc_scan_scratch(&ws);
+    if (e != hipSuccess)
+        return hip_status(e, "phx_scan_cells_noalloc: scratch address");
+    return hip_status(launch_scan(cell_counts, n_cells, cell_start, cell_end, nullptr, ws, (hipStream_t)stream),
+                      "phx_scan_cells_noalloc");
 }
 
 int phx_scatter_photons(const phx_float4 *photon_pos_in, const uint32_t *photon_power_in,

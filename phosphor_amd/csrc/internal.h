@@ -63,6 +63,11 @@ size_t scan_workspace_bytes(uint32_t n);
  * With d_n (device) the length is min(*d_n, n). */
 hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *total, void *ws,
                        hipStream_t s, const uint32_t *d_n = nullptr);
+/* Size of the file-scope device array the allocation-free scan uses as its
+ * workspace (ticket + one status word per tile), and its device address on
+ * the current device (looked up once per device, then cached). */
+constexpr size_t NOALLOC_SCAN_SCRATCH_BYTES = 64 * 1024;
+hipError_t noalloc_scan_scratch(void **out);
 
 hipError_t launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
 /* dst[i] = src[i] for n 16-byte items */

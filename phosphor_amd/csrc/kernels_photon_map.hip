@@ -21,6 +21,8 @@
  */
 #include "pass_bodies.h"
 
+#include <mutex>
+
 namespace phx {
 
 /* ---------------- fill ---------------- */
@@ -249,6 +251,36 @@ hipError_t launch_scan(const u32 *in, u32 n, u32 *out, u32 *out2, u32 *total, vo
         return e;
     hipLaunchKernelGGL(k_scan, dim3(n_tiles), dim3(SCAN_BLOCK), 0, s, in, n, d_n, out, out2, total, status, ticket);
     return hipGetLastError();
+}
+
+/* ---------------- scan scratch without allocation ---------------- */
+/* Workspace of phx_scan_cells_noalloc: the ticket (256 B) and one 8-byte
+   look-back status word per 4096-cell tile, 64 KiB in all.  Each GPU the
+   library runs on holds its own copy until the process ends.  Host code uses
+   the address hipGetSymbolAddress returns for the current device, never the
+   array's name (which is a host-side stand-in). */
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCAN_SCRATCH_BYTES];
+
+hipError_t noalloc_scan_scratch(void **out) {
+    constexpr int MAX_DEVICES = 64;
+    static std::mutex mu;
+    static void *ptr[MAX_DEVICES] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return e;
+    if (dev < 0 || dev >= MAX_DEVICES)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        ptr[dev] = p;
+    }
+    *out = ptr[dev];
+    return hipSuccess;
 }
 
 /* ---------------- hash (hash_photons.cl) ---------------- */

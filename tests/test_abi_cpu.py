@@ -284,3 +284,17 @@ def test_u32_index_overflow_refused(phx_lib):
     if want is not None:
         assert phx_lib.phx_renderer_create(C.byref(ok), C.byref(desc), C.byref(h)) == want
 
+
+
+def test_scan_noalloc_limits_checked_before_the_device(phx_lib):
+    """phx_scan_cells_noalloc refuses inputs beyond its 64 KiB scratch array
+    (PHX_SCAN_NOALLOC_MAX_CELLS) and null pointers before it touches a device."""
+    import os
+
+    assert phx_lib.phx_scan_cells_noalloc(1, 33423360 + 1, 1, 1, None) == abi.PHX_ERR_TOO_LARGE
+    assert b"PHX_SCAN_NOALLOC_MAX_CELLS" in phx_lib.phx_last_error()
+    assert phx_lib.phx_scan_cells_noalloc(1, 0xFFFFFFFF, 1, 1, None) == abi.PHX_ERR_TOO_LARGE
+    assert phx_lib.phx_scan_cells_noalloc(None, 10, None, None, None) == abi.PHX_ERR_INVALID_ARG
+    assert phx_lib.phx_scan_cells_workspace_size(33423360) == 64 * 1024
+    if not os.path.exists("/dev/kfd"):
+        assert phx_lib.phx_scan_cells_noalloc(None, 0, None, None, None) != 0   # no device here
