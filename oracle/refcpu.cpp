@@ -251,6 +251,22 @@ inline u32 get_hash(V4 pos, const PhxPhotonHashData &d) {
     return g * g * x + g * y + z + 1;
 }
 
+/* Half-cell octant of a point in its cell: bit 0/1/2 set where the x/y/z
+ * quotient's fraction (exact for the same rounded quotient as get_hash) is at
+ * least 0.5; 0 outside the grid.  The pass orders each cell's records by
+ * octant (stable), a canonical instance of the reference's in-cell order,
+ * which is atomic arrival (scatter_photons.cl:13). */
+inline u32 octant_of(V4 pos, const PhxPhotonHashData &d) {
+    u32 g = d.grid_res;
+    f32 fx = (pos.x - d.origin.x) / d.cell_sizes.x;
+    f32 fy = (pos.y - d.origin.y) / d.cell_sizes.y;
+    f32 fz = (pos.z - d.origin.z) / d.cell_sizes.z;
+    if (fx < 0.0f || fy < 0.0f || fz < 0.0f || fx >= (f32)g || fy >= (f32)g || fz >= (f32)g)
+        return 0;
+    u32 x = (u32)fx, y = (u32)fy, z = (u32)fz;
+    return (fx - (f32)x >= 0.5f ? 1u : 0u) | (fy - (f32)y >= 0.5f ? 2u : 0u) | (fz - (f32)z >= 0.5f ? 4u : 0u);
+}
+
 /* ---------------- hit.h ---------------- */
 struct HitRecord {
     f32 t, u, v;
@@ -1482,6 +1498,42 @@ void ref_scan_cells(const u32 *counts, u32 n, u32 *start, u32 *end) {
         s += counts[i];
     }
 }
+/* Within every cell [cstart, cend), a stable partition of the records by
+ * octant (octant_of): the photon map's order becomes (cell, octant, input
+ * order). */
+static void order_cells_by_octant(phx_float4 *pos, u32 *pw, u32 *dir, const u32 *cstart, const u32 *cend, u32 n_cells,
+                                  const PhxPhotonHashData &d) {
+#pragma omp parallel
+    {
+        std::vector<phx_float4> tp;
+        std::vector<u32> tw, td, oc;
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t h = 0; h < (int64_t)n_cells; h++) {
+            u32 a = cstart[h], b = cend[h];
+            if (b - a < 2)
+                continue;
+            u32 m = b - a;
+            tp.assign(pos + a, pos + b);
+            tw.assign(pw + a, pw + b);
+            td.assign(dir + a, dir + b);
+            oc.resize(m);
+            u32 cnt[9] = {0};
+            for (u32 i = 0; i < m; i++) {
+                oc[i] = octant_of(from(tp[i]), d);
+                cnt[oc[i] + 1]++;
+            }
+            for (u32 q = 1; q < 9; q++)
+                cnt[q] += cnt[q - 1];
+            for (u32 i = 0; i < m; i++) {
+                u32 dst = a + cnt[oc[i]]++;
+                pos[dst] = tp[i];
+                pw[dst] = tw[i];
+                dir[dst] = td[i];
+            }
+        }
+    }
+}
+
 /* scatter_photons.cl with a stable (input-order) cursor walk */
 void ref_sort_photons(const phx_float4 *pos_in, const u32 *pow_in, const u32 *dir_in, const u32 *hashes,
                       u32 *cell_end, u32 n, phx_float4 *pos_out, u32 *pow_out, u32 *dir_out) {
@@ -1736,6 +1788,8 @@ int ref_renderer_run_pass(RefRenderer *r) {
     r->sdir.resize(n);
     ref_sort_photons(r->ppos.data(), r->ppow.data(), r->pdir.data(), r->hashes.data(), r->cend.data(), n,
                      r->spos.data(), r->spow.data(), r->sdir.data());
+    order_cells_by_octant(r->spos.data(), r->spow.data(), r->sdir.data(), r->cstart.data(), r->cend.data(),
+                          r->bucket_count, r->hash);
     double t3 = now_ms();
     /* gather_flux (renderer.cpp:155-161) */
     uint64_t visits = 0;

@@ -46,7 +46,7 @@ struct PhxBatch {
     bool read_pending[2] = {false, false};
     bool emit_pending = false;
     bool alt_ready = false;
-    DevBuf alt_hp[6], alt_map[6]; /* map: pos, power, dir, dirpow, cell_start, cell_end */
+    DevBuf alt_hp[6], alt_map[7]; /* map: pos, power, dir, dirpow, cell_start, cell_end, oct_start */
     BatchPlan plan_alt;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -56,7 +56,7 @@ struct PhxBatch {
     DevBuf d_tickets, d_counter;
     DevBuf d_hp[6], d_flux, d_weight, d_radiance;
     DevBuf d_dep_pos, d_dep_power, d_dep_dir, d_map_pos, d_map_power, d_map_dir, d_hashes, d_map_dirpow;
-    DevBuf d_cell_counts, d_cell_start, d_cell_end;
+    DevBuf d_cell_counts, d_cell_start, d_cell_end, d_oct_start;
     DevBuf d_trace_ws, d_emit_ws, d_scan_ws, d_sort_ws;
     BatchPlan plan;
     uint32_t *h_info = nullptr; /* pinned: per-scene deposit counts, then per-scene passes without deposits */
@@ -88,9 +88,9 @@ struct PhxBatch {
     int collect();
     int run_pass();
     int alloc_alt();
-    void map_bufs(DevBuf *out[6]) {
-        DevBuf *m[6] = {&d_map_pos, &d_map_power, &d_map_dir, &d_map_dirpow, &d_cell_start, &d_cell_end};
-        for (int k = 0; k < 6; k++)
+    void map_bufs(DevBuf *out[7]) {
+        DevBuf *m[7] = {&d_map_pos, &d_map_power, &d_map_dir, &d_map_dirpow, &d_cell_start, &d_cell_end, &d_oct_start};
+        for (int k = 0; k < 7; k++)
             out[k] = m[k];
     }
     /* waits for every stream of the batch */
@@ -111,11 +111,13 @@ struct PhxBatch {
 int PhxBatch::alloc_alt() {
     if (alt_ready)
         return PHX_OK;
-    DevBuf *m[6];
+    DevBuf *m[7];
     map_bufs(m);
     for (int k = 0; k < 6; k++) {
         PHX_TRY(alt_hp[k].alloc(d_hp[k].bytes), "batch second hitpoint set");
         PHX_TRY(hipMemsetAsync(alt_hp[k].p, 0, alt_hp[k].bytes, stream), "batch second hitpoint set");
+    }
+    for (int k = 0; k < 7; k++) {
         PHX_TRY(alt_map[k].alloc(m[k]->bytes), "batch second photon map");
         PHX_TRY(hipMemsetAsync(alt_map[k].p, 0, alt_map[k].bytes, stream), "batch second photon map");
     }
@@ -128,6 +130,7 @@ int PhxBatch::alloc_alt() {
     plan_alt.map_dirpow = alt_map[3].as<phx_float4>();
     plan_alt.cell_start = alt_map[4].as<uint32_t>();
     plan_alt.cell_end = alt_map[5].as<uint32_t>();
+    plan_alt.oct_start = alt_map[6].as<uint32_t>();
     alt_ready = true;
     return PHX_OK;
 }
@@ -193,12 +196,12 @@ int PhxBatch::run_pass() {
         read_pending[1] = true;
         emit_pending = true;
         /* the set just written becomes the one in d_hp / d_map_* */
-        DevBuf *m[6];
+        DevBuf *m[7];
         map_bufs(m);
-        for (int k = 0; k < 6; k++) {
+        for (int k = 0; k < 6; k++)
             d_hp[k].swap(alt_hp[k]);
+        for (int k = 0; k < 7; k++)
             m[k]->swap(alt_map[k]);
-        }
         std::swap(plan, plan_alt);
         std::swap(ev_read[0], ev_read[1]);
         std::swap(read_pending[0], read_pending[1]);
@@ -294,7 +297,8 @@ int PhxBatch::build(const PhxSceneDesc *descs) {
         max_b = std::max(max_b, a.max_photon_bounces);
         rng[i] = host_pcg_seed(a.seed);
     }
-    if (px > 0xFFFFFFFFull || photons > 0xFFFFFFFFull || buckets > 0xFFFFFFFFull || deps > 0xFFFFFFFFull)
+    /* octant sort keys are 8 per bucket */
+    if (px > 0xFFFFFFFFull || photons > 0xFFFFFFFFull || buckets * 8u > 0xFFFFFFFFull || deps > 0xFFFFFFFFull)
         return fail(PHX_ERR_TOO_LARGE, "phx_batch_create: batch totals exceed 32-bit indices");
     ph_base[S] = (uint32_t)photons;
 
@@ -352,6 +356,7 @@ int PhxBatch::build(const PhxSceneDesc *descs) {
     PHX_TRY(d_cell_counts.alloc(buckets * 4), "batch cells");
     PHX_TRY(d_cell_start.alloc(buckets * 4), "batch cells");
     PHX_TRY(d_cell_end.alloc(buckets * 4), "batch cells");
+    PHX_TRY(d_oct_start.alloc(buckets * 32), "batch cells");
     PHX_TRY(d_trace_ws.alloc(batch_trace_workspace_bytes(max_tp, max_ds)), "batch trace workspace");
     PHX_TRY(d_emit_ws.alloc(batch_emit_workspace_bytes(max_ep, max_b)), "batch emit workspace");
     PHX_TRY(d_scan_ws.alloc(scan_workspace_bytes((uint32_t)buckets)), "batch scan workspace");
@@ -386,6 +391,7 @@ int PhxBatch::build(const PhxSceneDesc *descs) {
     P.cell_counts = d_cell_counts.as<uint32_t>();
     P.cell_start = d_cell_start.as<uint32_t>();
     P.cell_end = d_cell_end.as<uint32_t>();
+    P.oct_start = d_oct_start.as<uint32_t>();
     P.dep_cap = (uint32_t)deps;
     P.total_buckets = (uint32_t)buckets;
     P.total_tiles = (uint32_t)tiles;

@@ -101,7 +101,7 @@ hipError_t launch_hash(const phx_float4 *pos, uint32_t n, const PhxPhotonHashDat
 /* Same with the photon count read on the device (clamped to max_n); counts
  * may be null (keys only). */
 hipError_t launch_hash_dev(const phx_float4 *pos, const uint32_t *d_n, uint32_t max_n, const PhxPhotonHashData &d,
-                           uint32_t *hashes, uint32_t *counts, hipStream_t s);
+                           uint32_t *hashes, uint32_t *counts, hipStream_t s, bool octants = false);
 
 hipError_t launch_scatter_atomic(const phx_float4 *pos_in, const uint32_t *pow_in, const uint32_t *dir_in,
                                  const uint32_t *hashes, uint32_t *cell_end, uint32_t n, phx_float4 *pos_out,
@@ -112,10 +112,14 @@ hipError_t launch_sort(const phx_float4 *pos_in, const uint32_t *pow_in, const u
                        const uint32_t *hashes, const uint32_t *cell_start, uint32_t *cell_end, uint32_t n,
                        uint32_t n_cells, phx_float4 *pos_out, uint32_t *pow_out, uint32_t *dir_out, void *ws,
                        hipStream_t s, const uint32_t *d_n = nullptr, uint32_t *perm_out = nullptr,
-                       uint32_t *cell_start_out = nullptr);
+                       uint32_t *cell_start_out = nullptr, uint32_t *oct_start_out = nullptr);
 /* With cell_start_out the cell starts come from the sorted keys (binary
  * search: the same numbers as the exclusive scan of the cell counts) and
- * cell_start is ignored, so the caller needs neither counts nor scan. */
+ * cell_start is ignored, so the caller needs neither counts nor scan.  With
+ * oct_start_out (8·n_cells entries) the keys are
+ * octant keys (get_hash_octant, 8h + octant): records are sorted by cell,
+ * then octant, then input order, and oct_start_out receives every non-empty
+ * cell's octant starts (k_octant_starts). */
 
 hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const phx_float4 *hp_throughput,
                          const phx_float4 *hp_base_color, const phx_float4 *hp_direct, const phx_float4 *hp_emission,
@@ -123,7 +127,7 @@ hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, 
                          const uint32_t *cell_start, const uint32_t *cell_end, const PhxPhotonHashData &d,
                          float alpha, phx_float4 *flux, float *photon_weight, phx_float4 *radiance, uint32_t width,
                          hipStream_t s, float *rec_ratio = nullptr, float *rec_r2 = nullptr,
-                         const phx_float4 *pdirpow = nullptr);
+                         const phx_float4 *pdirpow = nullptr, const uint32_t *oct_start = nullptr);
 /* every stored photon record's decoded direction with its packed power in w
  * (count on the device, clamped to max_n) */
 hipError_t launch_decode_photons(const uint32_t *ppow, const uint32_t *pdir, const uint32_t *d_n, uint32_t max_n,
@@ -172,6 +176,7 @@ struct BatchPlan {
     phx_float4 *dep_pos = nullptr, *map_pos = nullptr;
     uint32_t *dep_power = nullptr, *dep_dir = nullptr, *map_power = nullptr, *map_dir = nullptr;
     uint32_t *hashes = nullptr, *cell_counts = nullptr, *cell_start = nullptr, *cell_end = nullptr;
+    uint32_t *oct_start = nullptr; /* 8 octant starts per bucket (records sorted by cell, then octant) */
     phx_float4 *map_dirpow = nullptr; /* decoded direction, packed power in w */
     uint32_t dep_cap = 0, total_buckets = 0, total_tiles = 0;
     void *trace_ws = nullptr, *emit_ws = nullptr, *scan_ws = nullptr, *sort_ws = nullptr;

@@ -259,9 +259,10 @@ __global__ __launch_bounds__(256) void k_bhash(const BatchScene *__restrict__ sc
         u32 s = find_scene(dep_base, 0u, n_scenes, i);
         const BatchScene &B = scenes[s];
         phx_float4 p = pos[i];
-        u32 h = B.bucket_base + get_hash(p.x, p.y, p.z, B.hash);
-        hashes[i] = h;
-        atomicAdd(&counts[h], 1u);
+        /* octant key of the batch-wide bucket (8 keys per bucket) */
+        u32 key = B.bucket_base * 8u + get_hash_octant(p.x, p.y, p.z, B.hash);
+        hashes[i] = key;
+        atomicAdd(&counts[key >> 3], 1u);
     }
 }
 
@@ -381,7 +382,8 @@ hipError_t launch_batch_pass(const BatchPlan &P, const BatchStreams &B) {
         }
         BTRY(launch_scan(P.cell_counts, P.total_buckets, P.cell_start, P.cell_end, nullptr, P.scan_ws, s));
         BTRY(launch_sort(P.dep_pos, P.dep_power, P.dep_dir, P.hashes, P.cell_start, P.cell_end, P.dep_cap,
-                         P.total_buckets, P.map_pos, P.map_power, P.map_dir, P.sort_ws, s, P.dep_counter));
+                         P.total_buckets, P.map_pos, P.map_power, P.map_dir, P.sort_ws, s, P.dep_counter, nullptr,
+                         nullptr, P.oct_start));
         BTRY(launch_decode_photons(P.map_power, P.map_dir, P.dep_counter, P.dep_cap, P.map_dirpow, s));
         BTRY(hipEventRecord(ev_join, side));
         s = main;
@@ -420,7 +422,7 @@ hipError_t launch_batch_pass(const BatchPlan &P, const BatchStreams &B) {
     /* gather_flux over every tile of every scene */
     GatherArgs ga{P.hp[0], P.hp[1], P.hp[2], P.hp[3], P.hp[4], P.hp[5], 0u, P.map_pos, P.map_power, P.map_dir,
                   P.cell_start, P.cell_end, PhxPhotonHashData{}, 0.0f, P.flux, P.photon_weight, P.radiance,
-                  nullptr, nullptr, P.map_dirpow};
+                  nullptr, nullptr, P.map_dirpow, P.oct_start};
     BGatherArgs gargs{P.scenes, P.tile_scene, P.total_tiles, ga};
     hipLaunchKernelGGL(k_bgather, dim3((P.total_tiles + 3) / 4), dim3(256), 0, s, gargs);
     return hipGetLastError();

@@ -286,14 +286,14 @@ hipError_t noalloc_scan_scratch(void **out) {
 /* ---------------- hash (hash_photons.cl) ---------------- */
 __global__ __launch_bounds__(256) void k_hash(const phx_float4 *__restrict__ pos, u32 n, const u32 *d_n, u32 max_n,
                                               PhxPhotonHashData d, u32 *__restrict__ hashes,
-                                              u32 *__restrict__ counts) {
+                                              u32 *__restrict__ counts, u32 octants) {
     if (d_n) {
         u32 c = *d_n;
         n = c < max_n ? c : max_n;
     }
     for (u32 i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         phx_float4 p = pos[i];
-        u32 h = get_hash(p.x, p.y, p.z, d);
+        u32 h = octants ? get_hash_octant(p.x, p.y, p.z, d) : get_hash(p.x, p.y, p.z, d);
         hashes[i] = h;
         if (counts)
             atomicAdd(&counts[h], 1u);
@@ -307,18 +307,20 @@ hipError_t launch_hash(const phx_float4 *pos, u32 n, const PhxPhotonHashData &d,
     u32 blocks = (n + 255) / 256;
     if (blocks > 8192)
         blocks = 8192;
-    hipLaunchKernelGGL(k_hash, dim3(blocks), dim3(256), 0, s, pos, n, (const u32 *)nullptr, 0u, d, hashes, counts);
+    hipLaunchKernelGGL(k_hash, dim3(blocks), dim3(256), 0, s, pos, n, (const u32 *)nullptr, 0u, d, hashes, counts,
+                       0u);
     return hipGetLastError();
 }
 
 hipError_t launch_hash_dev(const phx_float4 *pos, const u32 *d_n, u32 max_n, const PhxPhotonHashData &d, u32 *hashes,
-                           u32 *counts, hipStream_t s) {
+                           u32 *counts, hipStream_t s, bool octants) {
     u32 blocks = (max_n + 255) / 256;
     if (blocks > 8192)
         blocks = 8192;
     if (blocks == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_hash, dim3(blocks), dim3(256), 0, s, pos, 0u, d_n, max_n, d, hashes, counts);
+    hipLaunchKernelGGL(k_hash, dim3(blocks), dim3(256), 0, s, pos, 0u, d_n, max_n, d, hashes, counts,
+                       octants ? 1u : 0u);
     return hipGetLastError();
 }
 
@@ -472,18 +474,41 @@ __global__ void k_sort_hist_len(u32 n, const u32 *d_n, u32 *len) {
 /* cell_start[h] = number of sorted keys below h (the exclusive scan of the
  * cell counts, scan_small_chunks.cl, found by binary search instead) */
 __global__ __launch_bounds__(256) void k_cell_start_search(const u32 *__restrict__ keys, u32 n, const u32 *d_n,
-                                                           u32 n_cells, u32 *__restrict__ cell_start) {
+                                                           u32 n_cells, u32 *__restrict__ cell_start, u32 key_shift) {
     n = sort_count(n, d_n);
     for (u32 h = blockIdx.x * 256 + threadIdx.x; h < n_cells; h += gridDim.x * 256) {
         u32 lo = 0, hi = n;
+        u32 kh = h << key_shift;
         while (lo < hi) {
             u32 mid = lo + (hi - lo) / 2;
-            if (keys[mid] < h)
+            if (keys[mid] < kh)
                 lo = mid + 1;
             else
                 hi = mid;
         }
         cell_start[h] = lo;
+    }
+}
+
+/* oct_start[8h + s] = index of the first record of cell h in octant s or a
+ * later octant (the cell's end after its last octant), for every cell h that
+ * holds records, from the sorted octant keys (8h + s): each entry of such a
+ * cell is written once, by the record that starts the octant run it belongs
+ * to; entries of empty cells are left alone (the gather never reads them). */
+__global__ __launch_bounds__(256) void k_octant_starts(const u32 *__restrict__ keys, u32 n, const u32 *d_n,
+                                                       u32 *__restrict__ oct_start) {
+    n = sort_count(n, d_n);
+    for (u32 i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        u32 key = keys[i];
+        u32 h = key >> 3, o = key & 7u;
+        u32 prev = i > 0 ? keys[i - 1] : 0xFFFFFFFFu;
+        u32 lo = (i == 0 || (prev >> 3) != h) ? 0u : (prev == key ? 8u : (prev & 7u) + 1u);
+        u32 *row = oct_start + (size_t)h * 8u;
+        for (u32 q = lo; q <= o && lo < 8u; q++)
+            row[q] = i;
+        if (i + 1 == n || (keys[i + 1] >> 3) != h)
+            for (u32 q = o + 1; q < 8u; q++)
+                row[q] = i + 1;
     }
 }
 
@@ -519,7 +544,7 @@ constexpr u32 SORT_DEV_BLOCKS = 2048;
 hipError_t launch_sort(const phx_float4 *pos_in, const u32 *pow_in, const u32 *dir_in, const u32 *hashes,
                        const u32 *cell_start, u32 *cell_end, u32 n, u32 n_cells, phx_float4 *pos_out, u32 *pow_out,
                        u32 *dir_out, void *ws, hipStream_t s, const u32 *d_n, u32 *perm_out,
-                       u32 *cell_start_out) {
+                       u32 *cell_start_out, u32 *oct_start_out) {
     hipError_t e;
     const u32 *sorted_keys = hashes; /* n == 0: nothing is read */
     if (n > 0) {
@@ -549,7 +574,7 @@ hipError_t launch_sort(const phx_float4 *pos_in, const u32 *pow_in, const u32 *d
             if ((e = hipGetLastError()) != hipSuccess)
                 return e;
         }
-        u32 passes = (key_bits(n_cells) + RADIX_BITS - 1) / RADIX_BITS;
+        u32 passes = (key_bits(oct_start_out ? n_cells * 8u : n_cells) + RADIX_BITS - 1) / RADIX_BITS;
         const u32 *kin = hashes;
         const u32 *vin = nullptr;
         for (u32 pass = 0; pass < passes; pass++) {
@@ -563,7 +588,8 @@ hipError_t launch_sort(const phx_float4 *pos_in, const u32 *pow_in, const u32 *d
                 return e;
             u32 *kout = kbuf[pass & 1], *vout = vbuf[pass & 1];
             hipLaunchKernelGGL(k_radix_scatter, dim3(grid), dim3(SORT_BLOCK), 0, s, kin, vin, n, d_n, shift, offs,
-                               kout, vout, last ? (cell_start_out ? 2 : 1) : 0, pos_in, pow_in, dir_in, pos_out,
+                               kout, vout, last ? (cell_start_out || oct_start_out ? 2 : 1) : 0, pos_in, pow_in,
+                               dir_in, pos_out,
                                pow_out, dir_out,
                                perm_out);
             if ((e = hipGetLastError()) != hipSuccess)
@@ -578,10 +604,18 @@ hipError_t launch_sort(const phx_float4 *pos_in, const u32 *pow_in, const u32 *d
         if (blocks > 4096)
             blocks = 4096;
         hipLaunchKernelGGL(k_cell_start_search, dim3(blocks), dim3(256), 0, s, sorted_keys, n, d_n, n_cells,
-                           cell_start_out);
+                           cell_start_out, oct_start_out ? 3u : 0u);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
         cell_start = cell_start_out;
+    }
+    if (oct_start_out && n > 0) {
+        u32 blocks = (n + 255) / 256;
+        if (blocks > 8192)
+            blocks = 8192;
+        hipLaunchKernelGGL(k_octant_starts, dim3(blocks), dim3(256), 0, s, sorted_keys, n, d_n, oct_start_out);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
     }
     if (n_cells > 0) {
         u32 blocks = (n_cells + 255) / 256;
@@ -657,13 +691,13 @@ hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, 
                          u32 n_pixels, const phx_float4 *ppos, const u32 *ppow, const u32 *pdir, const u32 *cell_start,
                          const u32 *cell_end, const PhxPhotonHashData &d, f32 alpha, phx_float4 *flux,
                          f32 *photon_weight, phx_float4 *radiance, u32 width, hipStream_t s, f32 *rec_ratio,
-                         f32 *rec_r2, const phx_float4 *pdirpow) {
+                         f32 *rec_r2, const phx_float4 *pdirpow, const u32 *oct_start) {
     if (n_pixels == 0)
         return hipSuccess;
     GatherArgs a{hp_pos,     hp_normal, hp_throughput, hp_base_color, hp_direct, hp_emission,
                  n_pixels,   ppos,      ppow,          pdir,          cell_start, cell_end,
                  d,          alpha,     flux,          photon_weight, radiance,  rec_ratio,
-                 rec_r2,     pdirpow};
+                 rec_r2,     pdirpow,   pdirpow && width ? oct_start : nullptr};
     bool dec = pdirpow != nullptr;
     if (width == 0) { /* image shape unknown (C ABI call): flat pixel range */
         if (dec)

@@ -505,6 +505,11 @@ struct GatherArgs {
     /* optional: every stored record's direction already decoded (decode_oct
        of pdir, the same values) with its packed power (ppow) in w */
     const phx_float4 *pdirpow;
+    /* optional (records sorted by cell, then half-cell octant): the first
+       record of each octant of every non-empty cell, 8 per cell
+       (k_octant_starts); gather_pixel_flat then visits only the octants
+       within reach */
+    const u32 *oct_start;
 };
 
 /* photons of a cell whose records are loaded together before any is used:
@@ -632,6 +637,26 @@ PHX_DEV u32 query_cell(const PhxPhotonHashData &d, F4 pos, f32 r_sq, u32 i, F4 m
     return phx_cull_far(gx, gy, gz, r_sq) ? 0u : h;
 }
 
+/* Octants of a cell (low corner l) that may hold a photon within the radius:
+ * bit x + 2y + 4z for the half x, y, z (0 low, 1 high).  A record is in the
+ * high half of an axis when its quotient's fraction is at least 0.5
+ * (get_hash_octant), so it lies in [l + cs/2, l + cs] up to the same
+ * roundings as the cell's own edges plus one for the half-cell plane, which
+ * the per-axis margin m covers as well (phx_cull.h); an octant is skipped only
+ * when its box is provably farther than r, so the sums are unchanged. */
+PHX_DEV u32 octant_mask(const PhxPhotonHashData &d, F4 pos, f32 r_sq, F4 l, F4 m) {
+    f32 mx = l.x + d.cell_sizes.x * 0.5f, my = l.y + d.cell_sizes.y * 0.5f, mz = l.z + d.cell_sizes.z * 0.5f;
+    f32 gx0 = phx_cull_gap(pos.x, l.x, mx, m.x), gx1 = phx_cull_gap(pos.x, mx, l.x + d.cell_sizes.x, m.x);
+    f32 gy0 = phx_cull_gap(pos.y, l.y, my, m.y), gy1 = phx_cull_gap(pos.y, my, l.y + d.cell_sizes.y, m.y);
+    f32 gz0 = phx_cull_gap(pos.z, l.z, mz, m.z), gz1 = phx_cull_gap(pos.z, mz, l.z + d.cell_sizes.z, m.z);
+    u32 mask = 0;
+#pragma unroll
+    for (u32 q = 0; q < 8; q++)
+        if (!phx_cull_far((q & 1u) ? gx1 : gx0, (q & 2u) ? gy1 : gy0, (q & 4u) ? gz1 : gz0, r_sq))
+            mask |= 1u << q;
+    return mask;
+}
+
 /* per-axis culling margins for a hitpoint (phx_cull_margin) */
 PHX_DEV F4 cull_margins(const PhxPhotonHashData &d, F4 pos) {
     return mk(phx_cull_margin(d.origin.x, pos.x, d.cell_sizes.x, d.grid_res),
@@ -717,58 +742,187 @@ PHX_DEV void gather_pixel(const GatherArgs &a, u32 tid, const PhxPhotonHashData 
     gather_end(a, tid, alpha, fsf, wsf, r_sq, nf, found);
 }
 
+/* The 27-cell query of one hitpoint worked out once (photon_hash.h:65-100).
+ * The query point of cell (cx, cy, cz) ∈ {−1, 0, 1}³ is pos + (cx·cs.x,
+ * cy·cs.y, cz·cs.z), and get_hash's quotient along an axis depends only on
+ * that axis' offset, so 9 quotients (instead of 27 × 3) give every cell's
+ * indices — idx[a] holds the 3 indices of axis a in 9-bit fields, 511 for a
+ * query point outside the grid — and, per axis and offset, the squared gaps of
+ * the cell's low and high half (octant_mask's boxes and margins) give the
+ * reach of all 27 × 8 octants: bit 8·(i mod 4) + s of word i / 4 is set when
+ * octant s of cell i = (cx+1) + 3(cy+1) + 9(cz+1) may hold a photon within
+ * r (all 8 bits of a reached cell when there are no octant starts).  Scanning
+ * the bits in order visits cells in query order and octants in storage
+ * order: the order of the sums. */
+struct QueryReach {
+    u32 idx[3];
+    u32 b0, b1, b2, b3, b4, b5, b6;
+
+    __device__ __forceinline__ u32 word(u32 wi) const {
+        u32 w = b6;
+        w = wi == 5 ? b5 : w;
+        w = wi == 4 ? b4 : w;
+        w = wi == 3 ? b3 : w;
+        w = wi == 2 ? b2 : w;
+        w = wi == 1 ? b1 : w;
+        return wi == 0 ? b0 : w;
+    }
+};
+
+/* one axis of QueryReach: the indices of the three query points' cells and
+ * the squared half-cell gaps (+inf outside the grid) */
+PHX_DEV void axis_reach(f32 p, f32 o, f32 cs, u32 g, f32 m, u32 &packed, f32 (&sq)[3][2]) {
+    packed = 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        f32 q = p + (f32)(c - 1) * cs;
+        f32 f = (q - o) / cs;
+        bool out = f < 0.0f || f >= (f32)g;
+        u32 i = out ? 0u : (u32)f;
+        f32 l = o + (f32)i * cs;
+        f32 mid = l + cs * 0.5f;
+        f32 g0 = phx_cull_gap(p, l, mid, m), g1 = phx_cull_gap(p, mid, l + cs, m);
+        sq[c][0] = out ? __builtin_inff() : g0 * g0;
+        sq[c][1] = out ? __builtin_inff() : g1 * g1;
+        packed |= (out ? 511u : i) << (9 * c);
+    }
+}
+
+PHX_DEV QueryReach query_reach(const PhxPhotonHashData &d, F4 pos, f32 r_sq, bool octants) {
+    QueryReach R;
+    const F4 m = cull_margins(d, pos);
+    f32 sx[3][2], sy[3][2], sz[3][2];
+    axis_reach(pos.x, d.origin.x, d.cell_sizes.x, d.grid_res, m.x, R.idx[0], sx);
+    axis_reach(pos.y, d.origin.y, d.cell_sizes.y, d.grid_res, m.y, R.idx[1], sy);
+    axis_reach(pos.z, d.origin.z, d.cell_sizes.z, d.grid_res, m.z, R.idx[2], sz);
+    const f32 lim = r_sq * 1.00001f; /* phx_cull_far */
+    u32 w[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (u32 i = 0; i < 27; i++) {
+        const u32 cx = i % 3, cy = (i / 3) % 3, cz = i / 9;
+        u32 byte = 0;
+#pragma unroll
+        for (u32 q = 0; q < 8; q++) {
+            f32 sum = (sx[cx][q & 1u] + sy[cy][(q >> 1) & 1u]) + sz[cz][q >> 2];
+            byte |= (sum >= lim ? 0u : 1u) << q; /* NaN keeps the octant */
+        }
+        if (!octants)
+            byte = byte ? 0xFFu : 0u;
+        w[i / 4] |= byte << (8 * (i % 4));
+    }
+    R.b0 = w[0], R.b1 = w[1], R.b2 = w[2], R.b3 = w[3], R.b4 = w[4], R.b5 = w[5], R.b6 = w[6];
+    return R;
+}
+
 /* gather_pixel<true> with the cell loop and the record loop flattened into
- * one: each iteration first moves every lane whose cell is used up to its
- * next cell (query_cell, skipping empty and distant ones), then processes one
- * batch of records of every lane's current cell.  A lane that leaves a short
- * cell goes on at once instead of idling until the wave's longest cell is
- * done (3.45 -> 3.1 ms on the caustic workload).  Every pixel's cells and
- * records are summed in gather_pixel's order: same bits.  Every lane of the
- * wave calls it (valid = false for lanes without a pixel).  Handing pixels
- * out dynamically (a lane that finishes takes another pixel of its block)
- * was measured slower: the lanes of a wave then query unrelated cells and
- * stop sharing record loads. */
+ * one: each iteration first moves every lane whose record range is used up to
+ * its next range, then processes one batch of records of every lane's
+ * current range.  A lane that leaves a short range goes on at once instead of
+ * idling until the wave's longest one is done (3.45 -> 3.1 ms on the caustic
+ * workload).  With octant starts (a.oct_start) the ranges are the runs of
+ * consecutive octants of a cell that lie within reach (octant_mask), visited
+ * in octant order; without, whole cells.  Either way a pixel's accepted
+ * photons are summed in gather_pixel's order (cells in query order, records
+ * in storage order; skipped octants hold no photon the distance test
+ * accepts): same bits.  Every lane of the wave calls it (valid = false for
+ * lanes without a pixel).  Handing pixels out dynamically (a lane that
+ * finishes takes another pixel of its block) was measured slower: the lanes
+ * of a wave then query unrelated cells and stop sharing record loads. */
 PHX_DEV void gather_pixel_flat(const GatherArgs &a, bool valid, u32 tid, const PhxPhotonHashData &d, f32 alpha,
                                u32 bucket_base) {
     const u32 *cs = a.cell_start + bucket_base, *ce = a.cell_end + bucket_base;
+    const u32 *os = a.oct_start ? a.oct_start + (size_t)bucket_base * 8u : nullptr;
     F4 pos = ZERO, sn = ZERO, fsf = ZERO, nf = BLACK;
     f32 wsf = 0.0f, r_sq = 0.0f, found = 0.0f;
-    /* ci: next cell of the 27 (27: none left); [k, e): records of the current cell */
-    u32 ci = 27, k = 0, e = 0;
+    /* R: the cells and octants within reach; wi, w: the word of R being
+       scanned and its bits not yet visited (more: bits left); cell, h,
+       [c0, c1): the cell being visited, its bucket and records; [k, e):
+       records of the current range, [k2, e2): the range after it (fetched
+       when fewer than a batch of the current one are left, so that one batch
+       can take records from both) */
+    u32 wi = 0, w = 0, cell = 27, h = 0, c0 = 0, c1 = 0, k = 0, e = 0, k2 = 0, e2 = 0;
+    bool more = false;
     if (valid && gather_begin(a, tid, d, pos, sn, fsf, wsf, r_sq))
-        ci = 0;
-    const bool px = ci == 0;
+        more = true;
+    const bool px = more;
     const f32 two_r_sq = 2.0f * r_sq, inv_two_r_sq = px ? 1.0f / two_r_sq : 0.0f, wz = 0.0f * sn.w;
-    const F4 m = cull_margins(d, pos);
+    const QueryReach R = query_reach(d, pos, r_sq, os != nullptr);
+    const u32 g = d.grid_res;
+    w = px ? R.b0 : 0u;
+    constexpr u32 B = GATHER_BATCH_DECODED;
     for (;;) {
-        while (k >= e && ci < 27) {
-            F4 l;
-            u32 h = query_cell(d, pos, r_sq, ci, m, l);
-            ci++;
-            if (h != 0) {
-                k = cs[h];
-                e = ce[h];
+        for (;;) {
+            if (k >= e && k2 < e2) {
+                k = k2;
+                e = e2;
+                k2 = e2 = 0;
+            }
+            bool need = k >= e || (e - k < B && k2 >= e2);
+            if (!need || !more)
+                break;
+            while (w == 0 && wi < 6) {
+                wi++;
+                w = R.word(wi);
+            }
+            if (w == 0) {
+                more = false;
+                break;
+            }
+            /* the run of consecutive octants [q0, q1) of cell i from the lowest bit left */
+            u32 bit = (u32)__builtin_ctz(w);
+            u32 cb = bit >> 3;
+            u32 i = wi * 4u + cb;
+            u32 q0 = bit & 7u;
+            u32 byte = (w >> (cb * 8u)) & 0xFFu;
+            u32 q1 = q0 + (u32)__builtin_ctz(~(byte >> q0));
+            w &= ~(((1u << (q1 - q0)) - 1u) << bit);
+            if (i != cell) {
+                cell = i;
+                u32 ix = (R.idx[0] >> (9u * (i % 3u))) & 511u;
+                u32 iy = (R.idx[1] >> (9u * ((i / 3u) % 3u))) & 511u;
+                u32 iz = (R.idx[2] >> (9u * (i / 9u))) & 511u;
+                h = g * g * ix + g * iy + iz + 1u; /* get_hash of the query point */
+                c0 = cs[h];
+                c1 = ce[h];
+                if (c0 >= c1) { /* empty: its octant starts are not written */
+                    w &= ~(0xFFu << (cb * 8u));
+                    continue;
+                }
+            }
+            u32 rk = q0 == 0 ? c0 : os[(size_t)h * 8u + q0];
+            u32 re = q1 == 8 ? c1 : os[(size_t)h * 8u + q1];
+            if (rk < re) {
+                if (k >= e) {
+                    k = rk;
+                    e = re;
+                } else {
+                    k2 = rk;
+                    e2 = re;
+                }
             }
         }
         if (!__any(k < e))
             break;
-        /* one batch of the current cell (gather_cell<true>'s loop body) */
-        F4 pp[GATHER_BATCH_DECODED], pd[GATHER_BATCH_DECODED];
-        bool work = k < e;
-        u32 kl = work ? e - 1 : 0;
+        /* one batch: the next records of the current range, then of the next */
+        u32 n1 = k < e ? (e - k < B ? e - k : B) : 0u;
+        u32 n2 = (n1 < B && k2 < e2) ? (e2 - k2 < B - n1 ? e2 - k2 : B - n1) : 0u;
+        u32 nt = n1 + n2;
+        F4 pp[B], pd[B];
+        bool work = nt > 0;
+        u32 kl = n2 ? k2 + n2 - 1 : k + n1 - 1;
 #pragma unroll
-        for (u32 q = 0; q < GATHER_BATCH_DECODED; q++) {
-            u32 kk = k + q < e ? k + q : kl;
+        for (u32 q = 0; q < B; q++) {
+            u32 kk = q < n1 ? k + q : (q < nt ? k2 + (q - n1) : kl);
             pp[q] = work ? ld(a.ppos + kk) : ZERO;
             pd[q] = work ? ld(a.pdirpow + kk) : ZERO;
         }
 #pragma unroll
-        for (u32 q = 0; q < GATHER_BATCH_DECODED; q++)
+        for (u32 q = 0; q < B; q++)
             keep_loaded(pd[q]);
         if (work) {
 #pragma unroll
-            for (u32 q = 0; q < GATHER_BATCH_DECODED; q++) {
-                if (k + q >= e)
+            for (u32 q = 0; q < B; q++) {
+                if (q >= nt)
                     break;
                 F4 dd = pos - pp[q];
                 f32 dsq = dot(dd, dd);
@@ -780,7 +934,8 @@ PHX_DEV void gather_pixel_flat(const GatherArgs &a, bool valid, u32 tid, const P
                 nf = nf + wgt * decode_rgbe(phx_as_uint(pd[q].w));
                 found += wgt;
             }
-            k = k + GATHER_BATCH_DECODED < e ? k + GATHER_BATCH_DECODED : e;
+            k += n1;
+            k2 += n2;
         }
     }
     if (px)

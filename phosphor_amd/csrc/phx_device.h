@@ -272,6 +272,24 @@ PHX_DEV u32 get_hash(f32 px, f32 py, f32 pz, const PhxPhotonHashData &d) {
     return g * g * x + g * y + z + 1;
 }
 
+/* get_hash with the half-cell octant of the point inside its cell:
+ * 8·get_hash + (x half) + 2·(y half) + 4·(z half), a half being 1 where the
+ * quotient's fraction f − ⌊f⌋ (exact for the same rounded quotient) is at
+ * least 0.5; 0 outside the grid.  Sorting by this key keeps every cell's
+ * records contiguous (the cell ranges are unchanged) and orders them by
+ * octant inside the cell, so that the gather can skip octants. */
+PHX_DEV u32 get_hash_octant(f32 px, f32 py, f32 pz, const PhxPhotonHashData &d) {
+    u32 g = d.grid_res;
+    f32 fx = (px - d.origin.x) / d.cell_sizes.x;
+    f32 fy = (py - d.origin.y) / d.cell_sizes.y;
+    f32 fz = (pz - d.origin.z) / d.cell_sizes.z;
+    if (fx < 0.0f || fy < 0.0f || fz < 0.0f || fx >= (f32)g || fy >= (f32)g || fz >= (f32)g)
+        return 0;
+    u32 x = (u32)fx, y = (u32)fy, z = (u32)fz;
+    u32 o = (fx - (f32)x >= 0.5f ? 1u : 0u) | (fy - (f32)y >= 0.5f ? 2u : 0u) | (fz - (f32)z >= 0.5f ? 4u : 0u);
+    return (g * g * x + g * y + z + 1) * 8u + o;
+}
+
 /* scene pointers of one render (kernel argument block) */
 typedef SceneArgs SceneView;
 

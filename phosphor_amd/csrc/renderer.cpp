@@ -123,6 +123,7 @@ int PhxRenderer::set_scene(const PhxSceneDesc &sc) {
     PHX_TRY(p.d_photon_dirpow.alloc((size_t)photon_buf_size * sizeof(phx_float4)), "alloc photon map");
     PHX_TRY(p.d_cell_start.alloc((size_t)bucket_count * sizeof(uint32_t)), "alloc cells");
     PHX_TRY(p.d_cell_end.alloc((size_t)bucket_count * sizeof(uint32_t)), "alloc cells");
+    PHX_TRY(p.d_oct_start.alloc((size_t)bucket_count * 8u * sizeof(uint32_t)), "alloc cells");
     /* --batch-size (renderer.cpp:103-111) bounds the reference's device memory;
        deposits do not depend on it here (canonical order), so the whole round
        is emitted in one launch sequence whenever its workspace fits
@@ -157,12 +158,12 @@ void PhxRenderer::hp_bufs(int set, DevBuf *out[6]) {
         out[k] = set ? &alt_hp[k] : b[k];
 }
 
-/* the sorted photon map of a set: pos, power, dir, dirpow, cell_start, cell_end */
-void PhxRenderer::map_bufs(int set, DevBuf *out[6]) {
+/* the sorted photon map of a set: pos, power, dir, dirpow, cell_start, cell_end, oct_start */
+void PhxRenderer::map_bufs(int set, DevBuf *out[7]) {
     PhotonBuffers &p = buffers.photon_buf;
-    DevBuf *b[6] = {&p.d_photon_pos, &p.d_photon_power, &p.d_photon_dir, &p.d_photon_dirpow, &p.d_cell_start,
-                    &p.d_cell_end};
-    for (int k = 0; k < 6; k++)
+    DevBuf *b[7] = {&p.d_photon_pos, &p.d_photon_power, &p.d_photon_dir, &p.d_photon_dirpow, &p.d_cell_start,
+                    &p.d_cell_end,   &p.d_oct_start};
+    for (int k = 0; k < 7; k++)
         out[k] = set ? &alt_map[k] : b[k];
 }
 
@@ -206,7 +207,7 @@ int PhxRenderer::emit_photons(hipStream_t s) {
 /* renderer.cpp:123-153, sized by the device-side count */
 int PhxRenderer::hash_and_sort_photons(hipStream_t s, int set) {
     PhotonBuffers &p = buffers.photon_buf;
-    DevBuf *m[6];
+    DevBuf *m[7];
     map_bufs(set, m);
     const uint32_t *d_n = d_batch_size.as<uint32_t>();
     uint32_t n_max = photon_buf_size;
@@ -214,12 +215,12 @@ int PhxRenderer::hash_and_sort_photons(hipStream_t s, int set) {
        (the scan of the cell counts, renderer.cpp:134-147) are read off the
        sorted keys */
     PHX_TRY(launch_hash_dev(d_photon_pos.as<phx_float4>(), d_n, n_max, photon_hash_info,
-                            p.d_photon_hashes.as<uint32_t>(), nullptr, s),
+                            p.d_photon_hashes.as<uint32_t>(), nullptr, s, true),
             "hash_photons");
     PHX_TRY(launch_sort(d_photon_pos.as<phx_float4>(), d_photon_power.as<uint32_t>(), d_photon_dir.as<uint32_t>(),
                         p.d_photon_hashes.as<uint32_t>(), nullptr, m[5]->as<uint32_t>(), n_max, bucket_count,
                         m[0]->as<phx_float4>(), m[1]->as<uint32_t>(), m[2]->as<uint32_t>(), d_sort_ws.p, s, d_n,
-                        sort_perm, m[4]->as<uint32_t>()),
+                        sort_perm, m[4]->as<uint32_t>(), m[6]->as<uint32_t>()),
             "sort_photons");
     PHX_TRY(launch_decode_photons(m[1]->as<uint32_t>(), m[2]->as<uint32_t>(), d_n, n_max, m[3]->as<phx_float4>(), s),
             "decode_photons");
@@ -229,7 +230,7 @@ int PhxRenderer::hash_and_sort_photons(hipStream_t s, int set) {
 /* renderer.cpp:155-161 */
 int PhxRenderer::gather_flux(hipStream_t s, int set) {
     const ImageBuffers &i = buffers.image_buf;
-    DevBuf *h[6], *m[6];
+    DevBuf *h[6], *m[7];
     hp_bufs(set, h);
     map_bufs(set, m);
     return hip_status(launch_gather(h[0]->as<phx_float4>(), h[1]->as<phx_float4>(), h[2]->as<phx_float4>(),
@@ -238,7 +239,7 @@ int PhxRenderer::gather_flux(hipStream_t s, int set) {
                                     m[4]->as<uint32_t>(), m[5]->as<uint32_t>(), photon_hash_info, args.sppm_alpha,
                                     i.d_flux.as<phx_float4>(), i.d_photon_weight.as<float>(),
                                     i.d_radiance.as<phx_float4>(), i.image_width, s, rec_ratio, rec_r2,
-                                    m[3]->as<phx_float4>()),
+                                    m[3]->as<phx_float4>(), m[6]->as<uint32_t>()),
                       "gather_flux");
 }
 
@@ -247,12 +248,14 @@ int PhxRenderer::gather_flux(hipStream_t s, int set) {
 int PhxRenderer::alloc_alt() {
     if (alt_ready)
         return PHX_OK;
-    DevBuf *h[6], *m[6];
+    DevBuf *h[6], *m[7];
     hp_bufs(0, h);
     map_bufs(0, m);
     for (int k = 0; k < 6; k++) {
         PHX_TRY(alt_hp[k].alloc(h[k]->bytes), "alloc second hitpoint set");
         PHX_TRY(hipMemsetAsync(alt_hp[k].p, 0, alt_hp[k].bytes, stream), "clear second hitpoint set");
+    }
+    for (int k = 0; k < 7; k++) {
         PHX_TRY(alt_map[k].alloc(m[k]->bytes), "alloc second photon map");
         PHX_TRY(hipMemsetAsync(alt_map[k].p, 0, alt_map[k].bytes, stream), "clear second photon map");
     }
@@ -262,13 +265,13 @@ int PhxRenderer::alloc_alt() {
 
 /* after a pipelined pass: its set becomes the one in `buffers` */
 void PhxRenderer::swap_sets() {
-    DevBuf *h[6], *m[6];
+    DevBuf *h[6], *m[7];
     hp_bufs(0, h);
     map_bufs(0, m);
-    for (int k = 0; k < 6; k++) {
+    for (int k = 0; k < 6; k++)
         h[k]->swap(alt_hp[k]);
+    for (int k = 0; k < 7; k++)
         m[k]->swap(alt_map[k]);
-    }
     std::swap(ev_read[0], ev_read[1]);
     std::swap(read_pending[0], read_pending[1]);
 }

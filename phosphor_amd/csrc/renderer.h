@@ -16,6 +16,7 @@ struct PhotonBuffers { /* include/buffers/photon.hpp (sorted photon map) */
     DevBuf d_photon_pos, d_photon_power, d_photon_dir, d_photon_hashes;
     DevBuf d_photon_dirpow; /* each record's decoded direction, packed power in w */
     DevBuf d_cell_start, d_cell_end; /* cell counts are not kept: ranges come from the sorted keys */
+    DevBuf d_oct_start;              /* 8 octant starts per cell (records sorted by cell, then octant) */
     uint32_t n_photons = 0;
 };
 
@@ -69,7 +70,7 @@ struct PhxRenderer {
     uint32_t seed_slot = 0;
     bool alt_ready = false;
     DevBuf alt_hp[6];  /* pos, normal, throughput, base colour, direct, emission */
-    DevBuf alt_map[6]; /* pos, power, dir, dirpow, cell_start, cell_end */
+    DevBuf alt_map[7]; /* pos, power, dir, dirpow, cell_start, cell_end, oct_start */
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     uint32_t *h_info = nullptr; /* pinned: last deposit count, passes without deposits */
@@ -124,7 +125,7 @@ struct PhxRenderer {
     int enqueue_pass(bool events, bool pipe);
     int alloc_alt();
     void hp_bufs(int set, DevBuf *out[6]);
-    void map_bufs(int set, DevBuf *out[6]);
+    void map_bufs(int set, DevBuf *out[7]);
     void swap_sets();
     hipError_t sync_all();
     int capture_pass();

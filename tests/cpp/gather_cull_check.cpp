@@ -10,7 +10,11 @@
  * distance is computed as the gather computes it (dd = pos − p per
  * component, ((dx² + dy²) + dz²) + dw²), and for every accepted photon the
  * cell test (phx_cull_far on the cell box with phx_cull_margin) must keep
- * it.  Non-finite hitpoints must keep every cell.
+ * it, and so must the octant test of the gather's octant sort
+ * (pass_bodies.h octant_mask: the half-cell box of the photon's octant, whose
+ * bit per axis is fraction(f) ≥ 0.5 of get_hash_octant's quotient f, with the
+ * half-cell plane l + cs·0.5 and the same margins).  Non-finite hitpoints
+ * must keep every cell.
  *
  * The same trials with a fixed margin of 1e-4 cell and no relative slack
  * (mode "fixed") must find violations: that shows the trials reach the
@@ -37,19 +41,21 @@ struct Grid {
     uint32_t g;
 };
 
-/* get_hash's cell of p (photon_hash.h:21-37); false outside the grid */
-static bool hash_cell(const Grid &G, const float p[3], uint32_t c[3]) {
+/* get_hash's cell of p (photon_hash.h:21-37) and, per axis, whether p lies
+ * in the cell's high half (get_hash_octant); false outside the grid */
+static bool hash_cell(const Grid &G, const float p[3], uint32_t c[3], bool hi[3]) {
     for (int a = 0; a < 3; a++) {
         float f = (p[a] - G.o[a]) / G.cs[a];
         if (f < 0.0f || f >= (float)G.g)
             return false;
         c[a] = (uint32_t)f;
+        hi[a] = f - (float)c[a] >= 0.5f;
     }
     return true;
 }
 
 struct Counts {
-    uint64_t trials = 0, accepted = 0, near = 0, cell_fail = 0, nan_fail = 0;
+    uint64_t trials = 0, accepted = 0, near = 0, cell_fail = 0, oct_fail = 0, nan_fail = 0;
 };
 
 static void run(uint64_t seed, uint64_t trials, bool fixed, Counts &out) {
@@ -87,7 +93,8 @@ static void run(uint64_t seed, uint64_t trials, bool fixed, Counts &out) {
             p[a] = (float)((double)G.o[a] + (x + u) * (double)G.cs[a]);
         }
         uint32_t cell[3];
-        if (!hash_cell(G, p, cell))
+        bool hi[3];
+        if (!hash_cell(G, p, cell, hi))
             continue;
         /* radius from a thousandth of a cell to two cells */
         double cmax = std::fmax(G.cs[0], std::fmax(G.cs[1], G.cs[2]));
@@ -133,6 +140,16 @@ static void run(uint64_t seed, uint64_t trials, bool fixed, Counts &out) {
             c.near++;
         if (far)
             c.cell_fail++;
+        /* the octant that holds the photon, as octant_mask bounds it */
+        float og[3];
+        for (int a = 0; a < 3; a++) {
+            float mid = l[a] + G.cs[a] * 0.5f;
+            og[a] = hi[a] ? phx_cull_gap(pos[a], mid, l[a] + G.cs[a], m[a]) : phx_cull_gap(pos[a], l[a], mid, m[a]);
+        }
+        bool ofar = fixed ? (og[0] * og[0] + og[1] * og[1] + og[2] * og[2] >= r_sq)
+                          : phx_cull_far(og[0], og[1], og[2], r_sq);
+        if (ofar)
+            c.oct_fail++;
         /* a hitpoint that is not finite keeps every cell */
         if (t % 64 == 0 && !fixed) {
             float nanp = std::nanf("");
@@ -161,15 +178,16 @@ int main(int argc, char **argv) {
         s.accepted += c.accepted;
         s.near += c.near;
         s.cell_fail += c.cell_fail;
+        s.oct_fail += c.oct_fail;
         s.nan_fail += c.nan_fail;
     }
-    uint64_t fails = s.cell_fail + s.nan_fail;
+    uint64_t fails = s.cell_fail + s.oct_fail + s.nan_fail;
     if (fixed) {
         printf("fixed %" PRIu64 " violations in %" PRIu64 " accepted\n", fails, s.accepted);
         return fails > 0 ? 0 : 1;
     }
     printf("%s trials %" PRIu64 " accepted %" PRIu64 " outside-cell %" PRIu64 " cell_fail %" PRIu64
-           " nan_fail %" PRIu64 "\n",
-           fails ? "FAIL" : "ok", s.trials, s.accepted, s.near, s.cell_fail, s.nan_fail);
+           " octant_fail %" PRIu64 " nan_fail %" PRIu64 "\n",
+           fails ? "FAIL" : "ok", s.trials, s.accepted, s.near, s.cell_fail, s.oct_fail, s.nan_fail);
     return fails ? 1 : 0;
 }

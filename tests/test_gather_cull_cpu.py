@@ -1,5 +1,5 @@
-"""The gather's cell culling never skips a cell holding a photon that the
-distance test accepts: random grids up to 4096³ with origins anywhere in
+"""The gather's cell and octant culling never skips a cell or a half-cell
+octant holding a photon that the distance test accepts: random grids up to 4096³ with origins anywhere in
 ±1024, photons at cell edges, hitpoints at distance ≈ r
 (tests/cpp/gather_cull_check.cpp).  The same trials with a fixed margin of
 1e-4 cell must find violations, so the trials reach the cases the margin is
