@@ -924,15 +924,16 @@ PHX_DEV void gather_pixel_flat(const GatherArgs &a, bool valid, u32 tid, const P
             for (u32 q = 0; q < B; q++) {
                 if (q >= nt)
                     break;
+                /* photon_hash.h:84-98: within the radius and facing the hitpoint;
+                   one branch, so the sums have one merge point */
                 F4 dd = pos - pp[q];
                 f32 dsq = dot(dd, dd);
-                if (dsq >= r_sq)
-                    continue;
-                if (((pd[q].x * sn.x + pd[q].y * sn.y) + pd[q].z * sn.z) + wz > 0.0f)
-                    continue;
-                f32 wgt = gauss_weight(dsq, two_r_sq, inv_two_r_sq);
-                nf = nf + wgt * decode_rgbe(phx_as_uint(pd[q].w));
-                found += wgt;
+                bool acc = dsq < r_sq && !(((pd[q].x * sn.x + pd[q].y * sn.y) + pd[q].z * sn.z) + wz > 0.0f);
+                if (acc) {
+                    f32 wgt = gauss_weight(dsq, two_r_sq, inv_two_r_sq);
+                    nf = nf + wgt * decode_rgbe_ldexp(phx_as_uint(pd[q].w));
+                    found += wgt;
+                }
             }
             k += n1;
             k2 += n2;

@@ -202,6 +202,15 @@ PHX_DEV F4 decode_rgbe(u32 rgbe) {
     f32 scale = phx_ldexp1((i32)E - 128 - 8);
     return mk((f32)(rgbe & 0xFF) * scale, (f32)((rgbe >> 8) & 0xFF) * scale, (f32)((rgbe >> 16) & 0xFF) * scale, 0.0f);
 }
+/* decode_rgbe with the hardware's exact ldexp: c·2^(E−136) is a multiple of
+ * 2^-149 no larger than 255·2^119, representable exactly, so both forms give
+ * the same bits (E = 0 decodes to zero) */
+PHX_DEV F4 decode_rgbe_ldexp(u32 rgbe) {
+    u32 E = rgbe >> 24;
+    i32 k = E == 0 ? -200 : (i32)E - 136;
+    return mk(__builtin_amdgcn_ldexpf((f32)(rgbe & 0xFF), k), __builtin_amdgcn_ldexpf((f32)((rgbe >> 8) & 0xFF), k),
+              __builtin_amdgcn_ldexpf((f32)((rgbe >> 16) & 0xFF), k), 0.0f);
+}
 PHX_DEV u32 encode_oct(F4 dir) {
     dir = normalize(dir);
     f32 l = phx_fabsf(dir.x) + phx_fabsf(dir.y) + phx_fabsf(dir.z);

@@ -187,18 +187,21 @@ PHX_HD float phx_cosf(float x) {
 #define PHX_LN2_2 3.194618329871446e-05f  /* 0x1.0bfbe8p-15 */
 
 /* exp(x) = 2^k · p(r), k = nearest integer to x/ln2, r = x − k·ln2 (two-part
- * ln2), p the degree-7 Taylor polynomial; returns k, p in *p_out. */
+ * ln2, each step one fused multiply-add), p the degree-7 Taylor polynomial in
+ * Horner form with fused multiply-adds (one rounding per step; an explicit
+ * fma is IEEE on both the host and gfx950, so both give the same bits);
+ * returns k, p in *p_out. */
 PHX_HD int phx_exp_reduce(float x, float *p_out) {
-    float k = phx_floorf(x * PHX_LOG2E + 0.5f);
-    float r = (x - k * PHX_LN2_1) - k * PHX_LN2_2;
-    float p = 1.984126984e-4f;          /* 1/5040 */
-    p = 1.388888889e-3f + r * p;        /* 1/720 */
-    p = 8.333333333e-3f + r * p;        /* 1/120 */
-    p = 4.166666667e-2f + r * p;        /* 1/24 */
-    p = 1.666666667e-1f + r * p;        /* 1/6 */
-    p = 0.5f + r * p;
-    p = 1.0f + r * p;
-    p = 1.0f + r * p;
+    float k = phx_floorf(__builtin_fmaf(x, PHX_LOG2E, 0.5f));
+    float r = __builtin_fmaf(-k, PHX_LN2_2, __builtin_fmaf(-k, PHX_LN2_1, x));
+    float p = 1.984126984e-4f;                  /* 1/5040 */
+    p = __builtin_fmaf(r, p, 1.388888889e-3f); /* 1/720 */
+    p = __builtin_fmaf(r, p, 8.333333333e-3f); /* 1/120 */
+    p = __builtin_fmaf(r, p, 4.166666667e-2f); /* 1/24 */
+    p = __builtin_fmaf(r, p, 1.666666667e-1f); /* 1/6 */
+    p = __builtin_fmaf(r, p, 0.5f);
+    p = __builtin_fmaf(r, p, 1.0f);
+    p = __builtin_fmaf(r, p, 1.0f);
     *p_out = p;
     return (int)k;
 }
