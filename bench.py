@@ -78,6 +78,69 @@ def read_traffic(kernel: str, workload: str):
     return float(m.group(1)) if m else None
 
 
+# VALU issue peak of one MI355X: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64
+# instruction per SIMD every 4 cycles
+VALU_PEAK_G = 256 * 4 * 2.4 / 4  # G wave-instructions/s (614.4)
+
+
+def read_valu(kernel: str, workload: str):
+    """VALU wave-instructions per launch from the committed SQ counter summary."""
+    f = ROOT / "profiles" / f"{kernel}_{workload}_sq.txt"
+    if not f.exists():
+        return None
+    m = re.search(r"valu_instructions_per_launch\s*[:=]\s*([0-9.eE+]+)", f.read_text())
+    return float(m.group(1)) if m else None
+
+
+def map_bytes(n_photons: int, n_buckets: int) -> dict:
+    """Algorithmic HBM bytes of the photon-map stages: hash 20 B/photon (position
+    in, key out) + 4 B/bucket (counts); scan 8 B/bucket; scatter 52 B/photon
+    (key and 24-B record in, 24-B record out) + 8 B/bucket."""
+    return {"hash": 20 * n_photons + 4 * n_buckets, "scan": 8 * n_buckets, "sort": 52 * n_photons + 8 * n_buckets}
+
+
+def valu_roofline(instructions, ms: float):
+    """VALU issue rate of a kernel from its counted wave-instructions per
+    launch (committed SQ summary) and its measured time: the bound the gather
+    actually runs against."""
+    if not instructions:
+        return None
+    g = instructions / (ms * 1e-3) / 1e9
+    return {"bound": "valu_issue", "instructions_per_launch": instructions, "achieved": g, "peak": VALU_PEAK_G,
+            "unit": "G wave-instructions/s", "frac": g / VALU_PEAK_G,
+            "source": "SQ_INSTS_VALU from profiles/k_gather_flat_<workload>_sq.txt, time from this run"}
+
+
+def stage_abi(cfg, scene, passes: int) -> dict:
+    """The reference's pass through the stage-level C ABI (phosphor_amd.stages:
+    trace, emission in batches with the blocking count readback, hash, scan,
+    sort, gather on one stream, the reference's median tree), timed per stage
+    with HIP events; HBM GB/s of the photon-map stages and the gather from
+    their algorithmic bytes."""
+    from phosphor_amd.stages import STAGES, StagePass
+
+    sp = StagePass(cfg, scene)
+    sp.run_pass()
+    ms, n = [], []
+    for _ in range(passes):
+        sp.run_pass()
+        ms.append(sp.stage_ms.copy())
+        n.append(sp.last_n)
+    sp.close()
+    ms = np.mean(ms, axis=0)
+    n_ph = int(np.mean(n))
+    buckets = cfg.grid_res ** 3 + 1
+    by = map_bytes(n_ph, buckets)
+    by["gather"] = gather_bytes(cfg.width * cfg.height, n_ph, buckets)
+    out = {"passes": passes, "ms_per_pass": float(ms.sum()),
+           "stage_ms": {k: float(v) for k, v in zip(STAGES, ms)}, "hbm": {}}
+    for k in ("hash", "scan", "sort", "gather"):
+        t = float(ms[STAGES.index(k)])
+        gbs = by[k] / (t * 1e-3) / 1e9
+        out["hbm"][k] = {"bytes": by[k], "ms": t, "GB_s": gbs, "frac": gbs / HBM_PEAK_GBPS}
+    return out
+
+
 def run_cpu_baseline(cfg, scene, passes: int, threads: int):
     import oracle
 
@@ -320,6 +383,9 @@ def main() -> int:
                     help="each pass's gather ends before the next pass starts (default: pipelined; same results)")
     ap.add_argument("--stage-passes", type=int, default=4,
                     help="passes after the timed region, each waited for, that give the stage timings")
+    ap.add_argument("--abi-passes", type=int, default=2,
+                    help="passes of the reference's sequence through the stage-level C ABI, timed per stage "
+                         "(0 = skip)")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
@@ -383,6 +449,17 @@ def main() -> int:
         print(f"[bench] relative RMSE after {args.noise_passes} passes vs an independent "
               f"{args.noise_ref_passes}-pass render: {rm:.4f} ({time.perf_counter() - t1:.1f} s)",
               file=sys.stderr, flush=True)
+
+    abi = None
+    if args.abi_passes > 0:
+        try:
+            abi = stage_abi(cfg, scene, args.abi_passes)
+            print(f"[bench] stage-level ABI pass: {abi['ms_per_pass']:.2f} ms ("
+                  + ", ".join(f"{k} {v:.2f}" for k, v in abi["stage_ms"].items()) + " ms); "
+                  + ", ".join(f"{k} {v['GB_s']:.0f} GB/s" for k, v in abi["hbm"].items()), file=sys.stderr,
+                  flush=True)
+        except Exception as e:
+            print(f"[bench] stage-level ABI pass skipped: {e}", file=sys.stderr)
 
     occupancy = None
     try:
@@ -449,7 +526,13 @@ def main() -> int:
                      "hash_scan_sort": float(mean_stage[2]), "gather": float(mean_stage[3])},
         "roofline": {"kernel": "k_gather_flat", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": read_traffic("k_gather_flat", args.workload)},
+                     "traffic": read_traffic("k_gather_flat", args.workload),
+                     "valu": valu_roofline(read_valu("k_gather_flat", args.workload), g_ms)},
+        "map_stage": {"kernels": "hash (octant keys), LSD radix sort, cell and octant starts, record decode",
+                      "ms": float(mean_stage[2]),
+                      "GB_s": sum(map_bytes(n_photons, buckets).values()) / (float(mean_stage[2]) * 1e-3) / 1e9,
+                      "note": "algorithmic bytes of hash + scan + scatter over the whole map stage's time"},
+        "stage_abi": abi,
         "rays": rays,
         "occupancy": occupancy,
         "quality": quality or None,

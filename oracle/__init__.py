@@ -46,6 +46,7 @@ _SIGS = {
     "ref_renderer_create": (VP, VP, VP),
     "ref_renderer_destroy": (None, VP),
     "ref_renderer_count_visits": (None, VP, INT),
+    "ref_renderer_octant_order": (None, VP, INT),
     "ref_renderer_rays": (None, VP, VP),
     "ref_trace_range": (None, VP, U32, U32, U32),
     "ref_emit_range": (U32, VP, U32, U32, U32, VP, VP, VP, U32),
@@ -89,7 +90,7 @@ def _p(a):
 class RefRenderer:
     """CPU restatement of Renderer (renderer.cpp) with the same pass interface."""
 
-    def __init__(self, config, scene, count_visits: bool = False):
+    def __init__(self, config, scene, count_visits: bool = False, octant_order: bool = True):
         from phosphor_amd.abi import HASH_DTYPE  # layouts only
 
         self.config = config
@@ -100,6 +101,7 @@ class RefRenderer:
         if not self._h:
             raise RuntimeError("ref_renderer_create failed")
         lib().ref_renderer_count_visits(self._h, 1 if count_visits else 0)
+        lib().ref_renderer_octant_order(self._h, 1 if octant_order else 0)
         self.n_pixels = config.width * config.height
         self.bucket_count = config.grid_res ** 3 + 1
         self._hash_dtype = HASH_DTYPE

@@ -1589,6 +1589,9 @@ struct RefRenderer {
     uint64_t visits_trace = 0, visits_emit = 0, visits_gather = 0;
     uint64_t rays_trace = 0, rays_emit = 0;
     bool count_visits = false;
+    /* photon map in (cell, octant, deposit) order, the library's pass; off:
+       (cell, deposit), the order of phx_sort_photons and the stage-level ABI */
+    bool octant_order = true;
 };
 
 void ref_renderer_destroy(RefRenderer *r) { delete r; }
@@ -1698,6 +1701,7 @@ RefRenderer *ref_renderer_create(const PhxRenderArgs *a, const PhxSceneDesc *s) 
 }
 
 void ref_renderer_count_visits(RefRenderer *r, int on) { r->count_visits = on != 0; }
+void ref_renderer_octant_order(RefRenderer *r, int on) { r->octant_order = on != 0; }
 
 static double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -1788,8 +1792,9 @@ int ref_renderer_run_pass(RefRenderer *r) {
     r->sdir.resize(n);
     ref_sort_photons(r->ppos.data(), r->ppow.data(), r->pdir.data(), r->hashes.data(), r->cend.data(), n,
                      r->spos.data(), r->spow.data(), r->sdir.data());
-    order_cells_by_octant(r->spos.data(), r->spow.data(), r->sdir.data(), r->cstart.data(), r->cend.data(),
-                          r->bucket_count, r->hash);
+    if (r->octant_order)
+        order_cells_by_octant(r->spos.data(), r->spow.data(), r->sdir.data(), r->cstart.data(), r->cend.data(),
+                              r->bucket_count, r->hash);
     double t3 = now_ms();
     /* gather_flux (renderer.cpp:155-161) */
     uint64_t visits = 0;

@@ -677,13 +677,20 @@ hipError_t launch_decode_photons(const u32 *ppow, const u32 *pdir, const u32 *d_
     return hipGetLastError();
 }
 
-/* The gather of the image-shaped launch, one wave per 8×8 tile as k_gather,
- * with gather_pixel_flat's flattened loops. */
+/* The gather with gather_pixel_flat's flattened loops: one wave per 8×8 tile
+ * of the image as k_gather, or, with the image shape unknown (tiles_x = 0),
+ * one wave per 64 consecutive pixels; blocks assigned XCD by XCD either way. */
+template <bool DECODED>
 __global__ __launch_bounds__(256) void k_gather_flat(GatherArgs a, u32 width, u32 height, u32 tiles_x) {
     u32 wave = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     u32 lane = threadIdx.x & 63u;
+    if (tiles_x == 0) {
+        u32 tid = wave * 64u + lane;
+        gather_pixel_flat<DECODED>(a, tid < a.n_pixels, tid, a.d, a.alpha, 0u);
+        return;
+    }
     u32 x = (wave % tiles_x) * 8u + (lane & 7u), y = (wave / tiles_x) * 8u + (lane >> 3);
-    gather_pixel_flat(a, x < width && y < height, y * width + x, a.d, a.alpha, 0u);
+    gather_pixel_flat<DECODED>(a, x < width && y < height, y * width + x, a.d, a.alpha, 0u);
 }
 
 hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const phx_float4 *hp_throughput,
@@ -699,20 +706,21 @@ hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, 
                  d,          alpha,     flux,          photon_weight, radiance,  rec_ratio,
                  rec_r2,     pdirpow,   pdirpow && width ? oct_start : nullptr};
     bool dec = pdirpow != nullptr;
-    if (width == 0) { /* image shape unknown (C ABI call): flat pixel range */
+    if (width == 0) { /* image shape unknown (C ABI call): 64 consecutive pixels per wave */
+        u32 waves = (n_pixels + 63) / 64;
         if (dec)
-            hipLaunchKernelGGL(k_gather<true>, dim3((n_pixels + 255) / 256), dim3(256), 0, s, a, 0u, 0u);
+            hipLaunchKernelGGL(k_gather_flat<true>, dim3((waves + 3) / 4), dim3(256), 0, s, a, 0u, 0u, 0u);
         else
-            hipLaunchKernelGGL(k_gather<false>, dim3((n_pixels + 255) / 256), dim3(256), 0, s, a, 0u, 0u);
+            hipLaunchKernelGGL(k_gather_flat<false>, dim3((waves + 3) / 4), dim3(256), 0, s, a, 0u, 0u, 0u);
         return hipGetLastError();
     }
     u32 h = (n_pixels + width - 1) / width;
     u32 tiles_x = (width + 7) / 8, tiles_y = (h + 7) / 8;
     u32 waves = tiles_x * tiles_y;
     if (dec)
-        hipLaunchKernelGGL(k_gather_flat, dim3((waves + 3) / 4), dim3(256), 0, s, a, width, h, tiles_x);
+        hipLaunchKernelGGL(k_gather_flat<true>, dim3((waves + 3) / 4), dim3(256), 0, s, a, width, h, tiles_x);
     else
-        hipLaunchKernelGGL(k_gather<false>, dim3((waves + 3) / 4), dim3(256), 0, s, a, width, tiles_x);
+        hipLaunchKernelGGL(k_gather_flat<false>, dim3((waves + 3) / 4), dim3(256), 0, s, a, width, h, tiles_x);
     return hipGetLastError();
 }
 
@@ -722,7 +730,7 @@ bool kernel_info_map(PhxKernelInfo *out, u32 max, u32 &n) {
     PHX_INFO("k_hash", k_hash, 256)
     PHX_INFO("k_radix_scatter", k_radix_scatter, SORT_BLOCK)
     PHX_INFO("k_decode_photons", k_decode_photons, 256)
-    PHX_INFO("k_gather_flat", k_gather_flat, 256)
+    PHX_INFO("k_gather_flat", k_gather_flat<true>, 256)
     return ok;
 }
 

@@ -828,6 +828,7 @@ PHX_DEV QueryReach query_reach(const PhxPhotonHashData &d, F4 pos, f32 r_sq, boo
  * lanes without a pixel).  Handing pixels out dynamically (a lane that
  * finishes takes another pixel of its block) was measured slower: the lanes
  * of a wave then query unrelated cells and stop sharing record loads. */
+template <bool DECODED = true>
 PHX_DEV void gather_pixel_flat(const GatherArgs &a, bool valid, u32 tid, const PhxPhotonHashData &d, f32 alpha,
                                u32 bucket_base) {
     const u32 *cs = a.cell_start + bucket_base, *ce = a.cell_end + bucket_base;
@@ -910,15 +911,30 @@ PHX_DEV void gather_pixel_flat(const GatherArgs &a, bool valid, u32 tid, const P
         F4 pp[B], pd[B];
         bool work = nt > 0;
         u32 kl = n2 ? k2 + n2 - 1 : k + n1 - 1;
+        u32 pdr[DECODED ? 1 : B];
 #pragma unroll
         for (u32 q = 0; q < B; q++) {
             u32 kk = q < n1 ? k + q : (q < nt ? k2 + (q - n1) : kl);
             pp[q] = work ? ld(a.ppos + kk) : ZERO;
-            pd[q] = work ? ld(a.pdirpow + kk) : ZERO;
+            if (DECODED) {
+                pd[q] = work ? ld(a.pdirpow + kk) : ZERO;
+            } else {
+                pd[q].w = phx_as_float(work ? a.ppow[kk] : 0u);
+                pdr[q] = work ? a.pdir[kk] : 0u;
+            }
         }
 #pragma unroll
         for (u32 q = 0; q < B; q++)
-            keep_loaded(pd[q]);
+            keep_loaded(DECODED ? pd[q] : mk(pd[q].w, phx_as_float(pdr[q]), 0.0f, 0.0f));
+        if (!DECODED) {
+            /* records not decoded once per pass (phx_gather_flux): the same
+               decode_oct at every visit */
+#pragma unroll
+            for (u32 q = 0; q < B; q++) {
+                F4 dv = decode_oct(pdr[q]);
+                pd[q] = mk(dv.x, dv.y, dv.z, pd[q].w);
+            }
+        }
         if (work) {
 #pragma unroll
             for (u32 q = 0; q < B; q++) {
