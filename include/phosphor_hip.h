@@ -496,7 +496,9 @@ int phx_batch_estimate(PhxBatch *b, uint32_t scene, float *rgb_out);
  * renders `rounds` passes from a fresh state with θ and returns the linear
  * estimate (H·W·3 floats); phx_adjoint_backward maps ∂loss/∂estimate (same
  * layout) to ∂loss/∂θ by replaying the passes (detached sampling: every random
- * decision is held, the weights are differentiated). */
+ * decision is held, the weights are differentiated; the roughness gradient
+ * adds the score of each GGX half vector and of the visible-normal sampler's
+ * rejection loop, whose acceptance depends on α). */
 typedef struct PhxAdjoint PhxAdjoint;
 int phx_adjoint_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, uint32_t max_rounds,
                        PhxAdjoint **out);
@@ -512,6 +514,13 @@ int phx_adjoint_read_replay(PhxAdjoint *a, phx_float4 *emission, phx_float4 *dir
                             uint32_t *forward_counts);
 /* The renderer the adjoint drives (its hitpoints hold the last replayed pass). */
 PhxRenderer *phx_adjoint_renderer(PhxAdjoint *a);
+
+/* GGX VNDF rejection (bsdfs.h:27-72) as the roughness gradient uses it: for
+ * α ∈ (0, 1] and n·v = mu ∈ [0, 1], out[0] = probability that one draw of
+ * the visible-normal sampler is rejected (mirror direction below the
+ * surface), out[1] = its derivative with respect to α (host computation, no
+ * device needed). */
+int phx_adjoint_ggx_rejection(float alpha, float mu, double *out);
 
 #ifdef __cplusplus
 }

@@ -171,6 +171,7 @@ SIGNATURES = {
     "phx_adjoint_set_replay_check": (INT, VP, U32),
     "phx_adjoint_read_replay": (INT, VP, VP, VP, VP, VP),
     "phx_adjoint_renderer": (VP, VP),
+    "phx_adjoint_ggx_rejection": (INT, F32, F32, VP),
     "phx_batch_create": (INT, VP, VP, U32, VP, C.POINTER(VP)),
     "phx_batch_destroy": (None, VP),
     "phx_batch_run_pass": (INT, VP),

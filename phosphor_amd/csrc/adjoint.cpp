@@ -40,6 +40,109 @@ static uint32_t host_random_u32(uint32_t &state) { /* random.h:29-34 */
     return ((old >> ((old >> 28u) + 4u)) ^ old) * 277803737u;
 }
 
+/* ---- GGX VNDF rejection (bsdfs.h:27-72) ----
+ * The sampler draws h from Heitz's visible-normal density
+ * D_v(h) = G1(v)·max(0, v·h)·D(h)/(n·v) and rejects it when the mirror
+ * direction falls below the surface, 2(v·h)(n·h) < n·v, up to 8 times, then
+ * returns the normal.  With n·v = μ, s = √(1 − μ²) and h at polar angle θ,
+ * azimuth φ from the plane of incidence, 2(v·h)cos θ − μ = R·cos(2θ − δ)
+ * with tan δ = s·cos φ / μ, so for each φ the rejected half vectors are
+ * θ ∈ [(δ + π/2)/2, θ_hi], θ_hi = π/2 (or π/2 + δ where cos φ < 0, the
+ * edge of v·h > 0).  The region does not depend on α, so
+ *   P(α, μ)  = ∫ D_v dω over it,   ∂P/∂α = ∫ D_v·(∂log G1(v) + ∂log D) dω,
+ * both smooth integrals: Gauss–Legendre in θ on the interval, trapezoid in
+ * φ (periodic).  An accepted draw has probability density D_v(h)·A(α) with
+ * A = Σ_{k<8} P^k (any of the 8 tries), a fallback has probability P^8: the
+ * score terms ∂log A/∂α and 8·∂log P/∂α that the half vector's score needs
+ * besides ∂log D_v. */
+namespace {
+constexpr int REJ_PHI = 96, REJ_THETA = 24;
+
+struct GaussLegendre {
+    double x[REJ_THETA], w[REJ_THETA];
+    GaussLegendre() { /* nodes on [-1, 1] by Newton's method on P_n */
+        const int n = REJ_THETA;
+        for (int i = 0; i < n; i++) {
+            double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), pp = 0.0;
+            for (int it = 0; it < 100; it++) {
+                double p1 = 1.0, p2 = 0.0;
+                for (int j = 1; j <= n; j++) {
+                    double p3 = p2;
+                    p2 = p1;
+                    p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+                }
+                pp = n * (z * p1 - p2) / (z * z - 1.0);
+                double dz = p1 / pp;
+                z -= dz;
+                if (std::fabs(dz) < 1e-15)
+                    break;
+            }
+            x[i] = z;
+            w[i] = 2.0 / ((1.0 - z * z) * pp * pp);
+        }
+    }
+};
+
+void ggx_rejection(double alpha, double mu, double &P, double &dP) {
+    static const GaussLegendre gl;
+    mu = std::min(std::max(mu, 0.0), 1.0);
+    double s = std::sqrt(std::max(0.0, 1.0 - mu * mu));
+    double a2 = alpha * alpha;
+    double root = std::sqrt(a2 + (1.0 - a2) * mu * mu);
+    double g1_over_mu = 2.0 / (mu + root);                                /* G1(v)/μ */
+    double dlog_g1v = root > 0.0 ? -alpha * (1.0 - mu * mu) / (root * (mu + root)) : 0.0;
+    P = dP = 0.0;
+    for (int i = 0; i < REJ_PHI; i++) {
+        double phi = 2.0 * M_PI * (i + 0.5) / REJ_PHI;
+        double cphi = std::cos(phi);
+        double delta = std::atan2(s * cphi, mu);
+        double lo = 0.5 * (delta + 0.5 * M_PI);
+        double hi = cphi < 0.0 ? 0.5 * M_PI + delta : 0.5 * M_PI;
+        if (!(hi > lo))
+            continue;
+        double half = 0.5 * (hi - lo), mid = 0.5 * (hi + lo), sp = 0.0, sd = 0.0;
+        for (int j = 0; j < REJ_THETA; j++) {
+            double th = mid + half * gl.x[j];
+            double ct = std::cos(th), st = std::sin(th);
+            double vh = s * st * cphi + mu * ct;
+            if (vh <= 0.0)
+                continue;
+            double d = ct * ct * (a2 - 1.0) + 1.0;
+            double D = a2 / (M_PI * d * d);
+            double dens = g1_over_mu * vh * D * st; /* D_v · sin θ (solid angle) */
+            double dlog_d = 2.0 / alpha - 4.0 * alpha * ct * ct / d;
+            sp += gl.w[j] * dens;
+            sd += gl.w[j] * dens * (dlog_g1v + dlog_d);
+        }
+        P += half * sp;
+        dP += half * sd;
+    }
+    P *= 2.0 * M_PI / REJ_PHI;
+    dP *= 2.0 * M_PI / REJ_PHI;
+}
+} // namespace
+
+namespace phx {
+/* per material: ACC_MU samples over μ = n·v of [∂log A/∂α, 8·∂log P/∂α] */
+void rejection_table(float roughness, float *out) {
+    double r = std::min(std::max((double)roughness, 0.001), 1.0);
+    double alpha = r * r;
+    for (int i = 0; i < ACC_MU; i++) {
+        double P, dP;
+        ggx_rejection(alpha, (double)i / (ACC_MU - 1), P, dP);
+        double num = 0.0, den = 0.0, pk = 1.0;
+        for (int k = 0; k < 8; k++) { /* A = Σ_{k<8} P^k, A' = Σ k P^(k-1) P' */
+            den += pk;
+            if (k + 1 < 8)
+                num += (k + 1) * pk;
+            pk *= P;
+        }
+        out[2 * i] = (float)(num * dP / den);
+        out[2 * i + 1] = P > 0.0 ? (float)(8.0 * dP / P) : 0.0f;
+    }
+}
+} // namespace phx
+
 struct PhxAdjoint {
     PhxRenderer *r = nullptr;
     uint32_t M = 0, max_rounds = 0, rounds = 0, n_px = 0;
@@ -49,7 +152,7 @@ struct PhxAdjoint {
     std::vector<uint32_t> states; /* PCG state before each pass */
     std::vector<float> theta;
     phx::DevBuf d_ratio, d_r2, d_lam_rad, d_lam_tau, d_R, d_nf, d_adj_sorted, d_adj_dep, d_perm, d_grad, d_am,
-        d_light_mat;
+        d_light_mat, d_rej;
     phx::DevBuf d_chk_emission, d_chk_direct, d_chk_count;
     bool check = false;
 
@@ -176,6 +279,11 @@ int PhxAdjoint::backward(const float *dl, float *grad) {
     }
     PHX_TRY(hipMemcpy(d_lam_rad.p, lam_rad.data(), n_px * sizeof(phx_float4), hipMemcpyHostToDevice), "adjoint");
     PHX_TRY(hipMemcpy(d_lam_tau.p, lam_tau.data(), n_px * sizeof(phx_float4), hipMemcpyHostToDevice), "adjoint");
+    /* the VNDF rejection terms of each material's roughness */
+    std::vector<float> rej((size_t)M * ACC_MU * 2);
+    for (uint32_t m = 0; m < M; m++)
+        rejection_table(theta[5 * m + 3], rej.data() + (size_t)m * ACC_MU * 2);
+    PHX_TRY(copy_to_dev_buf(d_rej, rej.data(), rej.size(), sizeof(float), s), "adjoint");
     PHX_TRY(launch_fill_u32(d_R.as<uint32_t>(), 0x3F800000u, n_px, s), "adjoint"); /* 1.0f */
     PHX_TRY(launch_fill_u32(d_grad.as<uint32_t>(), 0u, 5 * (size_t)M, s), "adjoint");
     const PhxRenderArgs &a = r->args;
@@ -205,6 +313,7 @@ int PhxAdjoint::backward(const float *dl, float *grad) {
         P.max_deposits = r->photon_buf_size;
         P.d_seed = r->seed_ptr();
         P.am = d_am.as<AdjMaterial>();
+        P.rej = d_rej.as<float>();
         P.light_mat = d_light_mat.as<uint32_t>();
         P.hp_pos = im.d_hp_pos.as<phx_float4>();
         P.hp_normal = im.d_hp_normal.as<phx_float4>();
@@ -242,6 +351,13 @@ int PhxAdjoint::backward(const float *dl, float *grad) {
 }
 
 extern "C" {
+
+int phx_adjoint_ggx_rejection(float alpha, float mu, double *out) {
+    if (!out || !(alpha > 0.0f) || !(alpha <= 1.0f) || !(mu >= 0.0f) || !(mu <= 1.0f))
+        return fail(PHX_ERR_INVALID_ARG, "phx_adjoint_ggx_rejection: alpha outside (0, 1] or mu outside [0, 1]");
+    ggx_rejection(alpha, mu, out[0], out[1]);
+    return PHX_OK;
+}
 
 int phx_adjoint_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, uint32_t max_rounds,
                        PhxAdjoint **out) {

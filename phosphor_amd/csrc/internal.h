@@ -213,6 +213,12 @@ struct AdjMaterial {
     float pad[3];
 };
 
+/* samples over n·v ∈ [0, 1] of a material's VNDF rejection terms */
+constexpr int ACC_MU = 65;
+/* [∂log A/∂α, 8·∂log P/∂α] per sample (A: acceptance within 8 tries, P: the
+ * rejection probability of one try) for a roughness, into out[2·ACC_MU] */
+void rejection_table(float roughness, float *out);
+
 /* everything one pass of the backward sweep reads and writes */
 struct AdjPass {
     SceneArgs sc;
@@ -221,6 +227,7 @@ struct AdjPass {
     uint32_t w, h, n_direct, photons, max_bounces, M, max_deposits;
     const uint32_t *d_seed;
     const AdjMaterial *am;
+    const float *rej; /* [material][ACC_MU][2] rejection_table */
     const uint32_t *light_mat;
     const phx_float4 *hp_pos, *hp_normal, *hp_throughput, *hp_base_color, *hp_emission;
     const phx_float4 *ppos;

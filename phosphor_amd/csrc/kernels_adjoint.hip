@@ -58,13 +58,30 @@ PHX_DEV F4 dlog_fresnel_f0(F4 f0, F4 i, F4 n) {
               (1.0f - p) / (f0.z + (1.0f - f0.z) * p), 0.0f);
 }
 
+/* the rejection loop's share of the half vector's score at n·v = mu: the
+ * material's table (ACC_MU samples over [0, 1], linear interpolation) of
+ * ∂log A/∂α for an accepted draw, 8·∂log P/∂α for the fallback */
+PHX_DEV f32 rejection_score(const f32 *rej, f32 mu, u32 fell_back) {
+    f32 x = phx_clampf(mu, 0.0f, 1.0f) * (f32)(ACC_MU - 1);
+    u32 i = (u32)x;
+    i = i < (u32)(ACC_MU - 2) ? i : (u32)(ACC_MU - 2);
+    f32 f = x - (f32)i;
+    f32 a = rej[2 * i + fell_back], b = rej[2 * (i + 1) + fell_back];
+    return a + (b - a) * f;
+}
+
 /* log-derivative of the sampled event's weight (albedo per channel in xyz,
  * roughness in w) for a hit with material m; `score` receives the roughness
- * part that belongs to choosing the event at all (the half vector's score),
- * which is what a contribution conditional on the event carries */
-PHX_DEV F4 event_dlog(const AdjMaterial &am, const Ctx &c, const Bsdf &bs, F4 view, F4 sn, f32 &score_out) {
+ * part that belongs to choosing the event at all (the half vector's score:
+ * ∂log of its density under the VNDF and of the rejection loop's
+ * acceptance), which is what a contribution conditional on the event
+ * carries.  A fallback draw (h = the normal) has no VNDF density term. */
+PHX_DEV F4 event_dlog(const AdjMaterial &am, const f32 *rej, const Ctx &c, const Bsdf &bs, F4 view, F4 sn,
+                      f32 &score_out) {
     f32 alpha = c.roughness * c.roughness;
-    f32 score = dlog_g1(phx_fmaxf(dot(view, sn), 0.0f), alpha) + dlog_ndf(dot(bs.h, sn), alpha);
+    f32 ndv = phx_fmaxf(dot(view, sn), 0.0f);
+    f32 score = bs.fell_back ? 0.0f : dlog_g1(ndv, alpha) + dlog_ndf(dot(bs.h, sn), alpha);
+    score += rejection_score(rej, ndv, bs.fell_back);
     score_out = score * am.rough_gate;
     F4 r = mk(0.0f, 0.0f, 0.0f, 0.0f);
     f32 rough = score;
@@ -217,6 +234,7 @@ struct AdjPhotonArgs {
     u32 photons, max_bounces;
     const u32 *d_seed;
     const AdjMaterial *am;
+    const f32 *rej;
     const u32 *light_mat; /* material of each light (0xFFFFFFFF: none) */
     u32 M;
     const u32 *offs;         /* first deposit of each photon (exclusive scan of counts) */
@@ -278,7 +296,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adj_photons(AdjPhotonArgs a) {
             Bsdf bs = sample_bsdf(rng, c, c.shading_normal, sh.normal, view, ior, sh.front_face);
             u32 m = sh.mat_index < a.M ? sh.mat_index : 0u;
             f32 score;
-            F4 ev = event_dlog(a.am[m], c, bs, view, c.shading_normal, score);
+            F4 ev = event_dlog(a.am[m], a.rej + (size_t)m * ACC_MU * 2, c, bs, view, c.shading_normal, score);
             if (bs.event == BSDF_DIFFUSE && dot(dir, sh.normal) < 0.0f) {
                 /* the deposit carries the path so far and the choice of the
                    diffuse event here (the half-vector score) */
@@ -310,6 +328,7 @@ struct AdjCameraArgs {
     const u32 *d_seed;
     SceneView sc;
     const AdjMaterial *am;
+    const f32 *rej;
     const u32 *light_mat;
     u32 M;
     const phx_float4 *lam_rad, *lam_tau, *nf;
@@ -400,7 +419,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adj_camera(AdjCameraArgs a) {
             F4 view = -dir;
             Bsdf bs = sample_bsdf(rng, c, c.shading_normal, sh.normal, view, ior, sh.front_face);
             f32 score;
-            F4 ev = event_dlog(am, c, bs, view, c.shading_normal, score);
+            F4 ev = event_dlog(am, a.rej + (size_t)m * ACC_MU * 2, c, bs, view, c.shading_normal, score);
             F4 hit_pos = sh.position, hit_n = sh.normal, sn = c.shading_normal;
             if (bs.event != BSDF_TRANSMIT) {
                 bool diffuse = bs.event == BSDF_DIFFUSE;
@@ -556,13 +575,14 @@ hipError_t launch_adjoint_pass(const AdjPass &P, hipStream_t s) {
         return e;
     size_t lds = (size_t)P.M * 9 * ADJ_BLOCK * sizeof(f32);
     EmitLayout EL = emit_layout(P.emit_ws, P.photons, P.max_bounces);
-    AdjPhotonArgs pa{P.sc,      P.photons, P.max_bounces,  P.d_seed, P.am,   P.light_mat, P.M,
-                     EL.offs,   P.adj_dep, P.max_deposits, P.grad,   P.chk_count};
+    AdjPhotonArgs pa{P.sc,      P.photons,   P.max_bounces, P.d_seed,       P.am,   P.rej,      P.light_mat,
+                     P.M,       EL.offs,     P.adj_dep,     P.max_deposits, P.grad, P.chk_count};
     hipLaunchKernelGGL(k_adj_photons, dim3((P.photons + ADJ_BLOCK - 1) / ADJ_BLOCK), dim3(ADJ_BLOCK), lds, s, pa);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
-    AdjCameraArgs ca{P.cam,     P.w,       P.h,  P.n_direct, P.d_seed, P.sc,   P.am,           P.light_mat,
-                     P.M,       P.lam_rad, P.lam_tau, P.nf,  P.R,      P.grad, P.chk_emission, P.chk_direct};
+    AdjCameraArgs ca{P.cam, P.w,       P.h,       P.n_direct, P.d_seed, P.sc,   P.am,           P.rej,
+                     P.light_mat, P.M, P.lam_rad, P.lam_tau,  P.nf,     P.R,    P.grad,         P.chk_emission,
+                     P.chk_direct};
     hipLaunchKernelGGL(k_adj_camera, dim3((n_px + ADJ_BLOCK - 1) / ADJ_BLOCK), dim3(ADJ_BLOCK), lds, s, ca);
     return hipGetLastError();
 }

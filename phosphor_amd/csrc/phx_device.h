@@ -804,7 +804,7 @@ PHX_DEV F4 safe_reflect(F4 d, F4 n, F4 g) {
     return r;
 }
 /* bsdfs.h:27-72 (Heitz 2018 VNDF sampling with ≤ 8 rejection attempts) */
-PHX_DEV F4 ggx_sample_vndf(u32 &rng, F4 normal, F4 geom_normal, F4 view, f32 roughness) {
+PHX_DEV F4 ggx_sample_vndf(u32 &rng, F4 normal, F4 geom_normal, F4 view, f32 roughness, u32 &fell_back) {
     f32 alpha = roughness * roughness;
     TBN f = make_tbn(normal);
     F4 t = f.t, b = f.b;
@@ -829,9 +829,12 @@ PHX_DEV F4 ggx_sample_vndf(u32 &rng, F4 normal, F4 geom_normal, F4 view, f32 rou
         F4 nl = normalize(mk(alpha * nh.x, alpha * nh.y, phx_fmaxf(0.0f, nh.z), 0.0f));
         F4 ne = normalize(nl.x * t + nl.y * b + nl.z * normal);
         F4 refl = reflect(-view, ne);
-        if (dot(refl, geom_normal) >= 0.0f)
+        if (dot(refl, geom_normal) >= 0.0f) {
+            fell_back = 0u;
             return ne;
+        }
     }
+    fell_back = 1u;
     return normal;
 }
 PHX_DEV f32 smith_g1_ggx(f32 c, f32 alpha) {
@@ -878,13 +881,14 @@ PHX_DEV f32 ggx_ndf(f32 ndh, f32 alpha) {
 
 struct Bsdf {
     F4 dir, throughput;
-    F4 h; /* the sampled GGX half vector (read by the adjoint) */
+    F4 h;          /* the sampled GGX half vector (read by the adjoint) */
     u32 event;
+    u32 fell_back; /* every draw of the half vector was rejected: h is the normal (read by the adjoint) */
 };
 /* shading.h:150-213 (the order of random draws is part of the contract) */
 PHX_DEV Bsdf sample_bsdf(u32 &rng, const Ctx &c, F4 sn, F4 gn, F4 view, IorStack &ior, bool front) {
     Bsdf s;
-    F4 h = ggx_sample_vndf(rng, sn, gn, view, c.roughness);
+    F4 h = ggx_sample_vndf(rng, sn, gn, view, c.roughness, s.fell_back);
     s.h = h;
     f32 alpha = c.roughness * c.roughness;
     if (random_float(rng) < c.metallic) {
