@@ -111,34 +111,46 @@ def valu_roofline(instructions, ms: float):
             "source": "SQ_INSTS_VALU from profiles/k_gather_flat_<workload>_sq.txt, time from this run"}
 
 
-def stage_abi(cfg, scene, passes: int) -> dict:
-    """The reference's pass through the stage-level C ABI (phosphor_amd.stages:
-    trace, emission in batches with the blocking count readback, hash, scan,
-    sort, gather on one stream, the reference's median tree), timed per stage
-    with HIP events; HBM GB/s of the photon-map stages and the gather from
-    their algorithmic bytes."""
-    from phosphor_amd.stages import STAGES, StagePass
+def _stage_sequence(cfg, scene, passes: int, fast: bool) -> dict:
+    from phosphor_amd.stages import StagePass
 
-    sp = StagePass(cfg, scene)
+    sp = StagePass(cfg, scene, fast=fast)
     sp.run_pass()
     ms, n = [], []
     for _ in range(passes):
         sp.run_pass()
         ms.append(sp.stage_ms.copy())
         n.append(sp.last_n)
+    names, accelerated = sp.stages, sp.accelerated
     sp.close()
     ms = np.mean(ms, axis=0)
     n_ph = int(np.mean(n))
     buckets = cfg.grid_res ** 3 + 1
     by = map_bytes(n_ph, buckets)
+    by["map"] = by["hash"] + by["scan"] + by["sort"]
     by["gather"] = gather_bytes(cfg.width * cfg.height, n_ph, buckets)
     out = {"passes": passes, "ms_per_pass": float(ms.sum()),
-           "stage_ms": {k: float(v) for k, v in zip(STAGES, ms)}, "hbm": {}}
-    for k in ("hash", "scan", "sort", "gather"):
-        t = float(ms[STAGES.index(k)])
-        gbs = by[k] / (t * 1e-3) / 1e9
-        out["hbm"][k] = {"bytes": by[k], "ms": t, "GB_s": gbs, "frac": gbs / HBM_PEAK_GBPS}
+           "stage_ms": {k: float(v) for k, v in zip(names, ms)}, "hbm": {}}
+    if fast:
+        out["pair_tree"] = accelerated
+    for k in names:
+        if k in by:
+            t = float(ms[names.index(k)])
+            gbs = by[k] / (t * 1e-3) / 1e9
+            out["hbm"][k] = {"bytes": by[k], "ms": t, "GB_s": gbs, "frac": gbs / HBM_PEAK_GBPS}
     return out
+
+
+def stage_abi(cfg, scene, passes: int) -> dict:
+    """The reference's pass through the stage-level C ABI (phosphor_amd.stages),
+    timed per stage with HIP events, in two wirings: the reference's kernels
+    one for one (trace, emission in batches with the blocking count readback,
+    hash, scan, sort, gather on one stream, the reference's median tree), and
+    the drop-in's faster forms of the same stages (pair tree attached to the
+    median tree, octant-ordered map in one call, octant gather).  HBM GB/s of
+    the photon-map stages and the gather from their algorithmic bytes."""
+    return {"reference_kernels": _stage_sequence(cfg, scene, passes, False),
+            "fast_forms": _stage_sequence(cfg, scene, passes, True)}
 
 
 def run_cpu_baseline(cfg, scene, passes: int, threads: int):
@@ -454,10 +466,11 @@ def main() -> int:
     if args.abi_passes > 0:
         try:
             abi = stage_abi(cfg, scene, args.abi_passes)
-            print(f"[bench] stage-level ABI pass: {abi['ms_per_pass']:.2f} ms ("
-                  + ", ".join(f"{k} {v:.2f}" for k, v in abi["stage_ms"].items()) + " ms); "
-                  + ", ".join(f"{k} {v['GB_s']:.0f} GB/s" for k, v in abi["hbm"].items()), file=sys.stderr,
-                  flush=True)
+            for name, seq in abi.items():
+                print(f"[bench] stage-level ABI pass, {name}: {seq['ms_per_pass']:.2f} ms ("
+                      + ", ".join(f"{k} {v:.2f}" for k, v in seq["stage_ms"].items()) + " ms); "
+                      + ", ".join(f"{k} {v['GB_s']:.0f} GB/s" for k, v in seq["hbm"].items()), file=sys.stderr,
+                      flush=True)
         except Exception as e:
             print(f"[bench] stage-level ABI pass skipped: {e}", file=sys.stderr)
 

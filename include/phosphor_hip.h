@@ -276,6 +276,68 @@ int phx_gather_flux(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const
                     const uint32_t *cell_end, const PhxPhotonHashData *data, float sppm_alpha,
                     phx_float4 *flux, float *photon_weight, phx_float4 *radiance, void *stream);
 
+/* ---- Faster forms of the stages above, same results ----
+ *
+ * The SAH child-pair tree for the trace entry points.  No counterpart in the
+ * reference: its set_scene builds the median tree (bvh.cpp:74-84) and
+ * uploads it with the triangles (renderer.cpp:32-34, geometry.cpp); a
+ * drop-in calls phx_accel_attach once after that upload with the device
+ * pointers and the host arrays it uploaded.  Afterwards phx_trace_rays,
+ * phx_trace_rays_wavefront and phx_emit_photons, called on the current
+ * device with that bvh_nodes pointer and that tri_verts pointer, trace the
+ * library's SAH tree of the same triangles (leaves of up to leaf_size, 0 = 3)
+ * stored as 64-byte child pairs, and return the same hit records bit for bit.
+ * That holds where the reference's search cannot drop a subtree and is
+ * insensitive to visiting order: one triangle per leaf, nodes in preorder,
+ * leaf boxes holding their triangle's box grown by 1e-3 (bounding_box.h),
+ * interior boxes holding their children's, no interior node deeper than 22,
+ * coordinates within ±1024.  phx_accel_eligible checks this on the host (1 or
+ * 0; when 0, phx_last_error says which property fails); phx_accel_attach
+ * sets *accelerated to 0 and keeps the reference tree when it fails.  The
+ * device arrays must hold what the host arrays do (attach compares them) and
+ * must not change while attached; phx_accel_detach (which synchronises the
+ * device) before they are freed or rewritten. */
+int phx_accel_eligible(const PhxBvhNode *host_tree, uint32_t n_nodes, const PhxTriangleVertices *host_tri_verts,
+                       uint32_t n_triangles);
+int phx_accel_attach(const PhxBvhNode *device_tree, const PhxTriangleVertices *device_tri_verts,
+                     const PhxBvhNode *host_tree, uint32_t n_nodes, const PhxTriangleVertices *host_tri_verts,
+                     uint32_t n_triangles, uint32_t leaf_size, uint32_t *accelerated);
+int phx_accel_detach(const PhxBvhNode *device_tree);
+
+/* Device scratch needed by phx_sort_photons_by_octant. */
+size_t phx_sort_photons_by_octant_workspace_size(uint32_t n_photons, uint32_t n_cells);
+
+/* hash_photons.cl + the scan + scatter_photons.cl (renderer.cpp:123-153) in
+ * one call, for the gather below.  The reference leaves the order inside a
+ * cell to atomic arrival (scatter_photons.cl:13); here records are sorted by
+ * cell, then by the half-cell octant that holds them (per axis, whether the
+ * position lies in the upper half of its cell, from get_hash's quotients,
+ * photon_hash.h:21-37), then by input order.  Writes the sorted records,
+ * cell_start/cell_end for all n_cells (>= grid_res^3 + 1) buckets, and for
+ * every non-empty cell its eight octant starts (oct_start[8h + o]; entries of
+ * empty cells are left as they were).  out_dir_power (nullable) receives each
+ * sorted record's decoded direction (encoding.h decode_oct) with the packed
+ * power in w, the form phx_gather_flux_octants reads. */
+int phx_sort_photons_by_octant(const phx_float4 *photon_pos_in, const uint32_t *photon_power_in,
+                               const uint32_t *photon_dir_in, uint32_t n_photons, const PhxPhotonHashData *data,
+                               uint32_t n_cells, phx_float4 *out_photon_pos, uint32_t *out_photon_power,
+                               uint32_t *out_photon_dir, phx_float4 *out_dir_power, uint32_t *cell_start,
+                               uint32_t *cell_end, uint32_t *oct_start, void *workspace, size_t workspace_bytes,
+                               void *stream);
+
+/* gather_flux.cl on a map from phx_sort_photons_by_octant: the same
+ * per-pixel sums as phx_gather_flux over that map (the order of the records
+ * is the map's), with the photon directions read decoded (photon_dir_power),
+ * whole half-cell octants beyond the radius skipped (oct_start; null: cells
+ * only) and one wave per 8×8 tile of the image. */
+int phx_gather_flux_octants(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const phx_float4 *hp_throughput,
+                            const phx_float4 *hp_base_color, const phx_float4 *hp_direct,
+                            const phx_float4 *hp_emission, uint32_t image_width, uint32_t image_height,
+                            const phx_float4 *photon_pos, const phx_float4 *photon_dir_power, uint32_t n_photons,
+                            const uint32_t *cell_start, const uint32_t *cell_end, const uint32_t *oct_start,
+                            const PhxPhotonHashData *data, float sppm_alpha, phx_float4 *flux, float *photon_weight,
+                            phx_float4 *radiance, void *stream);
+
 /* ======================================================================
  * Device-memory plumbing (no reference counterpart: lets hosts without their
  * own HIP allocator, such as the Python tooling, own device buffers through

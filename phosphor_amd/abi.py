@@ -134,6 +134,13 @@ SIGNATURES = {
     "phx_sort_photons_workspace_size": (SZ, U32, U32),
     "phx_sort_photons": (INT, VP, VP, VP, VP, VP, VP, U32, U32, VP, VP, VP, VP, SZ, VP),
     "phx_gather_flux": (INT, VP, VP, VP, VP, VP, VP, U32, VP, VP, VP, U32, VP, VP, VP, F32, VP, VP, VP, VP),
+    "phx_accel_eligible": (INT, VP, U32, VP, U32),
+    "phx_accel_attach": (INT, VP, VP, VP, U32, VP, U32, U32, VP),
+    "phx_accel_detach": (INT, VP),
+    "phx_sort_photons_by_octant_workspace_size": (SZ, U32, U32),
+    "phx_sort_photons_by_octant": (INT, VP, VP, VP, U32, VP, U32, VP, VP, VP, VP, VP, VP, VP, VP, SZ, VP),
+    "phx_gather_flux_octants": (INT, VP, VP, VP, VP, VP, VP, U32, U32, VP, VP, U32, VP, VP, VP, VP, F32, VP, VP, VP,
+                                VP),
     "phx_device_count": (INT,),
     "phx_kernel_info": (INT, C.POINTER(KernelInfo), U32),
     "phx_device_malloc": (INT, C.POINTER(VP), SZ),

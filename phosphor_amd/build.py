@@ -25,7 +25,7 @@ ORACLE_DIR = ROOT / "oracle"
 ORACLE_LIB = ORACLE_DIR / "build" / "librefcpu.so"
 
 SOURCES = ["kernels_trace.hip", "kernels_photon_map.hip", "kernels_batch.hip", "kernels_adjoint.hip", "abi.cpp",
-           "renderer.cpp", "batch.cpp", "adjoint.cpp", "scene_upload.cpp", "bvh.cpp"]
+           "renderer.cpp", "batch.cpp", "adjoint.cpp", "scene_upload.cpp", "bvh.cpp", "accel.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -79,6 +79,7 @@ static SceneArgs scene_args(const PhxTriangleVertices *tv, const PhxTriangleNorm
     if (center)
         s.scene_center = *center;
     s.scene_radius = radius;
+    apply_attached_tree(s); /* the pair tree, where phx_accel_attach registered one for this tree */
     return s;
 }
 
@@ -301,6 +302,71 @@ int phx_gather_flux(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const
                                     photon_pos, photon_power, photon_dir, cell_start, cell_end, *data, sppm_alpha,
                                     flux, photon_weight, radiance, 0u, (hipStream_t)stream),
                       "phx_gather_flux");
+}
+
+static size_t octant_keys_bytes(uint32_t n) { return ((size_t)n * sizeof(uint32_t) + 255) & ~(size_t)255; }
+
+size_t phx_sort_photons_by_octant_workspace_size(uint32_t n_photons, uint32_t n_cells) {
+    return octant_keys_bytes(n_photons) + sort_workspace_bytes(n_photons, n_cells);
+}
+
+int phx_sort_photons_by_octant(const phx_float4 *photon_pos_in, const uint32_t *photon_power_in,
+                               const uint32_t *photon_dir_in, uint32_t n_photons, const PhxPhotonHashData *data,
+                               uint32_t n_cells, phx_float4 *out_photon_pos, uint32_t *out_photon_power,
+                               uint32_t *out_photon_dir, phx_float4 *out_dir_power, uint32_t *cell_start,
+                               uint32_t *cell_end, uint32_t *oct_start, void *workspace, size_t workspace_bytes,
+                               void *stream) {
+    if (!data || !cell_start || !cell_end || !oct_start ||
+        (n_photons > 0 && (!photon_pos_in || !photon_power_in || !photon_dir_in || !out_photon_pos ||
+                           !out_photon_power || !out_photon_dir)))
+        return fail(PHX_ERR_INVALID_ARG, "phx_sort_photons_by_octant: null pointer argument");
+    if (data->grid_res == 0 || data->grid_res > 255)
+        return fail(PHX_ERR_INVALID_ARG, "phx_sort_photons_by_octant: grid_res outside 1..255 (cmd_args.hpp:51)");
+    uint64_t g = data->grid_res;
+    if ((uint64_t)n_cells < g * g * g + 1)
+        return fail(PHX_ERR_INVALID_ARG, "phx_sort_photons_by_octant: n_cells below grid_res^3 + 1");
+    if ((uint64_t)n_cells * 8u > 0xFFFFFFFFull)
+        return fail(PHX_ERR_TOO_LARGE, "phx_sort_photons_by_octant: 8 * n_cells overflows u32");
+    if (!workspace || workspace_bytes < phx_sort_photons_by_octant_workspace_size(n_photons, n_cells))
+        return fail(PHX_ERR_WORKSPACE,
+                    "phx_sort_photons_by_octant: workspace smaller than phx_sort_photons_by_octant_workspace_size");
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t *keys = (uint32_t *)workspace;
+    void *sort_ws = (char *)workspace + octant_keys_bytes(n_photons);
+    hipError_t e = launch_hash(photon_pos_in, n_photons, *data, keys, nullptr, s, true);
+    if (e == hipSuccess)
+        e = launch_sort(photon_pos_in, photon_power_in, photon_dir_in, keys, nullptr, cell_end, n_photons, n_cells,
+                        out_photon_pos, out_photon_power, out_photon_dir, sort_ws, s, nullptr, nullptr, cell_start,
+                        oct_start);
+    if (e == hipSuccess && out_dir_power && n_photons > 0)
+        e = launch_decode_photons(out_photon_power, out_photon_dir, nullptr, n_photons, out_dir_power, s);
+    return hip_status(e, "phx_sort_photons_by_octant");
+}
+
+int phx_gather_flux_octants(const phx_float4 *hp_pos, const phx_float4 *hp_normal, const phx_float4 *hp_throughput,
+                            const phx_float4 *hp_base_color, const phx_float4 *hp_direct,
+                            const phx_float4 *hp_emission, uint32_t image_width, uint32_t image_height,
+                            const phx_float4 *photon_pos, const phx_float4 *photon_dir_power, uint32_t n_photons,
+                            const uint32_t *cell_start, const uint32_t *cell_end, const uint32_t *oct_start,
+                            const PhxPhotonHashData *data, float sppm_alpha, phx_float4 *flux, float *photon_weight,
+                            phx_float4 *radiance, void *stream) {
+    (void)n_photons;
+    uint64_t n_pixels = (uint64_t)image_width * image_height;
+    if (!data || (n_pixels > 0 && (!hp_pos || !hp_normal || !hp_throughput || !hp_base_color || !hp_direct ||
+                                   !hp_emission || !cell_start || !cell_end || !photon_pos || !photon_dir_power ||
+                                   !flux || !photon_weight || !radiance)))
+        return fail(PHX_ERR_INVALID_ARG, "phx_gather_flux_octants: null pointer argument");
+    if (data->grid_res == 0 || data->grid_res > 255)
+        return fail(PHX_ERR_INVALID_ARG, "phx_gather_flux_octants: grid_res outside 1..255");
+    if (n_pixels > 0xFFFFFFFFull)
+        return fail(PHX_ERR_TOO_LARGE, "phx_gather_flux_octants: width * height overflows u32");
+    if (n_pixels == 0)
+        return PHX_OK;
+    return hip_status(launch_gather(hp_pos, hp_normal, hp_throughput, hp_base_color, hp_direct, hp_emission,
+                                    (uint32_t)n_pixels, photon_pos, nullptr, nullptr, cell_start, cell_end, *data,
+                                    sppm_alpha, flux, photon_weight, radiance, image_width, (hipStream_t)stream,
+                                    nullptr, nullptr, photon_dir_power, oct_start),
+                      "phx_gather_flux_octants");
 }
 
 int phx_device_count(void) {

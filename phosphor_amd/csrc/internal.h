@@ -55,6 +55,31 @@ struct SceneArgs {
     const PhxTriangleVertices *tvo;
 };
 
+/* ---- the SAH pair tree (accel.cpp) ---- */
+/* Depth of the deepest interior node of a reference-layout tree (root 0);
+ * an error code when a child index is out of range. */
+int tree_depth(const PhxBvhNode *nodes, size_t n_nodes, uint32_t &max_depth);
+/* Whether rays may be traced on the pair tree instead of this reference
+ * tree with the same hits: one triangle per leaf, nodes in preorder, leaf
+ * boxes holding their triangle's box grown by 1e-3, interior boxes holding
+ * their children's, at most 22 interior levels, coordinates within ±1024.
+ * `why` names the first property that fails. */
+bool pair_tree_eligible(const PhxBvhNode *nodes, size_t n_nodes, const PhxTriangleVertices *tv, uint32_t n,
+                        std::string &why);
+struct PairTreeHost {
+    std::vector<PhxBvhPair> pairs;
+    std::vector<uint32_t> key, order;
+    std::vector<PhxTriangleVertices> tvo;
+    uint32_t root_ref = 0;
+};
+/* The SAH tree over the triangles (depth <= 22, leaves of up to leaf_size)
+ * as pairs, the triangles in its order and their reference leaf indices. */
+bool build_pair_tree(const PhxBvhNode *nodes, size_t n_nodes, const PhxTriangleVertices *tv, uint32_t n,
+                     uint32_t leaf_size, PairTreeHost &out);
+/* Fills the pair-tree fields of s when phx_accel_attach registered a pair
+ * tree for s.nodes and s.tv on the current device. */
+void apply_attached_tree(SceneArgs &s);
+
 /* ---- scan (single pass, decoupled look-back) ---- */
 constexpr uint32_t SCAN_TILE = 4096; /* 256 threads × 16 elements */
 size_t scan_workspace_bytes(uint32_t n);
@@ -97,7 +122,7 @@ hipError_t launch_emit(const SceneArgs &sc, uint32_t max_photons, uint32_t offse
                        const uint32_t *d_seed = nullptr, uint32_t wave_bounces = 0);
 
 hipError_t launch_hash(const phx_float4 *pos, uint32_t n, const PhxPhotonHashData &d, uint32_t *hashes,
-                       uint32_t *counts, hipStream_t s);
+                       uint32_t *counts, hipStream_t s, bool octants = false);
 /* Same with the photon count read on the device (clamped to max_n); counts
  * may be null (keys only). */
 hipError_t launch_hash_dev(const phx_float4 *pos, const uint32_t *d_n, uint32_t max_n, const PhxPhotonHashData &d,

@@ -301,14 +301,14 @@ __global__ __launch_bounds__(256) void k_hash(const phx_float4 *__restrict__ pos
 }
 
 hipError_t launch_hash(const phx_float4 *pos, u32 n, const PhxPhotonHashData &d, u32 *hashes, u32 *counts,
-                       hipStream_t s) {
+                       hipStream_t s, bool octants) {
     if (n == 0)
         return hipSuccess;
     u32 blocks = (n + 255) / 256;
     if (blocks > 8192)
         blocks = 8192;
     hipLaunchKernelGGL(k_hash, dim3(blocks), dim3(256), 0, s, pos, n, (const u32 *)nullptr, 0u, d, hashes, counts,
-                       0u);
+                       octants ? 1u : 0u);
     return hipGetLastError();
 }
 
@@ -704,7 +704,7 @@ hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, 
     GatherArgs a{hp_pos,     hp_normal, hp_throughput, hp_base_color, hp_direct, hp_emission,
                  n_pixels,   ppos,      ppow,          pdir,          cell_start, cell_end,
                  d,          alpha,     flux,          photon_weight, radiance,  rec_ratio,
-                 rec_r2,     pdirpow,   pdirpow && width ? oct_start : nullptr};
+                 rec_r2,     pdirpow,   oct_start};
     bool dec = pdirpow != nullptr;
     if (width == 0) { /* image shape unknown (C ABI call): 64 consecutive pixels per wave */
         u32 waves = (n_pixels + 63) / 64;

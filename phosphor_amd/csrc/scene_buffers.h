@@ -54,6 +54,10 @@ struct GeometryBuffers { /* include/buffers/geometry.hpp */
     uint32_t root_ref = 0;
 };
 
+/* uploads a pair tree (accel.cpp build_pair_tree) into g's pair-tree
+ * buffers and sets g.root_ref */
+hipError_t upload_pair_tree(const PairTreeHost &h, GeometryBuffers &g, hipStream_t s);
+
 struct ShadingBuffers { /* include/buffers/shading.hpp */
     DevBuf d_materials, d_uv_transforms;
     uint32_t n_materials = 0;

@@ -91,73 +91,32 @@ int upload_scene(const PhxSceneDesc &sc, hipStream_t s, bool reference_bvh, uint
             return fail(PHX_ERR_INVALID_ARG, "set_scene: BVH build failed");
     }
     out.root = nodes[0];
-    /* uploaded below: live until the final synchronisation */
-    std::vector<uint32_t> key, order;
-    std::vector<PhxBvhPair> pairs;
-    std::vector<PhxTriangleVertices> tv_ord;
     if (leaf_size == 0)
         leaf_size = 3; /* fastest on the caustic and atrium workloads */
     if (leaf_size > PHX_PAIR_MAX_LEAF)
         return fail(PHX_ERR_INVALID_ARG, "set_scene: bvh_leaf_size beyond PHX_PAIR_MAX_LEAF");
-    {
-        bool transmissive = false;
-        for (uint32_t i = 0; i < sc.n_materials; i++)
-            transmissive = transmissive || !(sc.materials[i].transmission < 1e-5f);
-        uint32_t max_depth = 0;
-        std::vector<std::pair<uint32_t, uint32_t>> todo{{0u, 0u}};
-        while (!todo.empty()) {
-            auto [i, dpt] = todo.back();
-            todo.pop_back();
-            if (phx_as_uint(nodes[i].min.w) != 0xFFFFFFFFu)
-                continue;
-            max_depth = std::max(max_depth, dpt);
-            uint32_t l = i + 1, r = phx_as_uint(nodes[i].max.w);
-            if (l >= nodes.size() || r >= nodes.size() || r <= i || dpt >= nodes.size())
-                return fail(PHX_ERR_INVALID_ARG, "set_scene: BVH child index out of range");
-            todo.push_back({l, dpt + 1});
-            todo.push_back({r, dpt + 1});
-        }
-        out.shadow_any_hit = (!transmissive && max_depth <= 22) ? 1u : 0u;
-        out.shallow_bvh = 0;
-        /* Where the reference tree never overflows its search stack, its
-           closest hit is the least t over all triangles, equal t going to the
-           larger leaf index: a property of the triangles, not of the tree.
-           Trace then on an SAH tree of the same leaves (far fewer nodes per
-           ray) carrying those leaf indices as tie-break keys -- within
-           |coordinate| <= 1024, where box-test rounding stays far inside the
-           1e-3 leaf-box margin that makes the search order-free. */
-        const PhxBvhNode &rt = nodes[0];
-        float extent = std::max({std::fabs(rt.min.x), std::fabs(rt.min.y), std::fabs(rt.min.z), std::fabs(rt.max.x),
-                                 std::fabs(rt.max.y), std::fabs(rt.max.z)});
-        if (!sc.bvh_nodes && !reference_bvh && max_depth <= 22 && extent <= 1024.0f && n <= PHX_PAIR_MAX_TRIS) {
-            std::vector<uint32_t> leaf_of(n, 0u);
-            for (uint32_t i = 0; i < nodes.size(); i++) {
-                uint32_t t = phx_as_uint(nodes[i].min.w);
-                if (t != 0xFFFFFFFFu)
-                    leaf_of[t] = i;
-            }
-            std::vector<PhxBvhNode> sah(nodes.size());
-            order.resize(n);
-            uint32_t got = phx_build_bvh_sah(sc.tri_verts, n, 22, leaf_size, sah.data(), order.data());
-            pairs.resize(std::max<size_t>(1, got / 2));
-            uint32_t root_ref = 0;
-            if (got && phx_build_bvh_pairs(sah.data(), got, pairs.data(), &root_ref) == (int)(got / 2)) {
-                /* triangles and their tie-break keys in the tree's order */
-                key.resize(n);
-                tv_ord.resize(n);
-                for (uint32_t i = 0; i < n; i++) {
-                    key[i] = leaf_of[order[i]];
-                    tv_ord[i] = sc.tri_verts[order[i]];
-                }
-                GeometryBuffers &g = out.geometry;
-                PHX_TRY(copy_to_dev_buf(g.d_bvh_pairs, pairs.data(), pairs.size(), sizeof(PhxBvhPair), s), "upload");
-                PHX_TRY(copy_to_dev_buf(g.d_tri_key, key.data(), n, sizeof(uint32_t), s), "upload");
-                PHX_TRY(copy_to_dev_buf(g.d_tri_order, order.data(), n, sizeof(uint32_t), s), "upload");
-                PHX_TRY(copy_to_dev_buf(g.d_tri_verts_ord, tv_ord.data(), n, sizeof(PhxTriangleVertices), s),
-                        "upload");
-                g.root_ref = root_ref;
-                out.shallow_bvh = 1;
-            }
+    bool transmissive = false;
+    for (uint32_t i = 0; i < sc.n_materials; i++)
+        transmissive = transmissive || !(sc.materials[i].transmission < 1e-5f);
+    uint32_t max_depth = 0;
+    if (tree_depth(nodes.data(), nodes.size(), max_depth) != PHX_OK)
+        return fail(PHX_ERR_INVALID_ARG, "set_scene: BVH child index out of range");
+    out.shadow_any_hit = (!transmissive && max_depth <= 22) ? 1u : 0u;
+    out.shallow_bvh = 0;
+    /* Where the reference tree never overflows its search stack, its closest
+       hit is the least t over all triangles, equal t going to the larger leaf
+       index: a property of the triangles, not of the tree.  Trace then on an
+       SAH tree of the same leaves (far fewer nodes per ray) carrying those
+       leaf indices as tie-break keys -- within |coordinate| <= 1024, where
+       box-test rounding stays far inside the 1e-3 leaf-box margin that makes
+       the search order-free.  A tree passed in is searched as given. */
+    std::string why;
+    if (!sc.bvh_nodes && !reference_bvh && pair_tree_eligible(nodes.data(), nodes.size(), sc.tri_verts, n, why)) {
+        PairTreeHost h; /* host copies: live until the stream has read them */
+        if (build_pair_tree(nodes.data(), nodes.size(), sc.tri_verts, n, leaf_size, h)) {
+            PHX_TRY(upload_pair_tree(h, out.geometry, s), "upload");
+            PHX_TRY(hipStreamSynchronize(s), "upload");
+            out.shallow_bvh = 1;
         }
     }
 

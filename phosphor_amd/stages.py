@@ -17,6 +17,14 @@ kernels' depth-first search of it), the photon map is ordered by cell with
 deposits in canonical order inside a cell.  HIP events between the stages
 give per-stage times; the library's own pass driver (``Renderer``) is the
 fast path, this one measures the drop-in boundary.
+
+``fast=True`` wires the drop-in's faster forms of the same stages:
+``phx_accel_attach`` once at set-up (the trace and emission entry points then
+trace the library's SAH child-pair tree, same hits), and
+``phx_sort_photons_by_octant`` + ``phx_gather_flux_octants`` in place of
+hash → scan → sort → gather (the map ordered by cell, then half-cell octant,
+records decoded once; stages "trace", "emit", "map", "gather").  The results
+equal the oracle's pass with an octant-ordered map, bit for bit.
 """
 from __future__ import annotations
 
@@ -30,6 +38,7 @@ from .abi import HASH_DTYPE
 
 M32 = (1 << 32) - 1
 STAGES = ("trace", "emit", "hash", "scan", "sort", "gather")
+FAST_STAGES = ("trace", "emit", "map", "gather")
 
 
 def _pcg_seed(seed: int) -> int:
@@ -105,8 +114,10 @@ class _Buf:
 class StagePass:
     """Renderer::set_scene + Renderer::run over the stage-level C ABI."""
 
-    def __init__(self, config: PassConfig, scene):
+    def __init__(self, config: PassConfig, scene, fast: bool = False):
         L = lib()
+        self.fast = fast
+        self.stages = FAST_STAGES if fast else STAGES
         c = self.config = config
         self.scene = sc = scene
         self.n_pixels = c.width * c.height
@@ -147,6 +158,12 @@ class StagePass:
         self.tv, self.tn, self.tt, self.tuv, self.tmat = (buf(a.nbytes, a) for a in (
             sc.tri_verts, sc.tri_normals, sc.tri_tangents, sc.tri_uvs, sc.tri_mat_index))
         self.nodes = buf(nodes.nbytes, nodes)
+        self.accelerated = False
+        if fast:
+            acc = C.c_uint32(0)
+            check(L.phx_accel_attach(self.nodes.ptr, self.tv.ptr, nodes.ctypes.data, nodes.shape[0],
+                                     sc.tri_verts.ctypes.data, n, 0, C.byref(acc)), "accel_attach")
+            self.accelerated = bool(acc.value)
         self.etv, self.etn, self.etuv, self.etmat = (buf(a.nbytes, a) for a in (
             sc.etri_verts, sc.etri_normals, sc.etri_uvs, sc.etri_mat_index))
         self.mats, self.uvt, self.lights = (buf(a.nbytes, a) for a in (sc.materials, sc.uv_transforms, sc.lights))
@@ -177,15 +194,23 @@ class StagePass:
         self.sort_wsb = L.phx_sort_photons_workspace_size(pb, self.bucket_count)
         self.trace_ws, self.emit_ws = buf(self.trace_wsb, fill=None), buf(self.emit_wsb, fill=None)
         self.scan_ws, self.sort_ws = buf(self.scan_wsb, fill=None), buf(self.sort_wsb, fill=None)
+        if fast:
+            self.oct_start = buf(32 * self.bucket_count)
+            self.dirpow = buf(16 * pb)
+            self.map_wsb = L.phx_sort_photons_by_octant_workspace_size(pb, self.bucket_count)
+            self.map_ws = buf(self.map_wsb, fill=None)
         self.rng = _pcg_seed(c.seed)                                          # renderer.cpp:54
         self.rounds = 0
         self.total_photons = 0
         self.last_photon_count = 0
         self.hip = _Hip()
-        self.events = [self.hip.event() for _ in range(len(STAGES) + 1)]
-        self.stage_ms = np.zeros(len(STAGES))
+        self.events = [self.hip.event() for _ in range(len(self.stages) + 1)]
+        self.stage_ms = np.zeros(len(self.stages))
 
     def close(self):
+        if getattr(self, "accelerated", False):
+            lib().phx_accel_detach(self.nodes.ptr)
+            self.accelerated = False
         for b in getattr(self, "_bufs", []):
             b.free()
         self._bufs = []
@@ -227,6 +252,19 @@ class StagePass:
         if n == 0:
             raise RuntimeError("no photon hit")
         hip.record(ev[2])
+        if self.fast:
+            check(L.phx_sort_photons_by_octant(self.pos.ptr, self.pow.ptr, self.dir.ptr, n, self.hash.ctypes.data,
+                                               self.bucket_count, self.spos.ptr, self.spow.ptr, self.sdir.ptr,
+                                               self.dirpow.ptr, self.cstart.ptr, self.cend.ptr, self.oct_start.ptr,
+                                               self.map_ws.ptr, self.map_wsb, None), "sort_photons_by_octant")
+            hip.record(ev[3])
+            check(L.phx_gather_flux_octants(*(b.ptr for b in self.hp), c.width, c.height, self.spos.ptr,
+                                            self.dirpow.ptr, n, self.cstart.ptr, self.cend.ptr, self.oct_start.ptr,
+                                            self.hash.ctypes.data, c.sppm_alpha, self.flux.ptr, self.weight.ptr,
+                                            self.radiance.ptr, None), "gather_flux_octants")
+            hip.record(ev[4])
+            self._finish(n)
+            return
         check(L.phx_memset_device(self.counts.ptr, 0, 4 * self.bucket_count), "counts")   # renderer.cpp:128
         check(L.phx_hash_photons(self.pos.ptr, n, self.hash.ctypes.data, self.hashes.ptr, self.counts.ptr, None),
               "hash_photons")
@@ -243,10 +281,14 @@ class StagePass:
                                 c.sppm_alpha, self.flux.ptr, self.weight.ptr, self.radiance.ptr, None),
               "gather_flux")
         hip.record(ev[6])
-        self.stage_ms = np.array([hip.elapsed(ev[k], ev[k + 1]) for k in range(len(STAGES))])
+        self._finish(n)
+
+    def _finish(self, n: int) -> None:
+        ev, hip = self.events, self.hip
+        self.stage_ms = np.array([hip.elapsed(ev[k], ev[k + 1]) for k in range(len(self.stages))])
         self.last_n = n
         self.rounds += 1
-        self.total_photons += c.photons_per_round                              # renderer.cpp:78
+        self.total_photons += self.config.photons_per_round                    # renderer.cpp:78
 
     def read_state(self):
         npx = self.n_pixels

@@ -298,3 +298,64 @@ def test_scan_noalloc_limits_checked_before_the_device(phx_lib):
     assert phx_lib.phx_scan_cells_workspace_size(33423360) == 64 * 1024
     if not os.path.exists("/dev/kfd"):
         assert phx_lib.phx_scan_cells_noalloc(None, 0, None, None, None) != 0   # no device here
+
+
+def test_accel_eligibility_checks_each_property(phx_lib):
+    """phx_accel_eligible accepts the reference's median tree and refuses a
+    tree that breaks any property the pair tree's exactness rests on: a
+    shrunk leaf box, an interior box not holding a child, more than 22
+    interior levels, coordinates beyond 1024, nodes out of preorder, a
+    triangle in two leaves, a node count other than 2n - 1."""
+    from phosphor_amd import scenes
+
+    from tests.test_oracle_transport import _comb_tree
+
+    def median(tv):
+        n = tv.shape[0]
+        t = np.zeros((2 * n - 1, 8), np.float32)
+        assert phx_lib.phx_build_bvh_median(tv.ctypes.data, n, t.ctypes.data) == 2 * n - 1
+        return t
+
+    def verdict(tree, tv, n_nodes=None):
+        tree = np.ascontiguousarray(tree, np.float32)
+        tv = np.ascontiguousarray(tv, np.float32)
+        got = phx_lib.phx_accel_eligible(tree.ctypes.data, tree.shape[0] if n_nodes is None else n_nodes,
+                                         tv.ctypes.data, tv.shape[0])
+        return got, phx_lib.phx_last_error().decode()
+
+    sc = scenes.caustic_scene(8, 8, level=3)
+    tv = sc.tri_verts
+    tree = median(tv)
+    assert verdict(tree, tv)[0] == 1
+    interior = tree[:, 3].view(np.uint32) == 0xFFFFFFFF
+    leaf = int(np.nonzero(~interior)[0][0])
+    t = tree.copy()
+    t[leaf, 0] += np.float32(1e-4)
+    assert verdict(t, tv) == (0, "phx_accel_eligible: a leaf box smaller than its triangle's box grown by 1e-3")
+    inner = int(np.nonzero(interior)[0][1])
+    t = tree.copy()
+    t[inner, 0] += np.float32(0.01)
+    assert verdict(t, tv) == (0, "phx_accel_eligible: an interior box does not hold its children's boxes")
+    tris, comb = _comb_tree(30)
+    assert verdict(comb, tris)[1].startswith("phx_accel_eligible: deeper than 22 interior levels")
+    far = (tv * np.float32(2000.0)).astype(np.float32)
+    assert verdict(median(far), far) == (0, "phx_accel_eligible: coordinates beyond 1024")
+    assert verdict(tree, tv, n_nodes=tree.shape[0] - 1)[0] == 0
+    # three triangles; median tree: 0 = (leaf 1, interior 2 = (leaf 3, leaf 4))
+    small = tv[:3].copy()
+    ok = median(small)
+    assert verdict(ok, small)[0] == 1
+    leaves = np.nonzero(ok[:, 3].view(np.uint32) != 0xFFFFFFFF)[0]
+    # 0 = (interior 1 = (leaf 2, leaf 4), leaf 3): valid children, not preorder
+    bad = np.zeros_like(ok)
+    for i, r in ((0, 3), (1, 4)):
+        bad[i] = ok[0]
+        bad[i, 7] = np.uint32(r).view(np.float32)
+    for i, src in zip((2, 3, 4), leaves):
+        bad[i] = ok[src]
+    assert verdict(bad, small) == (0, "phx_accel_eligible: nodes not stored in preorder")
+    dup = ok.copy()
+    dup[leaves[1], 3] = dup[leaves[0], 3]
+    dup[leaves[1], :3] = np.minimum(ok[leaves[0], :3], ok[leaves[1], :3])
+    dup[leaves[1], 4:7] = np.maximum(ok[leaves[0], 4:7], ok[leaves[1], 4:7])
+    assert verdict(dup, small) == (0, "phx_accel_eligible: a triangle index out of range or in two leaves")

@@ -134,6 +134,90 @@ def test_trace_emit_gather_entry_points():
     assert np.array_equal(from_dev(drad, np.uint32, (n_px, 4)), rr.view(np.uint32))
 
 
+def test_fast_forms_on_uploaded_buffers():
+    """phx_accel_attach on the oracle's median tree: phx_trace_rays,
+    phx_trace_rays_wavefront and phx_emit_photons then run on the pair tree
+    and still equal the oracle bit for bit; phx_sort_photons_by_octant of the
+    oracle's deposits equals its octant-ordered photon map, and
+    phx_gather_flux_octants on that map equals its state.  After
+    phx_accel_detach the reference tree is searched again."""
+    import ctypes as C
+
+    sc = scenes.caustic_scene(48, 40, level=3)
+    cfg = PassConfig(width=48, height=40, photons_per_round=6000, max_photons_per_batch=6000, max_photon_bounces=12,
+                     grid_res=60, seed=11, initial_radius=0.015)
+    ref = oracle.RefRenderer(cfg, sc)
+    assert ref.run_pass() == 0
+    ds = DeviceScene(sc, ref)
+    L = P.lib()
+    nodes = ref.bvh_nodes()
+    acc = C.c_uint32(7)
+    P.check(L.phx_accel_attach(dptr(ds.nodes), dptr(ds.tv), nodes.ctypes.data, nodes.shape[0],
+                               sc.tri_verts.ctypes.data, ds.n, 0, C.byref(acc)), "attach")
+    assert acc.value == 1
+    seed = oracle.lib().ref_random_u32(np.array([oracle.lib().ref_pcg_seed(11)], np.uint32).ctypes.data)
+    n_px = cfg.width * cfg.height
+    want = ref.read_hitpoints()
+    for attached in (True, False):
+        hp = [empty_dev(16 * n_px) for _ in range(6)]
+        P.check(L.phx_trace_rays(sc.camera.ctypes.data, cfg.width, cfg.height, seed, cfg.direct_samples,
+                                 dptr(ds.tv), dptr(ds.tn), dptr(ds.tt), dptr(ds.tuv), dptr(ds.nodes), dptr(ds.tmat),
+                                 ds.n, dptr(ds.mats), dptr(ds.uvt), dptr(ds.tex_meta), dptr(ds.atlas),
+                                 dptr(ds.lights), len(sc.lights), dptr(ds.lum), ds.total_lum, ds.center.ctypes.data,
+                                 ds.radius, dptr(ds.etv), dptr(ds.etn), dptr(ds.etuv), dptr(ds.etmat), None, 0, 0, 0,
+                                 *(dptr(b) for b in hp), None), "trace")
+        got = np.stack([from_dev(b, np.float32, (n_px, 4)) for b in hp], axis=1)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f"attached={attached}"
+        _check_wavefront(ds, sc, cfg, seed, want, any_hit=0)
+        max_photons = cfg.photons_per_round * cfg.max_photon_bounces
+        dpos, dpw, ddr = empty_dev(16 * max_photons), empty_dev(4 * max_photons), empty_dev(4 * max_photons)
+        counter = to_dev(np.zeros(1, np.uint32))
+        wsb = L.phx_emit_photons_workspace_size(cfg.photons_per_round, cfg.max_photon_bounces)
+        ws = empty_dev(wsb, None)
+        P.check(L.phx_emit_photons(dptr(ds.lights), len(sc.lights), max_photons, 0, cfg.photons_per_round,
+                                   cfg.max_photon_bounces, seed, dptr(ds.nodes), dptr(ds.tv), dptr(ds.tn),
+                                   dptr(ds.tt), dptr(ds.tuv), dptr(ds.tmat), ds.n, dptr(ds.etv), dptr(ds.etn),
+                                   dptr(ds.etuv), dptr(ds.etmat), dptr(ds.mats), dptr(ds.uvt), dptr(ds.tex_meta),
+                                   dptr(ds.atlas), dptr(ds.lum), ds.total_lum, ds.center.ctypes.data, ds.radius,
+                                   dptr(dpos), dptr(dpw), dptr(ddr), dptr(counter), dptr(ws), wsb, None), "emit")
+        n = int(from_dev(counter, np.uint32, (1,))[0])
+        assert n == ref.last_photon_count
+        wpos, wpw, wdr = ref.read_deposits()
+        assert np.array_equal(from_dev(dpos, np.uint32, (n, 4)), wpos[:n].view(np.uint32))
+        assert np.array_equal(from_dev(dpw, np.uint32, (n,)), wpw[:n])
+        assert np.array_equal(from_dev(ddr, np.uint32, (n,)), wdr[:n])
+        if attached:
+            P.check(L.phx_accel_detach(dptr(ds.nodes)), "detach")
+
+    # the octant-ordered map in one call, then the octant gather
+    d = ref.hash_info()
+    buckets = cfg.grid_res ** 3 + 1
+    spos, spw, sdr, sdp = empty_dev(16 * n), empty_dev(4 * n), empty_dev(4 * n), empty_dev(16 * n)
+    dcs, dce, doct = empty_dev(4 * buckets), empty_dev(4 * buckets), empty_dev(32 * buckets, 0xFF)
+    wsb = L.phx_sort_photons_by_octant_workspace_size(n, buckets)
+    ws = empty_dev(wsb, None)
+    P.check(L.phx_sort_photons_by_octant(dptr(dpos), dptr(dpw), dptr(ddr), n, d.ctypes.data, buckets, dptr(spos),
+                                         dptr(spw), dptr(sdr), dptr(sdp), dptr(dcs), dptr(dce), dptr(doct), dptr(ws),
+                                         wsb, None), "sort_by_octant")
+    ppos, ppw, pdr, cs, ce = ref.read_photon_map()
+    assert np.array_equal(from_dev(spos, np.uint32, (n, 4)), ppos.view(np.uint32))
+    assert np.array_equal(from_dev(spw, np.uint32, (n,)), ppw)
+    assert np.array_equal(from_dev(sdr, np.uint32, (n,)), pdr)
+    assert np.array_equal(from_dev(dcs, np.uint32, (buckets,)), cs)
+    assert np.array_equal(from_dev(dce, np.uint32, (buckets,)), ce)
+    flux = np.zeros((n_px, 4), np.float32)
+    flux[:, 3] = np.float32(cfg.initial_radius) ** 2
+    dflux, dw, drad = to_dev(flux), to_dev(np.zeros(n_px, np.float32)), to_dev(np.zeros((n_px, 4), np.float32))
+    dhp = [to_dev(np.ascontiguousarray(want[:, k])) for k in range(6)]
+    P.check(L.phx_gather_flux_octants(*(dptr(b) for b in dhp), cfg.width, cfg.height, dptr(spos), dptr(sdp), n,
+                                      dptr(dcs), dptr(dce), dptr(doct), d.ctypes.data, cfg.sppm_alpha, dptr(dflux),
+                                      dptr(dw), dptr(drad), None), "gather_octants")
+    rf, rw, rr = ref.read_state()
+    assert np.array_equal(from_dev(dflux, np.uint32, (n_px, 4)), rf.view(np.uint32))
+    assert np.array_equal(from_dev(dw, np.uint32, (n_px,)), rw.view(np.uint32))
+    assert np.array_equal(from_dev(drad, np.uint32, (n_px, 4)), rr.view(np.uint32))
+
+
 def test_kernel_info_reports_registers_and_occupancy():
     """phx_kernel_info describes the traversal, emission and photon-map
     kernels: VGPRs within the 512-register file, at least one resident wave
@@ -147,12 +231,17 @@ def test_kernel_info_reports_registers_and_occupancy():
     assert info["k_trace_shadow"]["block"] == 128 and info["k_emit_wf_bounce"]["block"] == 128
 
 
+@pytest.mark.parametrize("fast", [False, True], ids=["reference_kernels", "fast_forms"])
 @pytest.mark.parametrize("case", ["caustic", "atrium"])
-def test_stage_pass_matches_the_oracle(case):
+def test_stage_pass_matches_the_oracle(case, fast):
     """The reference's pass issued through the stage-level ABI (StagePass:
     trace_rays_wavefront, emit_photons in batches, hash, scan, sort, gather on
     one stream, the reference's median tree) equals the oracle's pass with a
-    cell-ordered photon map after each of three passes, bit for bit."""
+    cell-ordered photon map after each of three passes, bit for bit.  With
+    the fast forms (the pair tree attached to the median tree, the map sorted
+    by octant in one call, the octant gather) it equals the oracle's pass
+    with an octant-ordered map, and the trace entry points do run on the
+    pair tree."""
     from phosphor_amd.stages import StagePass
 
     if case == "caustic":
@@ -163,9 +252,10 @@ def test_stage_pass_matches_the_oracle(case):
         sc = scenes.atrium_scene(160, 90, target_tris=40_000, env_size=(256, 128))
         cfg = PassConfig(width=160, height=90, photons_per_round=1 << 15, max_photons_per_batch=1 << 15,
                          max_photon_bounces=16, grid_res=64, seed=3)
-    print(f"case stage pass {case}: {sc.n_triangles} triangles", flush=True)
-    sp = StagePass(cfg, sc)
-    ref = oracle.RefRenderer(cfg, sc, octant_order=False)
+    print(f"case stage pass {case} fast={fast}: {sc.n_triangles} triangles", flush=True)
+    sp = StagePass(cfg, sc, fast=fast)
+    assert sp.accelerated == fast
+    ref = oracle.RefRenderer(cfg, sc, octant_order=fast)
     for p in range(3):
         sp.run_pass()
         assert ref.run_pass() == 0

@@ -128,7 +128,14 @@ def test_comb_tree_through_the_trace_entry_points():
     ref = oracle.RefRenderer(cfg, sc)
     assert ref.run_pass() == 0
     ds = DeviceScene(sc, ref)
-    assert ref.bvh_nodes().shape[0] == 2 * DEPTH + 1
+    nodes = ref.bvh_nodes()
+    assert nodes.shape[0] == 2 * DEPTH + 1
+    # the pair tree is refused for a tree whose search can overflow
+    import ctypes as C
+    acc = C.c_uint32(7)
+    P.check(P.lib().phx_accel_attach(dptr(ds.nodes), dptr(ds.tv), nodes.ctypes.data, nodes.shape[0],
+                                     sc.tri_verts.ctypes.data, ds.n, 0, C.byref(acc)), "attach")
+    assert acc.value == 0
     seed = oracle.lib().ref_random_u32(np.array([oracle.lib().ref_pcg_seed(cfg.seed)], np.uint32).ctypes.data)
     n_px = cfg.width * cfg.height
     hp = [empty_dev(16 * n_px) for _ in range(6)]
