@@ -35,6 +35,16 @@ def _norm(v):
     return (v / f32(np.sqrt(np.dot(v, v)))).astype(f32)
 
 
+def make_tbn(d: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Tangent frame of a unit direction (utils.h:44-53)."""
+    d = np.asarray(d, f32)
+    if abs(d[0]) > abs(d[1]):
+        t = _norm(np.array([-d[2], 0.0, d[0]], f32))
+    else:
+        t = _norm(np.array([0.0, d[2], -d[1]], f32))
+    return t, np.cross(d, t).astype(f32)
+
+
 def make_camera(position, look_at, up, hfov_deg: float, aspect: float, defocus_angle: float = 0.0,
                 focus_distance: float = 1.0) -> np.ndarray:
     """CameraParams of Camera::recalculate/focus/to_params (camera.cpp:15-56), in float32."""
@@ -239,14 +249,20 @@ class SceneBuilder:
         light["power"][:3] = power
         self.lights.append(light)
 
+    def add_spot_light(self, position, power, direction, inner_rad: float, outer_rad: float) -> None:
+        """light.cpp:13-22 make_spot_light."""
+        light = np.zeros(1, LIGHT_DTYPE)[0]
+        light["kind"] = 1
+        light["position"][:3] = position
+        light["power"][:3] = power
+        light["direction"][:3] = direction
+        light["aux"] = (inner_rad, outer_rad, 0.0, 0.0)
+        self.lights.append(light)
+
     def add_directional_light(self, direction, power) -> None:
-        """light.cpp:24-36 make_directional_light (tangent frame of utils.h:44-53)."""
+        """light.cpp:24-36 make_directional_light."""
         d = np.asarray(direction, f32)
-        if abs(d[0]) > abs(d[1]):
-            t = _norm(np.array([-d[2], 0.0, d[0]], f32))
-        else:
-            t = _norm(np.array([0.0, d[2], -d[1]], f32))
-        b = np.cross(d, t).astype(f32)
+        t, b = make_tbn(d)
         light = np.zeros(1, LIGHT_DTYPE)[0]
         light["kind"] = LIGHT_DIRECTIONAL
         light["direction"][:3] = d
