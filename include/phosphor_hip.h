@@ -487,6 +487,13 @@ int phx_renderer_synchronize(PhxRenderer *r);
 /* Stage timings of the last pass in milliseconds: trace, emit, hash+scan+sort,
  * gather (zeros for graph replays, which have no stage events). */
 int phx_renderer_last_stage_ms(PhxRenderer *r, float out_ms[4]);
+/* No counterpart in the reference (measurement): the gather kernel's time in
+ * each of the last min(passes since the reset, 64, max) eager passes, oldest
+ * first, from HIP events recorded around it on the stream it runs on (so a
+ * pipelined pass's figure includes what runs beside it); *n_out receives the
+ * count.  Waits for those gathers. */
+int phx_renderer_gather_ms(PhxRenderer *r, float *out_ms, uint32_t max, uint32_t *n_out);
+int phx_renderer_reset_gather_ms(PhxRenderer *r);
 /* Duration of the last pass in milliseconds (both modes). */
 int phx_renderer_last_pass_ms(PhxRenderer *r, float *out_ms);
 /* Checkpoint / resume of the progressive state (flux, photon_weight,

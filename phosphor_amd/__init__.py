@@ -141,6 +141,17 @@ class Renderer:
         check(lib().phx_renderer_last_stage_ms(self._h, ptr(out)), "stage_ms")
         return out
 
+    def gather_ms(self, max_passes: int = 64) -> np.ndarray:
+        """The gather kernel's time (ms) in each of the last eager passes since
+        reset_gather_ms (at most 64), from HIP events on its stream."""
+        out = np.zeros(max_passes, np.float32)
+        n = C.c_uint32(0)
+        check(lib().phx_renderer_gather_ms(self._h, ptr(out), max_passes, C.byref(n)), "gather_ms")
+        return out[:n.value].copy()
+
+    def reset_gather_ms(self) -> None:
+        check(lib().phx_renderer_reset_gather_ms(self._h), "reset_gather_ms")
+
     def pass_ms(self) -> float:
         out = np.zeros(1, np.float32)
         check(lib().phx_renderer_last_pass_ms(self._h, ptr(out)), "pass_ms")
@@ -249,6 +260,17 @@ class Batch:
 
     def synchronize(self) -> None:
         check(lib().phx_batch_synchronize(self._h), "batch_synchronize")
+
+    def gather_ms(self, max_passes: int = 64) -> np.ndarray:
+        """The gather kernel's time (ms) in each of the last eager passes since
+        reset_gather_ms (at most 64), from HIP events on its stream."""
+        out = np.zeros(max_passes, np.float32)
+        n = C.c_uint32(0)
+        check(lib().phx_renderer_gather_ms(self._h, ptr(out), max_passes, C.byref(n)), "gather_ms")
+        return out[:n.value].copy()
+
+    def reset_gather_ms(self) -> None:
+        check(lib().phx_renderer_reset_gather_ms(self._h), "reset_gather_ms")
 
     def pass_ms(self) -> float:
         out = np.zeros(1, np.float32)

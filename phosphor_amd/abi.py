@@ -159,6 +159,8 @@ SIGNATURES = {
     "phx_renderer_run_pass": (INT, VP),
     "phx_renderer_synchronize": (INT, VP),
     "phx_renderer_last_stage_ms": (INT, VP, VP),
+    "phx_renderer_gather_ms": (INT, VP, VP, U32, VP),
+    "phx_renderer_reset_gather_ms": (INT, VP),
     "phx_renderer_last_pass_ms": (INT, VP, VP),
     "phx_renderer_checkpoint_bytes": (C.c_size_t, VP),
     "phx_renderer_save_checkpoint": (INT, VP, VP, C.c_size_t),

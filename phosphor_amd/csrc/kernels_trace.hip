@@ -299,6 +299,7 @@ struct WfShadowArgs {
     SceneView sc;
     WfState st;
     u32 any_hit;
+    u32 group; /* n when n divides 64 (each event's sum formed in the wave), else 0 */
 };
 
 /* six waves per SIMD (76 VGPRs, no spills) */
@@ -307,16 +308,21 @@ __global__ __launch_bounds__(WF_SHADOW_BLOCK) __attribute__((amdgpu_waves_per_eu
     u32 *stk = s_stack + threadIdx.x;
     u32 count = *a.st.counter;
     for (u32 slot = blockIdx.x * WF_SHADOW_BLOCK + threadIdx.x; slot < count; slot += gridDim.x * WF_SHADOW_BLOCK) {
-        if (a.any_hit)
-            shadow_one<true>(a.sc, a.st, slot, stk);
-        else
-            shadow_one<false>(a.sc, a.st, slot, stk);
+        F4 v = a.any_hit ? shadow_value<true>(a.sc, a.st, slot, stk) : shadow_value<false>(a.sc, a.st, slot, stk);
+        if (a.group) {
+            /* count is a multiple of n: an event's slots pass this loop together */
+            F4 sum = group_sample_sum(v, a.group);
+            if ((slot & (a.group - 1u)) == 0u)
+                st(a.st.sh_tr + slot, sum);
+        } else {
+            st(a.st.sh_tr + slot, v);
+        }
     }
 }
 
 struct WfResolveArgs {
     WfState st;
-    u32 n_pixels, n;
+    u32 n_pixels, n, summed;
     phx_float4 *hp_direct, *hp_emission;
 };
 
@@ -324,7 +330,7 @@ __global__ __launch_bounds__(256) void k_trace_resolve(WfResolveArgs a) {
     u32 tid = blockIdx.x * 256 + threadIdx.x;
     if (tid >= a.n_pixels)
         return;
-    resolve_pixel(a.st, a.n, tid, tid, a.hp_direct, a.hp_emission);
+    resolve_pixel(a.st, a.n, tid, tid, a.hp_direct, a.hp_emission, a.summed != 0u);
 }
 
 WfState wf_layout(void *ws, u32 n_pixels, u32 n, bool with_scene) {
@@ -405,6 +411,8 @@ hipError_t launch_trace_wavefront(const PhxCameraParams &cam, u32 w, u32 h, u32 
     sa.sc = sc;
     sa.st = S;
     sa.any_hit = shadow_any_hit;
+    /* every slot allocation is a multiple of direct_samples (one per event) */
+    sa.group = (direct_samples > 0 && 64u % direct_samples == 0u) ? direct_samples : 0u;
     size_t expect = (size_t)n * direct_samples;
     u32 blocks = (u32)((expect + WF_SHADOW_BLOCK - 1) / WF_SHADOW_BLOCK);
     if (blocks > 16384)
@@ -416,6 +424,7 @@ hipError_t launch_trace_wavefront(const PhxCameraParams &cam, u32 w, u32 h, u32 
     ra.st = S;
     ra.n_pixels = n;
     ra.n = direct_samples;
+    ra.summed = sa.group ? 1u : 0u;
     ra.hp_direct = hp_direct;
     ra.hp_emission = hp_emission;
     hipLaunchKernelGGL(k_trace_resolve, dim3((n + 255) / 256), dim3(256), 0, s, ra);

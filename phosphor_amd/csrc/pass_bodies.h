@@ -292,17 +292,15 @@ PHX_DEV bool shadow_fold(const SceneView &sc, const HitRecord &rec, F4 o, F4 d, 
 }
 
 /* shadow_transmittance (light_sampling.h:302-333) of one queued shadow ray,
- * stored multiplied by the sample's NEE weight (the term resolve_pixel adds);
- * ANY = any-hit traversal (exact when no material transmits and the stack
- * never overflows). */
-template <bool ANY> PHX_DEV void shadow_one(const SceneView &sc, const WfState &S, u32 slot, u32 *stk) {
+ * multiplied by the sample's NEE weight (the term resolve_pixel adds), BLACK
+ * for a slot without a sample; ANY = any-hit traversal (exact when no
+ * material transmits and the stack never overflows). */
+template <bool ANY> PHX_DEV F4 shadow_value(const SceneView &sc, const WfState &S, u32 slot, u32 *stk) {
     u32 e = S.sh_ev[slot];
     F4 pre, o, d;
     f32 t_max;
-    if (e == NO_EVENT || !shadow_sample(sc, S, slot, e, pre, o, d, t_max)) {
-        st(S.sh_tr + slot, BLACK);
-        return;
-    }
+    if (e == NO_EVENT || !shadow_sample(sc, S, slot, e, pre, o, d, t_max))
+        return BLACK;
     f32 t_min = EPS;
     F4 trans = WHITE;
     for (u32 depth = 0; depth < MAX_RAY_BOUNCES; depth++) {
@@ -313,11 +311,34 @@ template <bool ANY> PHX_DEV void shadow_one(const SceneView &sc, const WfState &
         if (!shadow_fold<ANY>(sc, rec, o, d, t_min, trans))
             break;
     }
-    st(S.sh_tr + slot, pre * trans);
+    return pre * trans;
 }
 
-/* emission and direct of one pixel from its records and shadow results */
-PHX_DEV void resolve_pixel(const WfState &S, u32 n, u32 wf, u32 out, phx_float4 *hp_direct, phx_float4 *hp_emission) {
+template <bool ANY> PHX_DEV void shadow_one(const SceneView &sc, const WfState &S, u32 slot, u32 *stk) {
+    st(S.sh_tr + slot, shadow_value<ANY>(sc, S, slot, stk));
+}
+
+/* The sum over an event's `n` samples in sample order, BLACK + s0 + s1 + ...
+ * (light_sampling.h's loop), formed by the wave whose n adjacent lanes hold
+ * them: n divides 64 and the event's first slot is a multiple of n, so the
+ * group starts at lane (lane & ~(n - 1)).  Every lane of the wave that holds
+ * a sample takes part; each gets its group's sum. */
+PHX_DEV F4 group_sample_sum(F4 v, u32 n) {
+    u32 lane = threadIdx.x & 63u;
+    int g0 = (int)(lane & ~(n - 1u));
+    F4 acc = BLACK;
+    for (u32 d = 0; d < n; d++) {
+        int src = g0 + (int)d;
+        acc = acc + mk(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64), __shfl(v.w, src, 64));
+    }
+    return acc;
+}
+
+/* emission and direct of one pixel from its records and shadow results;
+ * `summed`: the first slot of each event holds the sum over its samples
+ * (group_sample_sum), the others are not read */
+PHX_DEV void resolve_pixel(const WfState &S, u32 n, u32 wf, u32 out, phx_float4 *hp_direct, phx_float4 *hp_emission,
+                           bool summed = false) {
     u32 meta = S.meta[wf];
     u32 nrec = meta & 7u;
     size_t rbase = (size_t)wf * MAX_RAY_BOUNCES;
@@ -333,8 +354,11 @@ PHX_DEV void resolve_pixel(const WfState &S, u32 n, u32 wf, u32 out, phx_float4 
            BLACK for a sample without a shadow ray (products formed by
            shadow_one) */
         F4 sum = BLACK;
-        for (u32 d = 0; d < n; d++)
-            sum = sum + ld(S.sh_tr + base + d);
+        if (summed)
+            sum = ld(S.sh_tr + base);
+        else
+            for (u32 d = 0; d < n; d++)
+                sum = sum + ld(S.sh_tr + base + d);
         F4 thr = ld(S.rec_thr + rbase + k);
         if (kind == NEE_SPECULAR) {
             e = e + thr * (sum / (f32)n);

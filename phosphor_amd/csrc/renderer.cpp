@@ -345,21 +345,35 @@ int PhxRenderer::enqueue_pass(bool events, bool pipe) {
         return rc;
     if (events)
         PHX_TRY(hipEventRecord(ev[1], stream), "event");
+    hipEvent_t *ring = ev_gring[gring_count % GATHER_RING];
     if (!pipe) {
         PHX_TRY(hipStreamWaitEvent(stream, ev_join, 0), "event");
-        if (events)
+        if (events) {
             PHX_TRY(hipEventRecord(ev_gather, stream), "event");
-        return gather_flux(stream, 0);
+            PHX_TRY(hipEventRecord(ring[0], stream), "event");
+        }
+        if ((rc = gather_flux(stream, 0)) != PHX_OK)
+            return rc;
+        if (events) {
+            PHX_TRY(hipEventRecord(ring[1], stream), "event");
+            gring_count++;
+        }
+        return PHX_OK;
     }
     PHX_TRY(hipEventRecord(ev_trace_done, stream), "event");
     PHX_TRY(hipStreamWaitEvent(gstream, ev_trace_done, 0), "event");
     PHX_TRY(hipStreamWaitEvent(gstream, ev_join, 0), "event");
-    if (events)
+    if (events) {
         PHX_TRY(hipEventRecord(ev_gather, gstream), "event");
+        PHX_TRY(hipEventRecord(ring[0], gstream), "event");
+    }
     if ((rc = gather_flux(gstream, 1)) != PHX_OK)
         return rc;
-    if (events)
+    if (events) {
+        PHX_TRY(hipEventRecord(ring[1], gstream), "event");
+        gring_count++;
         PHX_TRY(hipEventRecord(ev[4], gstream), "event");
+    }
     PHX_TRY(hipEventRecord(ev_read[1], gstream), "event");
     read_pending[1] = true;
     swap_sets();
@@ -463,6 +477,10 @@ int phx_renderer_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, Ph
     for (hipEvent_t *ev : {&r->ev_emit, &r->ev_gather, &r->ev_trace})
         if ((e = hipEventCreate(ev)) != hipSuccess)
             return hip_status(e, "phx_renderer_create: event");
+    for (auto &pair : r->ev_gring)
+        for (hipEvent_t &ev : pair)
+            if ((e = hipEventCreate(&ev)) != hipSuccess)
+                return hip_status(e, "phx_renderer_create: event");
     for (hipEvent_t *ev : {&r->ev_info, &r->ev_fork, &r->ev_join, &r->ev_trace_done, &r->ev_emit_done[0], &r->ev_emit_done[1],
                            &r->ev_read[0], &r->ev_read[1]})
         if ((e = hipEventCreateWithFlags(ev, hipEventDisableTiming)) != hipSuccess)
@@ -519,6 +537,30 @@ int phx_renderer_last_stage_ms(PhxRenderer *r, float out_ms[4]) {
     for (int k = 0; k < 4; k++)
         if ((e = hipEventElapsedTime(&out_ms[k], from[k], to[k])) != hipSuccess)
             return hip_status(e, "stage timing");
+    return PHX_OK;
+}
+
+int phx_renderer_gather_ms(PhxRenderer *r, float *out_ms, uint32_t max, uint32_t *n_out) {
+    if (!r || !n_out || (max && !out_ms))
+        return fail(PHX_ERR_INVALID_ARG, "null argument");
+    uint32_t have = std::min(r->gring_count, PhxRenderer::GATHER_RING);
+    uint32_t n = std::min(have, max);
+    *n_out = n;
+    for (uint32_t k = 0; k < n; k++) {
+        uint32_t slot = (r->gring_count - n + k) % PhxRenderer::GATHER_RING;
+        hipError_t e = hipEventSynchronize(r->ev_gring[slot][1]);
+        if (e == hipSuccess)
+            e = hipEventElapsedTime(&out_ms[k], r->ev_gring[slot][0], r->ev_gring[slot][1]);
+        if (e != hipSuccess)
+            return hip_status(e, "gather timing");
+    }
+    return PHX_OK;
+}
+
+int phx_renderer_reset_gather_ms(PhxRenderer *r) {
+    if (!r)
+        return fail(PHX_ERR_INVALID_ARG, "null argument");
+    r->gring_count = 0;
     return PHX_OK;
 }
 

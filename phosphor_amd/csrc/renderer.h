@@ -65,6 +65,11 @@ struct PhxRenderer {
        slot. */
     hipStream_t gstream = nullptr;
     hipEvent_t ev_trace_done = nullptr, ev_emit_done[2] = {}, ev_read[2] = {};
+    /* start and end of each pass's gather on the stream it runs on, for the
+       last GATHER_RING eager passes (phx_renderer_gather_ms) */
+    static constexpr uint32_t GATHER_RING = 64;
+    hipEvent_t ev_gring[GATHER_RING][2] = {};
+    uint32_t gring_count = 0;
     bool read_pending[2] = {false, false};
     bool emit_pending[2] = {false, false};
     uint32_t seed_slot = 0;
@@ -104,6 +109,10 @@ struct PhxRenderer {
         for (hipEvent_t &e : ev)
             if (e)
                 (void)hipEventDestroy(e);
+        for (auto &pair : ev_gring)
+            for (hipEvent_t e : pair)
+                if (e)
+                    (void)hipEventDestroy(e);
         for (hipEvent_t e : {ev_emit, ev_gather, ev_trace, ev_fork, ev_join, ev_info, ev_trace_done, ev_emit_done[0],
                              ev_emit_done[1], ev_read[0],
                              ev_read[1]})
