@@ -86,7 +86,7 @@ __global__ __launch_bounds__(WF_CAM_BLOCK) void k_bcam(BCamArgs a) {
         bool live = local < B.n_pixels;
         u32 out = B.px_base + local;
         camera_path(B.sc, B.cam, B.width, B.height, a.round_seed[s], B.direct_samples, live, local, out, out - a.px0,
-                    s, a.st, a.o, s_stack + threadIdx.x, s_ior + threadIdx.x, WF_CAM_BLOCK);
+                    s, a.st, a.o, s_stack + threadIdx.x, s_ior + threadIdx.x, WF_CAM_BLOCK, B.direct_samples);
     }
 }
 

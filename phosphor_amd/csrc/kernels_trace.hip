@@ -292,7 +292,7 @@ __global__ __launch_bounds__(WF_CAM_BLOCK) void k_trace_camera(WfCamArgs a) {
     bool live = tid < a.w * a.h;
     u32 seed = a.d_seed ? *a.d_seed : a.seed;
     camera_path(a.sc, a.cam, a.w, a.h, seed, a.n, live, tid, tid, tid, 0u, a.st, a.o, s_stack + threadIdx.x,
-                s_ior + threadIdx.x, WF_CAM_BLOCK);
+                s_ior + threadIdx.x, WF_CAM_BLOCK, single_delta_light(a.sc) ? 1u : a.n);
 }
 
 struct WfShadowArgs {
@@ -307,9 +307,10 @@ __global__ __launch_bounds__(WF_SHADOW_BLOCK) __attribute__((amdgpu_waves_per_eu
     __shared__ u32 s_stack[BVH_STACK_SIZE * WF_SHADOW_BLOCK];
     u32 *stk = s_stack + threadIdx.x;
     u32 count = *a.st.counter;
+    bool shared = single_delta_light(a.sc); /* one slot per event (camera_path) */
     for (u32 slot = blockIdx.x * WF_SHADOW_BLOCK + threadIdx.x; slot < count; slot += gridDim.x * WF_SHADOW_BLOCK) {
         F4 v = a.any_hit ? shadow_value<true>(a.sc, a.st, slot, stk) : shadow_value<false>(a.sc, a.st, slot, stk);
-        if (a.group) {
+        if (a.group && !shared) {
             /* count is a multiple of n: an event's slots pass this loop together */
             F4 sum = group_sample_sum(v, a.group);
             if ((slot & (a.group - 1u)) == 0u)
@@ -321,6 +322,7 @@ __global__ __launch_bounds__(WF_SHADOW_BLOCK) __attribute__((amdgpu_waves_per_eu
 }
 
 struct WfResolveArgs {
+    SceneView sc;
     WfState st;
     u32 n_pixels, n, summed;
     phx_float4 *hp_direct, *hp_emission;
@@ -330,7 +332,7 @@ __global__ __launch_bounds__(256) void k_trace_resolve(WfResolveArgs a) {
     u32 tid = blockIdx.x * 256 + threadIdx.x;
     if (tid >= a.n_pixels)
         return;
-    resolve_pixel(a.st, a.n, tid, tid, a.hp_direct, a.hp_emission, a.summed != 0u);
+    resolve_pixel(a.st, a.n, tid, tid, a.hp_direct, a.hp_emission, a.summed != 0u, single_delta_light(a.sc));
 }
 
 WfState wf_layout(void *ws, u32 n_pixels, u32 n, bool with_scene) {
@@ -421,6 +423,7 @@ hipError_t launch_trace_wavefront(const PhxCameraParams &cam, u32 w, u32 h, u32 
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     WfResolveArgs ra;
+    ra.sc = sc;
     ra.st = S;
     ra.n_pixels = n;
     ra.n = direct_samples;

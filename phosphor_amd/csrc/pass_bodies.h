@@ -61,15 +61,26 @@ struct CamOut {
     phx_float4 *hp_pos, *hp_normal, *hp_throughput, *hp_base_color;
 };
 
+/* The scene's only light is a point, spot or directional light.  Then every
+ * light sample of an NEE event selects that light with select_light's one
+ * draw and takes no other: its shadow ray, weight and transmittance are the
+ * same for all n samples of the event (sample_light_fragment), so one queue
+ * slot per event holds the value each sample has. */
+PHX_DEV bool single_delta_light(const SceneView &sc) {
+    return sc.n_lights == 1 && sc.lights[0].kind != PHX_LIGHT_TEXTURED;
+}
+
 /* One camera path of trace_rays.cl up to its first diffuse hit, with every
  * NEE event's fragments drawn and its shadow rays queued.  `pix` is the pixel
  * within its image (PCG stream and raster position), `out` its index in the
  * hitpoint arrays, `wf` its index in the wavefront state.  Every lane of the
  * wave must call this together (the queue slots are allocated wave-wide);
- * lanes with live = false only take part in the allocation. */
+ * lanes with live = false only take part in the allocation.  `slots` queue
+ * slots per event: n, or 1 for a scene with a single delta light (the slot
+ * gets the state before the first sample). */
 PHX_DEV void camera_path(const SceneView &sc, const PhxCameraParams &cam, u32 w, u32 h, u32 seed, u32 n, bool live,
                          u32 pix, u32 out, u32 wf, u32 tag, const WfState &S, const CamOut &o, u32 *stk, f32 *s_ior,
-                         u32 block) {
+                         u32 block, u32 slots) {
     u32 px = live ? pix % w : 0, py = live ? pix / w : 0;
     u32 rng = pcg_seed(seed + pix);
     F4 origin = ZERO, dir = ZERO;
@@ -149,7 +160,7 @@ PHX_DEV void camera_path(const SceneView &sc, const PhxCameraParams &cam, u32 w,
         u32 rank = (u32)__popcll(want & ((1ull << lane) - 1ull));
         u32 wave_base = 0;
         if (want) {
-            u32 total = (u32)__popcll(want) * n;
+            u32 total = (u32)__popcll(want) * slots;
             u32 first = (u32)__ffsll((long long)want) - 1u;
             if (lane == first)
                 wave_base = atomicAdd(S.counter, total);
@@ -161,7 +172,7 @@ PHX_DEV void camera_path(const SceneView &sc, const PhxCameraParams &cam, u32 w,
                each sample; this lane only steps its stream past the draws
                the samples take (skip_light_fragment), so the path goes on
                with the state the reference has after direct lighting. */
-            u32 base = wave_base + rank * n;
+            u32 base = wave_base + rank * slots;
             size_t e = rbase + depth;
             S.rec_base[e] = base;
             st(S.ev_pos + e, hit_pos);
@@ -173,13 +184,15 @@ PHX_DEV void camera_path(const SceneView &sc, const PhxCameraParams &cam, u32 w,
                n·v < EPS: no draws */
             bool draws = !spec || !(dot(view, sn) < EPS);
             for (u32 d = 0; d < n; d++) {
-                size_t slot = (size_t)base + d;
-                S.sh_rng[slot] = rng;
-                S.sh_ev[slot] = draws ? (u32)e : NO_EVENT;
+                if (d < slots) {
+                    size_t slot = (size_t)base + d;
+                    S.sh_rng[slot] = rng;
+                    S.sh_ev[slot] = draws ? (u32)e : NO_EVENT;
+                    if (S.sh_scene)
+                        S.sh_scene[slot] = tag;
+                }
                 if (draws)
                     skip_light_fragment(rng, sc);
-                if (S.sh_scene)
-                    S.sh_scene[slot] = tag;
             }
         }
         if (diffuse)
@@ -336,9 +349,10 @@ PHX_DEV F4 group_sample_sum(F4 v, u32 n) {
 
 /* emission and direct of one pixel from its records and shadow results;
  * `summed`: the first slot of each event holds the sum over its samples
- * (group_sample_sum), the others are not read */
+ * (group_sample_sum), the others are not read; `shared`: each event has one
+ * slot, the value of each of its n samples (single_delta_light) */
 PHX_DEV void resolve_pixel(const WfState &S, u32 n, u32 wf, u32 out, phx_float4 *hp_direct, phx_float4 *hp_emission,
-                           bool summed = false) {
+                           bool summed = false, bool shared = false) {
     u32 meta = S.meta[wf];
     u32 nrec = meta & 7u;
     size_t rbase = (size_t)wf * MAX_RAY_BOUNCES;
@@ -354,7 +368,11 @@ PHX_DEV void resolve_pixel(const WfState &S, u32 n, u32 wf, u32 out, phx_float4 
            BLACK for a sample without a shadow ray (products formed by
            shadow_one) */
         F4 sum = BLACK;
-        if (summed)
+        if (shared) {
+            F4 v = ld(S.sh_tr + base);
+            for (u32 d = 0; d < n; d++)
+                sum = sum + v;
+        } else if (summed)
             sum = ld(S.sh_tr + base);
         else
             for (u32 d = 0; d < n; d++)
