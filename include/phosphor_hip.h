@@ -455,6 +455,12 @@ typedef struct PhxRenderArgs {
                                        (0 = default 5); same results for every value */
     uint32_t bvh_leaf_size;         /* triangles per leaf of the SAH tree rays are traced on, 1 ..
                                        PHX_PAIR_MAX_LEAF (0 = default 3); same results for every value */
+    /* Launch shapes (the reference's --workgroup, cmd_args.hpp:44, per
+       kernel); same results for every value.  Threads per block of the
+       camera, shadow and emission kernels: 64, 128 or 256 (0 = 128); width of
+       the gather's 64-pixel wave tile: 1, 2, 4, 8, 16, 32 or 64 pixels (height
+       64 / width; 0 = 8). */
+    uint32_t camera_block, shadow_block, emit_block, gather_tile_width;
 } PhxRenderArgs;
 
 /* Capture the pass into a hipGraph on the first run_pass and replay it. */

@@ -63,6 +63,8 @@ class RenderArgs(C.Structure):
         ("direct_samples", C.c_uint32), ("seed", C.c_uint32), ("sppm_alpha", C.c_float),
         ("initial_radius", C.c_float), ("trace_mode", C.c_uint32), ("flags", C.c_uint32),
         ("emit_wave_bounces", C.c_uint32), ("bvh_leaf_size", C.c_uint32),
+        ("camera_block", C.c_uint32), ("shadow_block", C.c_uint32), ("emit_block", C.c_uint32),
+        ("gather_tile_width", C.c_uint32),
     ]
 
 

@@ -45,7 +45,9 @@ static_assert(offsetof(PhxMaterial, metallic) == 48 && offsetof(PhxMaterial, dif
 static_assert(sizeof(PhxTextureMeta) == 16, "TextureMeta");
 static_assert(sizeof(PhxLight) == 128 && offsetof(PhxLight, kind) == 112, "Light 128 B (light.h:14-31)");
 static_assert(sizeof(PhxPhotonHashData) == 48 && offsetof(PhxPhotonHashData, grid_res) == 32, "PhotonHashData");
-static_assert(sizeof(PhxRenderArgs) == 56 && offsetof(PhxRenderArgs, bvh_leaf_size) == 52, "PhxRenderArgs");
+static_assert(sizeof(PhxRenderArgs) == 72 && offsetof(PhxRenderArgs, bvh_leaf_size) == 52 &&
+                  offsetof(PhxRenderArgs, gather_tile_width) == 68,
+              "PhxRenderArgs");
 static_assert(sizeof(PhxKernelInfo) == 52, "PhxKernelInfo");
 
 static SceneArgs scene_args(const PhxTriangleVertices *tv, const PhxTriangleNormals *tn,

@@ -55,6 +55,15 @@ struct SceneArgs {
     const PhxTriangleVertices *tvo;
 };
 
+/* Launch shapes of the one-scene pass (threads per block of the camera,
+ * shadow and emission kernels: 64, 128 or 256; the gather's 64-pixel wave
+ * tile: gather_tile_w × 64/gather_tile_w pixels, or 64 consecutive pixels
+ * when gather_tile_w is 64).  They change how work is grouped, never a
+ * result. */
+struct LaunchShape {
+    uint32_t camera = 128, shadow = 128, emit = 128, gather_tile_w = 8;
+};
+
 /* ---- the SAH pair tree (accel.cpp) ---- */
 /* Depth of the deepest interior node of a reference-layout tree (root 0);
  * an error code when a child index is out of range. */
@@ -113,13 +122,15 @@ hipError_t launch_trace_wavefront(const PhxCameraParams &cam, uint32_t w, uint32
                                   uint32_t direct_samples, const SceneArgs &sc, uint32_t shadow_any_hit,
                                   phx_float4 *hp_pos, phx_float4 *hp_normal, phx_float4 *hp_throughput,
                                   phx_float4 *hp_base_color, phx_float4 *hp_direct, phx_float4 *hp_emission,
-                                  void *ws, hipStream_t s, const uint32_t *d_seed = nullptr);
+                                  void *ws, hipStream_t s, const uint32_t *d_seed = nullptr,
+                                  const struct LaunchShape *shape = nullptr);
 
 size_t emit_workspace_bytes(uint32_t photons_to_emit, uint32_t max_bounces);
 hipError_t launch_emit(const SceneArgs &sc, uint32_t max_photons, uint32_t offset, uint32_t photons_to_emit,
                        uint32_t max_bounces, uint32_t seed, phx_float4 *photon_pos, uint32_t *photon_power,
                        uint32_t *photon_dir, uint32_t *n_photons, void *ws, hipStream_t s,
-                       const uint32_t *d_seed = nullptr, uint32_t wave_bounces = 0);
+                       const uint32_t *d_seed = nullptr, uint32_t wave_bounces = 0,
+                       const struct LaunchShape *shape = nullptr);
 
 hipError_t launch_hash(const phx_float4 *pos, uint32_t n, const PhxPhotonHashData &d, uint32_t *hashes,
                        uint32_t *counts, hipStream_t s, bool octants = false);
@@ -152,7 +163,8 @@ hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, 
                          const uint32_t *cell_start, const uint32_t *cell_end, const PhxPhotonHashData &d,
                          float alpha, phx_float4 *flux, float *photon_weight, phx_float4 *radiance, uint32_t width,
                          hipStream_t s, float *rec_ratio = nullptr, float *rec_r2 = nullptr,
-                         const phx_float4 *pdirpow = nullptr, const uint32_t *oct_start = nullptr);
+                         const phx_float4 *pdirpow = nullptr, const uint32_t *oct_start = nullptr,
+                         uint32_t gather_tile_w = 8);
 /* every stored photon record's decoded direction with its packed power in w
  * (count on the device, clamped to max_n) */
 hipError_t launch_decode_photons(const uint32_t *ppow, const uint32_t *pdir, const uint32_t *d_n, uint32_t max_n,

@@ -681,7 +681,7 @@ hipError_t launch_decode_photons(const u32 *ppow, const u32 *pdir, const u32 *d_
  * of the image as k_gather, or, with the image shape unknown (tiles_x = 0),
  * one wave per 64 consecutive pixels; blocks assigned XCD by XCD either way. */
 template <bool DECODED>
-__global__ __launch_bounds__(256) void k_gather_flat(GatherArgs a, u32 width, u32 height, u32 tiles_x) {
+__global__ __launch_bounds__(256) void k_gather_flat(GatherArgs a, u32 width, u32 height, u32 tiles_x, u32 tw_log2) {
     u32 wave = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     u32 lane = threadIdx.x & 63u;
     if (tiles_x == 0) {
@@ -689,7 +689,8 @@ __global__ __launch_bounds__(256) void k_gather_flat(GatherArgs a, u32 width, u3
         gather_pixel_flat<DECODED>(a, tid < a.n_pixels, tid, a.d, a.alpha, 0u);
         return;
     }
-    u32 x = (wave % tiles_x) * 8u + (lane & 7u), y = (wave / tiles_x) * 8u + (lane >> 3);
+    u32 tw = 1u << tw_log2, th = 64u >> tw_log2; /* the wave's tile: tw × th pixels */
+    u32 x = (wave % tiles_x) * tw + (lane & (tw - 1u)), y = (wave / tiles_x) * th + (lane >> tw_log2);
     gather_pixel_flat<DECODED>(a, x < width && y < height, y * width + x, a.d, a.alpha, 0u);
 }
 
@@ -698,7 +699,7 @@ hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, 
                          u32 n_pixels, const phx_float4 *ppos, const u32 *ppow, const u32 *pdir, const u32 *cell_start,
                          const u32 *cell_end, const PhxPhotonHashData &d, f32 alpha, phx_float4 *flux,
                          f32 *photon_weight, phx_float4 *radiance, u32 width, hipStream_t s, f32 *rec_ratio,
-                         f32 *rec_r2, const phx_float4 *pdirpow, const u32 *oct_start) {
+                         f32 *rec_r2, const phx_float4 *pdirpow, const u32 *oct_start, u32 gather_tile_w) {
     if (n_pixels == 0)
         return hipSuccess;
     GatherArgs a{hp_pos,     hp_normal, hp_throughput, hp_base_color, hp_direct, hp_emission,
@@ -709,18 +710,23 @@ hipError_t launch_gather(const phx_float4 *hp_pos, const phx_float4 *hp_normal, 
     if (width == 0) { /* image shape unknown (C ABI call): 64 consecutive pixels per wave */
         u32 waves = (n_pixels + 63) / 64;
         if (dec)
-            hipLaunchKernelGGL(k_gather_flat<true>, dim3((waves + 3) / 4), dim3(256), 0, s, a, 0u, 0u, 0u);
+            hipLaunchKernelGGL(k_gather_flat<true>, dim3((waves + 3) / 4), dim3(256), 0, s, a, 0u, 0u, 0u, 0u);
         else
-            hipLaunchKernelGGL(k_gather_flat<false>, dim3((waves + 3) / 4), dim3(256), 0, s, a, 0u, 0u, 0u);
+            hipLaunchKernelGGL(k_gather_flat<false>, dim3((waves + 3) / 4), dim3(256), 0, s, a, 0u, 0u, 0u, 0u);
         return hipGetLastError();
     }
+    u32 tw_log2 = 3;
+    for (u32 k = 0; k <= 6; k++)
+        if (gather_tile_w == (1u << k))
+            tw_log2 = k;
+    u32 tw = 1u << tw_log2, th = 64u >> tw_log2;
     u32 h = (n_pixels + width - 1) / width;
-    u32 tiles_x = (width + 7) / 8, tiles_y = (h + 7) / 8;
+    u32 tiles_x = (width + tw - 1) / tw, tiles_y = (h + th - 1) / th;
     u32 waves = tiles_x * tiles_y;
     if (dec)
-        hipLaunchKernelGGL(k_gather_flat<true>, dim3((waves + 3) / 4), dim3(256), 0, s, a, width, h, tiles_x);
+        hipLaunchKernelGGL(k_gather_flat<true>, dim3((waves + 3) / 4), dim3(256), 0, s, a, width, h, tiles_x, tw_log2);
     else
-        hipLaunchKernelGGL(k_gather_flat<false>, dim3((waves + 3) / 4), dim3(256), 0, s, a, width, h, tiles_x);
+        hipLaunchKernelGGL(k_gather_flat<false>, dim3((waves + 3) / 4), dim3(256), 0, s, a, width, h, tiles_x, tw_log2);
     return hipGetLastError();
 }
 

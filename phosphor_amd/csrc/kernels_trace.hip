@@ -285,14 +285,16 @@ struct WfCamArgs {
     CamOut o;
 };
 
-__global__ __launch_bounds__(WF_CAM_BLOCK) void k_trace_camera(WfCamArgs a) {
-    __shared__ u32 s_stack[BVH_STACK_SIZE * WF_CAM_BLOCK];
-    __shared__ f32 s_ior[IOR_STACK_SIZE * WF_CAM_BLOCK];
-    u32 tid = blockIdx.x * WF_CAM_BLOCK + threadIdx.x;
+/* B: threads per block (64, 128 or 256; the LDS stacks are laid out
+   [depth][thread] with stride B) */
+template <u32 B> __global__ __launch_bounds__(B) void k_trace_camera(WfCamArgs a) {
+    __shared__ u32 s_stack[BVH_STACK_SIZE * B];
+    __shared__ f32 s_ior[IOR_STACK_SIZE * B];
+    u32 tid = blockIdx.x * B + threadIdx.x;
     bool live = tid < a.w * a.h;
     u32 seed = a.d_seed ? *a.d_seed : a.seed;
     camera_path(a.sc, a.cam, a.w, a.h, seed, a.n, live, tid, tid, tid, 0u, a.st, a.o, s_stack + threadIdx.x,
-                s_ior + threadIdx.x, WF_CAM_BLOCK, single_delta_light(a.sc) ? 1u : a.n);
+                s_ior + threadIdx.x, B, single_delta_light(a.sc) ? 1u : a.n);
 }
 
 struct WfShadowArgs {
@@ -303,13 +305,14 @@ struct WfShadowArgs {
 };
 
 /* six waves per SIMD (76 VGPRs, no spills) */
-__global__ __launch_bounds__(WF_SHADOW_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_trace_shadow(WfShadowArgs a) {
-    __shared__ u32 s_stack[BVH_STACK_SIZE * WF_SHADOW_BLOCK];
+template <u32 B>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_trace_shadow(WfShadowArgs a) {
+    __shared__ u32 s_stack[BVH_STACK_SIZE * B];
     u32 *stk = s_stack + threadIdx.x;
     u32 count = *a.st.counter;
     bool shared = single_delta_light(a.sc); /* one slot per event (camera_path) */
-    for (u32 slot = blockIdx.x * WF_SHADOW_BLOCK + threadIdx.x; slot < count; slot += gridDim.x * WF_SHADOW_BLOCK) {
-        F4 v = a.any_hit ? shadow_value<true>(a.sc, a.st, slot, stk) : shadow_value<false>(a.sc, a.st, slot, stk);
+    for (u32 slot = blockIdx.x * B + threadIdx.x; slot < count; slot += gridDim.x * B) {
+        F4 v = a.any_hit ? shadow_value<true, B>(a.sc, a.st, slot, stk) : shadow_value<false, B>(a.sc, a.st, slot, stk);
         if (a.group && !shared) {
             /* count is a multiple of n: an event's slots pass this loop together */
             F4 sum = group_sample_sum(v, a.group);
@@ -388,7 +391,7 @@ hipError_t launch_trace_wavefront(const PhxCameraParams &cam, u32 w, u32 h, u32 
                                   const SceneArgs &sc, u32 shadow_any_hit, phx_float4 *hp_pos,
                                   phx_float4 *hp_normal, phx_float4 *hp_throughput, phx_float4 *hp_base_color,
                                   phx_float4 *hp_direct, phx_float4 *hp_emission, void *ws, hipStream_t s,
-                                  const u32 *d_seed) {
+                                  const u32 *d_seed, const LaunchShape *shape) {
     u32 n = w * h;
     if (n == 0)
         return hipSuccess;
@@ -406,7 +409,13 @@ hipError_t launch_trace_wavefront(const PhxCameraParams &cam, u32 w, u32 h, u32 
     ca.sc = sc;
     ca.st = S;
     ca.o = CamOut{hp_pos, hp_normal, hp_throughput, hp_base_color};
-    hipLaunchKernelGGL(k_trace_camera, dim3((n + WF_CAM_BLOCK - 1) / WF_CAM_BLOCK), dim3(WF_CAM_BLOCK), 0, s, ca);
+    u32 cb = shape ? shape->camera : (u32)WF_CAM_BLOCK;
+    if (cb == 64)
+        hipLaunchKernelGGL(k_trace_camera<64>, dim3((n + 63) / 64), dim3(64), 0, s, ca);
+    else if (cb == 256)
+        hipLaunchKernelGGL(k_trace_camera<256>, dim3((n + 255) / 256), dim3(256), 0, s, ca);
+    else
+        hipLaunchKernelGGL(k_trace_camera<128>, dim3((n + 127) / 128), dim3(128), 0, s, ca);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     WfShadowArgs sa;
@@ -416,10 +425,16 @@ hipError_t launch_trace_wavefront(const PhxCameraParams &cam, u32 w, u32 h, u32 
     /* every slot allocation is a multiple of direct_samples (one per event) */
     sa.group = (direct_samples > 0 && 64u % direct_samples == 0u) ? direct_samples : 0u;
     size_t expect = (size_t)n * direct_samples;
-    u32 blocks = (u32)((expect + WF_SHADOW_BLOCK - 1) / WF_SHADOW_BLOCK);
-    if (blocks > 16384)
-        blocks = 16384;
-    hipLaunchKernelGGL(k_trace_shadow, dim3(blocks), dim3(WF_SHADOW_BLOCK), 0, s, sa);
+    u32 sb = shape ? shape->shadow : (u32)WF_SHADOW_BLOCK;
+    u32 blocks = (u32)((expect + sb - 1) / sb);
+    if (blocks > 16384 * 128 / sb)
+        blocks = 16384 * 128 / sb;
+    if (sb == 64)
+        hipLaunchKernelGGL(k_trace_shadow<64>, dim3(blocks), dim3(64), 0, s, sa);
+    else if (sb == 256)
+        hipLaunchKernelGGL(k_trace_shadow<256>, dim3(blocks), dim3(256), 0, s, sa);
+    else
+        hipLaunchKernelGGL(k_trace_shadow<128>, dim3(blocks), dim3(128), 0, s, sa);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     WfResolveArgs ra;
@@ -481,25 +496,26 @@ __global__ __launch_bounds__(256) void k_emit_wf_start(EmitWfArgs a) {
  * only its own state and its own staging slots. */
 /* five waves per SIMD (96 VGPRs and 44 bytes of spills per lane) measured
    faster than four without spills */
-__global__ __launch_bounds__(EMIT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_emit_wf_bounce(EmitWfArgs a) {
-    __shared__ u32 s_stack[BVH_STACK_SIZE * EMIT_WF_BLOCK];
+template <u32 B>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) void k_emit_wf_bounce(EmitWfArgs a) {
+    __shared__ u32 s_stack[BVH_STACK_SIZE * B];
     u32 *stk = s_stack + threadIdx.x;
     const EmitWfState &S = a.st;
     u32 depth = a.depth;
     u32 n_live = depth == 0 ? a.photons : S.qcount[depth];
     const u32 *qin = S.queue[depth & 1];
     u32 *qout = S.queue[(depth + 1) & 1];
-    u32 stride = gridDim.x * EMIT_WF_BLOCK;
+    u32 stride = gridDim.x * B;
     u32 rounds = (n_live + stride - 1) / stride; /* same trip count for every lane of the grid */
     for (u32 r = 0; r < rounds; r++) {
-        if (r * stride + blockIdx.x * EMIT_WF_BLOCK >= n_live)
+        if (r * stride + blockIdx.x * B >= n_live)
             break; /* the whole block is past the live photons */
-        u32 qi = r * stride + blockIdx.x * EMIT_WF_BLOCK + threadIdx.x;
+        u32 qi = r * stride + blockIdx.x * B + threadIdx.x;
         bool keep = false;
         u32 li = 0;
         if (qi < n_live) {
             li = depth == 0 ? qi : qin[qi];
-            keep = emit_advance(a.sc, S, a.g, li, depth, a.max_bounces, true, stk);
+            keep = emit_advance<B>(a.sc, S, a.g, li, depth, a.max_bounces, true, stk);
         }
         wave_append(keep, li, &S.qcount[depth + 1], qout);
     }
@@ -508,15 +524,16 @@ __global__ __launch_bounds__(EMIT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(5
 /* Runs every photon still live at bounce a.depth to the end of its path in
  * one launch, so the long tail of the path-length distribution costs one
  * launch instead of one per bounce. */
-__global__ __launch_bounds__(EMIT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_emit_wf_finish(EmitWfArgs a) {
-    __shared__ u32 s_stack[BVH_STACK_SIZE * EMIT_WF_BLOCK];
+template <u32 B>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) void k_emit_wf_finish(EmitWfArgs a) {
+    __shared__ u32 s_stack[BVH_STACK_SIZE * B];
     u32 *stk = s_stack + threadIdx.x;
     const EmitWfState &S = a.st;
     u32 n_live = a.depth == 0 ? a.photons : S.qcount[a.depth];
     const u32 *qin = S.queue[a.depth & 1];
-    for (u32 qi = blockIdx.x * EMIT_WF_BLOCK + threadIdx.x; qi < n_live; qi += gridDim.x * EMIT_WF_BLOCK) {
+    for (u32 qi = blockIdx.x * B + threadIdx.x; qi < n_live; qi += gridDim.x * B) {
         u32 li = a.depth == 0 ? qi : qin[qi];
-        emit_advance(a.sc, S, a.g, li, a.depth, a.max_bounces, false, stk);
+        emit_advance<B>(a.sc, S, a.g, li, a.depth, a.max_bounces, false, stk);
     }
 }
 
@@ -600,7 +617,7 @@ hipError_t launch_emit_compact(const EmitLayout &L, u32 photons, u32 max_photons
 
 hipError_t launch_emit(const SceneArgs &sc, u32 max_photons, u32 offset, u32 photons, u32 max_bounces, u32 seed,
                        phx_float4 *photon_pos, u32 *photon_power, u32 *photon_dir, u32 *n_photons, void *ws,
-                       hipStream_t s, const u32 *d_seed, u32 wave_bounces) {
+                       hipStream_t s, const u32 *d_seed, u32 wave_bounces, const LaunchShape *shape) {
     if (photons == 0)
         return hipSuccess;
     EmitLayout L = emit_layout(ws, photons, max_bounces);
@@ -623,9 +640,12 @@ hipError_t launch_emit(const SceneArgs &sc, u32 max_photons, u32 offset, u32 pho
         return e;
     /* enough blocks to fill the GPU; bounces with few live photons retire
        the surplus blocks at once */
-    u32 blocks = (photons + EMIT_WF_BLOCK - 1) / EMIT_WF_BLOCK;
-    if (blocks > 4096)
-        blocks = 4096;
+    u32 eb = shape ? shape->emit : (u32)EMIT_WF_BLOCK;
+    if (eb != 64 && eb != 256)
+        eb = 128;
+    u32 blocks = (photons + eb - 1) / eb;
+    if (blocks > 4096 * 128 / eb)
+        blocks = 4096 * 128 / eb;
     /* bounce-synchronous launches while most photons are alive, then one
        launch for the rest of every path */
     if (wave_bounces == 0)
@@ -633,13 +653,23 @@ hipError_t launch_emit(const SceneArgs &sc, u32 max_photons, u32 offset, u32 pho
     wave_bounces = max_bounces < wave_bounces ? max_bounces : wave_bounces;
     for (u32 depth = 0; depth < wave_bounces; depth++) {
         a.depth = depth;
-        hipLaunchKernelGGL(k_emit_wf_bounce, dim3(blocks), dim3(EMIT_WF_BLOCK), 0, s, a);
+        if (eb == 64)
+            hipLaunchKernelGGL(k_emit_wf_bounce<64>, dim3(blocks), dim3(64), 0, s, a);
+        else if (eb == 256)
+            hipLaunchKernelGGL(k_emit_wf_bounce<256>, dim3(blocks), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL(k_emit_wf_bounce<128>, dim3(blocks), dim3(128), 0, s, a);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
     }
     if (wave_bounces < max_bounces) {
         a.depth = wave_bounces;
-        hipLaunchKernelGGL(k_emit_wf_finish, dim3(blocks), dim3(EMIT_WF_BLOCK), 0, s, a);
+        if (eb == 64)
+            hipLaunchKernelGGL(k_emit_wf_finish<64>, dim3(blocks), dim3(64), 0, s, a);
+        else if (eb == 256)
+            hipLaunchKernelGGL(k_emit_wf_finish<256>, dim3(blocks), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL(k_emit_wf_finish<128>, dim3(blocks), dim3(128), 0, s, a);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
     }
@@ -649,11 +679,11 @@ hipError_t launch_emit(const SceneArgs &sc, u32 max_photons, u32 offset, u32 pho
 bool kernel_info_trace(PhxKernelInfo *out, u32 max, u32 &n) {
     PhxKernelInfo k;
     bool ok = true;
-    PHX_INFO("k_trace_camera", k_trace_camera, WF_CAM_BLOCK)
-    PHX_INFO("k_trace_shadow", k_trace_shadow, WF_SHADOW_BLOCK)
+    PHX_INFO("k_trace_camera", k_trace_camera<128>, WF_CAM_BLOCK)
+    PHX_INFO("k_trace_shadow", k_trace_shadow<128>, WF_SHADOW_BLOCK)
     PHX_INFO("k_trace_resolve", k_trace_resolve, 256)
-    PHX_INFO("k_emit_wf_bounce", k_emit_wf_bounce, EMIT_WF_BLOCK)
-    PHX_INFO("k_emit_wf_finish", k_emit_wf_finish, EMIT_WF_BLOCK)
+    PHX_INFO("k_emit_wf_bounce", k_emit_wf_bounce<128>, EMIT_WF_BLOCK)
+    PHX_INFO("k_emit_wf_finish", k_emit_wf_finish<128>, EMIT_WF_BLOCK)
     return ok;
 }
 

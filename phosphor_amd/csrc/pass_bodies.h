@@ -308,7 +308,8 @@ PHX_DEV bool shadow_fold(const SceneView &sc, const HitRecord &rec, F4 o, F4 d, 
  * multiplied by the sample's NEE weight (the term resolve_pixel adds), BLACK
  * for a slot without a sample; ANY = any-hit traversal (exact when no
  * material transmits and the stack never overflows). */
-template <bool ANY> PHX_DEV F4 shadow_value(const SceneView &sc, const WfState &S, u32 slot, u32 *stk) {
+template <bool ANY, u32 SB = WF_SHADOW_BLOCK>
+PHX_DEV F4 shadow_value(const SceneView &sc, const WfState &S, u32 slot, u32 *stk) {
     u32 e = S.sh_ev[slot];
     F4 pre, o, d;
     f32 t_max;
@@ -318,8 +319,8 @@ template <bool ANY> PHX_DEV F4 shadow_value(const SceneView &sc, const WfState &
     F4 trans = WHITE;
     for (u32 depth = 0; depth < MAX_RAY_BOUNCES; depth++) {
         HitRecord rec;
-        if (!(ANY ? scene_any_hit(sc, o, d, t_min, t_max, rec, stk, WF_SHADOW_BLOCK)
-                  : scene_intersect(sc, o, d, t_min, t_max, rec, stk, WF_SHADOW_BLOCK)))
+        if (!(ANY ? scene_any_hit(sc, o, d, t_min, t_max, rec, stk, SB)
+                  : scene_intersect(sc, o, d, t_min, t_max, rec, stk, SB)))
             break;
         if (!shadow_fold<ANY>(sc, rec, o, d, t_min, trans))
             break;
@@ -464,11 +465,13 @@ PHX_DEV bool emit_shade(const SceneView &sc, const EmitStaging &g, u32 li, u32 d
 }
 
 /* One bounce of emit_photons.cl:49-113 on a photon's state; returns false
- * when the path ends (escape, roulette or the bounce limit). */
+ * when the path ends (escape, roulette or the bounce limit).  SB: the block
+ * size, the stride of the LDS stack. */
+template <u32 SB = EMIT_WF_BLOCK>
 PHX_DEV bool emit_step(const SceneView &sc, const EmitStaging &g, u32 li, u32 depth, u32 max_bounces, F4 &origin,
                        F4 &dir, F4 &power, F4 &throughput, u32 &rng, IorStack &ior, u32 &n, u32 *stk) {
     HitRecord rec;
-    if (!scene_intersect<true>(sc, origin, dir, EPS, INF, rec, stk, EMIT_WF_BLOCK))
+    if (!scene_intersect<true>(sc, origin, dir, EPS, INF, rec, stk, SB))
         return false;
     return emit_shade(sc, g, li, depth, max_bounces, rec, origin, dir, power, throughput, rng, ior, n);
 }
@@ -476,6 +479,7 @@ PHX_DEV bool emit_step(const SceneView &sc, const EmitStaging &g, u32 li, u32 de
 /* Loads photon li's state, runs bounces depth .. (depth + 1 if one_bounce,
  * else to the end of its path), stores what changed; returns whether the
  * photon is still live after a single bounce. */
+template <u32 SB = EMIT_WF_BLOCK>
 PHX_DEV bool emit_advance(const SceneView &sc, const EmitWfState &S, const EmitStaging &g, u32 li, u32 depth,
                           u32 max_bounces, bool one_bounce, u32 *stk) {
     F4 origin = ld(S.origin + li), dir = ld(S.dir + li);
@@ -483,7 +487,7 @@ PHX_DEV bool emit_advance(const SceneView &sc, const EmitWfState &S, const EmitS
         /* the rest of the state is loaded after the traversal, which then
            runs with fewer live registers */
         HitRecord rec;
-        if (!scene_intersect<true>(sc, origin, dir, EPS, INF, rec, stk, EMIT_WF_BLOCK))
+        if (!scene_intersect<true>(sc, origin, dir, EPS, INF, rec, stk, SB))
             return false;
         F4 power = ld(S.power + li), throughput = ld(S.thr + li);
         u32 rng = S.rng[li];
@@ -507,7 +511,7 @@ PHX_DEV bool emit_advance(const SceneView &sc, const EmitWfState &S, const EmitS
     IorStack ior{S.ior + li, g.stride, S.ior_cur[li]};
     u32 n = S.count[li], n0 = n;
     for (u32 d = depth; d < max_bounces; d++)
-        if (!emit_step(sc, g, li, d, max_bounces, origin, dir, power, throughput, rng, ior, n, stk))
+        if (!emit_step<SB>(sc, g, li, d, max_bounces, origin, dir, power, throughput, rng, ior, n, stk))
             break;
     if (n != n0)
         S.count[li] = n;

@@ -59,6 +59,11 @@ int check_render_args(const PhxRenderArgs &a, const char *who) {
         return fail(PHX_ERR_INVALID_ARG, w + ": unknown flags");
     if (a.emit_wave_bounces > 32)
         return fail(PHX_ERR_INVALID_ARG, w + ": emit_wave_bounces outside 0..32");
+    for (uint32_t b : {a.camera_block, a.shadow_block, a.emit_block})
+        if (b != 0 && b != 64 && b != 128 && b != 256)
+            return fail(PHX_ERR_INVALID_ARG, w + ": camera, shadow and emission blocks must be 64, 128 or 256 threads");
+    if (a.gather_tile_width > 64 || (a.gather_tile_width & (a.gather_tile_width - 1)) != 0)
+        return fail(PHX_ERR_INVALID_ARG, w + ": gather_tile_width must be 0 or a power of two up to 64");
     /* pixel indices, shadow-queue slots (5 NEE events per pixel at most, each
        with direct_samples light samples) and deposit slots are u32 on the device */
     uint64_t px = (uint64_t)a.width * a.height;
@@ -177,7 +182,8 @@ int PhxRenderer::trace_rays(int set) {
                                                  buffers.scene_args(), shadow_any_hit, h[0]->as<phx_float4>(),
                                                  h[1]->as<phx_float4>(), h[2]->as<phx_float4>(),
                                                  h[3]->as<phx_float4>(), h[4]->as<phx_float4>(),
-                                                 h[5]->as<phx_float4>(), d_trace_ws.p, stream, seed_ptr()),
+                                                 h[5]->as<phx_float4>(), d_trace_ws.p, stream, seed_ptr(),
+                                                 &shape),
                           "trace_rays");
     return hip_status(launch_trace(i.camera, i.image_width, i.image_height, 0u, args.direct_samples,
                                    buffers.scene_args(), h[0]->as<phx_float4>(), h[1]->as<phx_float4>(),
@@ -197,7 +203,7 @@ int PhxRenderer::emit_photons(hipStream_t s) {
         uint32_t to_emit = std::min(per_batch, ppr - off);
         PHX_TRY(launch_emit(sc, photon_buf_size, off, to_emit, args.max_photon_bounces, 0u,
                             d_photon_pos.as<phx_float4>(), d_photon_power.as<uint32_t>(), d_photon_dir.as<uint32_t>(),
-                            d_batch_size.as<uint32_t>(), d_emit_ws.p, s, seed_ptr(), args.emit_wave_bounces),
+                            d_batch_size.as<uint32_t>(), d_emit_ws.p, s, seed_ptr(), args.emit_wave_bounces, &shape),
                 "emit_photons");
     }
     PHX_TRY(launch_note_count(d_batch_size.as<uint32_t>(), d_info.as<uint32_t>(), s), "emit_photons: count");
@@ -239,7 +245,7 @@ int PhxRenderer::gather_flux(hipStream_t s, int set) {
                                     m[4]->as<uint32_t>(), m[5]->as<uint32_t>(), photon_hash_info, args.sppm_alpha,
                                     i.d_flux.as<phx_float4>(), i.d_photon_weight.as<float>(),
                                     i.d_radiance.as<phx_float4>(), i.image_width, s, rec_ratio, rec_r2,
-                                    m[3]->as<phx_float4>(), m[6]->as<uint32_t>()),
+                                    m[3]->as<phx_float4>(), m[6]->as<uint32_t>(), shape.gather_tile_w),
                       "gather_flux");
 }
 
@@ -464,6 +470,14 @@ int phx_renderer_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, Ph
         return fail(PHX_ERR_NO_DEVICE, "phx_renderer_create: no HIP device");
     std::unique_ptr<PhxRenderer> r(new PhxRenderer());
     r->args = *args;
+    if (args->camera_block)
+        r->shape.camera = args->camera_block;
+    if (args->shadow_block)
+        r->shape.shadow = args->shadow_block;
+    if (args->emit_block)
+        r->shape.emit = args->emit_block;
+    if (args->gather_tile_width)
+        r->shape.gather_tile_w = args->gather_tile_width;
     hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
     if (e == hipSuccess)
         e = hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking);

@@ -42,6 +42,7 @@ struct RenderingBuffers { /* include/buffers/rendering.hpp */
 struct PhxRenderer {
     typedef phx::DevBuf DevBuf;
     PhxRenderArgs args{};
+    phx::LaunchShape shape; /* from args: block sizes and gather tile */
     hipStream_t stream = nullptr;
     /* the emission and the photon map run on `side` while the camera trace
        runs on `stream` (the two are independent until the gather) */

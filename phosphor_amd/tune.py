@@ -6,8 +6,11 @@ Two kinds of knobs:
   (wavefront or one kernel), BVH (SAH tree visited near-first, or the
   reference's median tree visited depth-first), the SAH tree's leaf size (1
   to 4 triangles; the reference's is 1, bvh.cpp:24), emission batch size
-  (``--batch-size``, cmd_args.hpp:42) and the number of bounce-synchronous
-  emission launches.
+  (``--batch-size``, cmd_args.hpp:42), the number of bounce-synchronous
+  emission launches, and the launch shapes (the reference's ``--workgroup``,
+  cmd_args.hpp:44, per kernel): threads per block of the camera, shadow and
+  emission kernels (64, 128, 256) and the width of the gather's 64-pixel
+  wave tile (1 … 64 pixels).
   A candidate is accepted only if its SPPM state after the measured passes is
   bit-identical to the state under the given settings.
 * the grid resolution (``--grid-res``, cmd_args.hpp:51) changes the hash
@@ -76,7 +79,7 @@ class TuneResult:
 
 def autotune(cfg: PassConfig, scene, passes: int = 3, warmup: int = 1, wave_bounces=(1, 2, 3, 4, 6),
              batch_sizes=None, trace_modes=(0, 1), leaf_sizes=(1, 2, 3, 4), cell_factors=(1.0, 1.155, 1.5, 2.0),
-             rmse_tol: float = 1e-4) -> TuneResult:
+             block_sizes=(64, 128, 256), tile_widths=(4, 8, 16, 64), rmse_tol: float = 1e-4) -> TuneResult:
     """Returns the fastest accepted configuration and every trial
     (knob, value, ms per pass, exact, relative RMSE, accepted)."""
     base_ms, base_state, base_est = _run(cfg, scene, warmup, passes)
@@ -105,6 +108,11 @@ def autotune(cfg: PassConfig, scene, passes: int = 3, warmup: int = 1, wave_boun
     if batch_sizes is None:
         batch_sizes = sorted({min(cfg.photons_per_round, b) for b in (1 << 18, 1 << 20, cfg.photons_per_round)})
     consider("max_photons_per_batch", batch_sizes, True)
+    if best.trace_mode == 0:
+        consider("camera_block", [b for b in block_sizes if b != (best.camera_block or 128)], True)
+        consider("shadow_block", [b for b in block_sizes if b != (best.shadow_block or 128)], True)
+    consider("emit_block", [b for b in block_sizes if b != (best.emit_block or 128)], True)
+    consider("gather_tile_width", [t for t in tile_widths if t != (best.gather_tile_width or 8)], True)
     if cfg.initial_radius > 0.0:
         lo, hi = scene.bounds()
         extent = float((hi - lo).max()) + 2e-3

@@ -50,7 +50,7 @@ def test_struct_layouts_match_reference():
     assert abi.HASH_DTYPE.itemsize == 48 and abi.HASH_DTYPE.fields["grid_res"][1] == 32
     assert abi.UV_BLOCK_DTYPE.itemsize == 192 and abi.TEX_META_DTYPE.itemsize == 16
     # the library's own structs, as static_assert'ed in csrc/abi.cpp
-    assert C.sizeof(abi.RenderArgs) == 56 and C.sizeof(abi.KernelInfo) == 52
+    assert C.sizeof(abi.RenderArgs) == 72 and C.sizeof(abi.KernelInfo) == 52
 
 
 def test_version_and_error_paths_without_device(phx_lib):
@@ -253,6 +253,28 @@ def test_renderer_flags_checked_before_the_device(phx_lib):
     descs = (abi.SceneDesc * 1)(desc)
     assert phx_lib.phx_batch_create(C.cast(args, C.c_void_p), C.cast(descs, C.c_void_p), 1, C.byref(opt),
                                     C.byref(h)) == abi.PHX_ERR_NO_DEVICE
+
+
+def test_launch_shapes_checked_before_the_device(phx_lib):
+    """Block sizes other than 64, 128 or 256 threads and gather tile widths
+    that are not powers of two up to 64 are refused before any device work;
+    the allowed values pass the check."""
+    from phosphor_amd import PassConfig, scenes
+
+    sc = scenes.cornell_box(8, 8)
+    desc = sc.desc()
+    h = C.c_void_p()
+    base = dict(width=8, height=8, photons_per_round=16, grid_res=8)
+    for kw in (dict(camera_block=100), dict(shadow_block=32), dict(emit_block=512), dict(gather_tile_width=3),
+               dict(gather_tile_width=128)):
+        a = PassConfig(**base, **kw).to_c()
+        assert phx_lib.phx_renderer_create(C.byref(a), C.byref(desc), C.byref(h)) == abi.PHX_ERR_INVALID_ARG, kw
+        assert not h.value
+    if phx_lib.phx_device_count() == 0:
+        for kw in (dict(camera_block=64, shadow_block=256, emit_block=128, gather_tile_width=64),
+                   dict(gather_tile_width=1)):
+            a = PassConfig(**base, **kw).to_c()
+            assert phx_lib.phx_renderer_create(C.byref(a), C.byref(desc), C.byref(h)) == abi.PHX_ERR_NO_DEVICE, kw
 
 
 def test_u32_index_overflow_refused(phx_lib):
