@@ -25,6 +25,7 @@ Prints one JSON line.  With --gpus > 1 it exits: the pass runs on one GPU.
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import dataclasses
 import json
 import os
@@ -246,6 +247,18 @@ def bench_batch(args) -> int:
         b.close()
         print(f"[bench] batch ({mode}): {batch_s[mode] * 1e3 / args.steps:.1f} ms/pass", file=sys.stderr, flush=True)
     plain_s = {}
+    # a 256-byte copy (phx::k_copy16) before and after each scene's timed
+    # passes, outside the timed region: it delimits the scenes in a kernel
+    # trace, from which the fraction of each window with a kernel running
+    # (the room batching has to fill) is read
+    mark_src, mark_dst = C.c_void_p(), C.c_void_p()
+    P.check(P.lib().phx_device_malloc(C.byref(mark_src), 256), "marker")
+    P.check(P.lib().phx_device_malloc(C.byref(mark_dst), 256), "marker")
+
+    def mark():
+        P.check(P.lib().phx_device_copy(mark_dst, mark_src, 256, None), "marker")
+        P.check(P.lib().phx_device_synchronize(), "marker")
+
     for mode in ("graph", "pipelined"):
         dt = 0.0
         for i, (c, s) in enumerate(work):
@@ -253,11 +266,13 @@ def bench_batch(args) -> int:
             for _ in range(args.warmup):
                 r.run_pass()
             r.synchronize()
+            mark()
             t0 = time.perf_counter()
             for _ in range(args.steps):
                 r.run_pass()
             r.synchronize()
             dt += time.perf_counter() - t0
+            mark()
             r.close()
             if i % 16 == 15:
                 print(f"[bench] one-by-one ({mode}): {i + 1} scenes, {dt:.2f} s so far", file=sys.stderr,
