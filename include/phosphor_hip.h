@@ -476,6 +476,10 @@ typedef struct PhxRenderArgs {
  * the sorted photon map then alternate between two sets of buffers (twice
  * their device memory).  Results are identical either way. */
 #define PHX_PASS_SERIAL 4u
+/* Trace and shade each photon bounce in one kernel per bounce instead of a
+ * persistent trace kernel (lanes take new rays as their searches end) and a
+ * shading kernel.  Results are identical either way. */
+#define PHX_EMIT_FUSED 8u
 
 typedef struct PhxRenderer PhxRenderer;
 

@@ -162,27 +162,24 @@ __global__ __launch_bounds__(EMIT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(5
     __shared__ u32 s_stack[BVH_STACK_SIZE * EMIT_WF_BLOCK];
     u32 *stk = s_stack + threadIdx.x;
     const EmitWfState &S = a.st;
-    u32 depth = a.depth;
-    u32 n_live = depth == 0 ? a.photons : S.qcount[depth];
-    const u32 *qin = S.queue[depth & 1];
-    u32 *qout = S.queue[(depth + 1) & 1];
-    u32 stride = gridDim.x * EMIT_WF_BLOCK;
-    u32 rounds = (n_live + stride - 1) / stride;
-    for (u32 r = 0; r < rounds; r++) {
-        if (r * stride + blockIdx.x * EMIT_WF_BLOCK >= n_live)
-            break;
-        u32 qi = r * stride + blockIdx.x * EMIT_WF_BLOCK + threadIdx.x;
+    const u32 depth = a.depth;
+    const u32 k = blockIdx.x % EMIT_NQ, step = (gridDim.x / EMIT_NQ) * EMIT_WF_BLOCK;
+    const u32 n = emit_qlen(S, depth, k, a.photons);
+    u32 *cnt_out = emit_qcounter(S.qcount, depth + 1, k);
+    u32 *qout = S.queue[(depth + 1) & 1] + (size_t)k * S.slice;
+    for (u32 base = (blockIdx.x / EMIT_NQ) * EMIT_WF_BLOCK; base < n; base += step) {
+        u32 i = base + threadIdx.x;
         bool keep = false;
         u32 li = 0;
-        if (qi < n_live) {
-            li = depth == 0 ? qi : qin[qi];
+        if (i < n) {
+            li = emit_qentry(S, depth, k, i);
             for_each_scene(find_scene(a.ph_base, a.s0, a.s1, a.first + li), [&](u32 s0) {
                 const BatchScene &B = a.scenes[s0];
                 if (depth < B.max_bounces)
                     keep = emit_advance(B.sc, S, a.g, li, depth, B.max_bounces, true, stk);
             });
         }
-        wave_append(keep, li, &S.qcount[depth + 1], qout);
+        wave_append(keep, li, cnt_out, qout);
     }
 }
 
@@ -191,10 +188,10 @@ __global__ __launch_bounds__(EMIT_WF_BLOCK) __attribute__((amdgpu_waves_per_eu(5
     __shared__ u32 s_stack[BVH_STACK_SIZE * EMIT_WF_BLOCK];
     u32 *stk = s_stack + threadIdx.x;
     const EmitWfState &S = a.st;
-    u32 n_live = a.depth == 0 ? a.photons : S.qcount[a.depth];
-    const u32 *qin = S.queue[a.depth & 1];
-    for (u32 qi = blockIdx.x * EMIT_WF_BLOCK + threadIdx.x; qi < n_live; qi += gridDim.x * EMIT_WF_BLOCK) {
-        u32 li = a.depth == 0 ? qi : qin[qi];
+    const u32 k = blockIdx.x % EMIT_NQ, step = (gridDim.x / EMIT_NQ) * EMIT_WF_BLOCK;
+    const u32 n = emit_qlen(S, a.depth, k, a.photons);
+    for (u32 i = (blockIdx.x / EMIT_NQ) * EMIT_WF_BLOCK + threadIdx.x; i < n; i += step) {
+        u32 li = emit_qentry(S, a.depth, k, i);
         for_each_scene(find_scene(a.ph_base, a.s0, a.s1, a.first + li), [&](u32 s0) {
             const BatchScene &B = a.scenes[s0];
             if (a.depth < B.max_bounces)
@@ -331,13 +328,11 @@ hipError_t launch_batch_pass(const BatchPlan &P, const BatchStreams &B) {
         BTRY(launch_fill_u32(P.dep_counter, 0u, 1, s));
         for (const BatchGroup &G : P.emit_groups) {
             EmitLayout L = emit_layout(P.emit_ws, G.count, P.max_bounces);
-            BTRY(launch_fill_u32(L.st.qcount, 0u, (size_t)P.max_bounces + 2, s));
+            BTRY(launch_fill_u32(L.st.qcount, 0u, emit_counter_words(P.max_bounces), s));
             BEmitArgs a{P.scenes, P.ph_base, P.round_seed, G.s0, G.s1, G.first, G.count, 0u, L.st, L.g};
             hipLaunchKernelGGL(k_bemit_start, dim3((G.count + 255) / 256), dim3(256), 0, s, a);
             BLAUNCH();
-            u32 blocks = (G.count + EMIT_WF_BLOCK - 1) / EMIT_WF_BLOCK;
-            if (blocks > 4096)
-                blocks = 4096;
+            u32 blocks = emit_queue_blocks(EMIT_WF_BLOCK, L.st.slice);
             u32 wave_bounces = P.max_bounces < EMIT_WF_BOUNCES ? P.max_bounces : EMIT_WF_BOUNCES;
             for (u32 d = 0; d < wave_bounces; d++) {
                 a.depth = d;

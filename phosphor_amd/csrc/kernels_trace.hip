@@ -491,6 +491,21 @@ __global__ __launch_bounds__(256) void k_emit_wf_start(EmitWfArgs a) {
     emit_start_one(a.sc, seed, li + a.offset, li, a.st, a.photons);
 }
 
+/* Per-entry kernels walk the sub-queues with a grid that is a multiple of
+ * EMIT_NQ: block b serves sub-queue b mod EMIT_NQ, the blocks of one
+ * sub-queue stride over its entries, and survivors are appended to the same
+ * sub-queue of the next bounce.  Every loop bound is uniform over a block. */
+struct EmitQueueWalk {
+    u32 k, first, step;
+};
+PHX_DEV EmitQueueWalk emit_queue_walk(u32 B) {
+    EmitQueueWalk w;
+    w.k = blockIdx.x % EMIT_NQ;
+    w.first = (blockIdx.x / EMIT_NQ) * B;
+    w.step = (gridDim.x / EMIT_NQ) * B;
+    return w;
+}
+
 /* Bounce a.depth of every live photon: one lane per queue entry, survivors
  * appended to the next queue.  Queue order does not matter: a photon writes
  * only its own state and its own staging slots. */
@@ -501,23 +516,20 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) void k_e
     __shared__ u32 s_stack[BVH_STACK_SIZE * B];
     u32 *stk = s_stack + threadIdx.x;
     const EmitWfState &S = a.st;
-    u32 depth = a.depth;
-    u32 n_live = depth == 0 ? a.photons : S.qcount[depth];
-    const u32 *qin = S.queue[depth & 1];
-    u32 *qout = S.queue[(depth + 1) & 1];
-    u32 stride = gridDim.x * B;
-    u32 rounds = (n_live + stride - 1) / stride; /* same trip count for every lane of the grid */
-    for (u32 r = 0; r < rounds; r++) {
-        if (r * stride + blockIdx.x * B >= n_live)
-            break; /* the whole block is past the live photons */
-        u32 qi = r * stride + blockIdx.x * B + threadIdx.x;
+    const u32 depth = a.depth;
+    const EmitQueueWalk w = emit_queue_walk(B);
+    const u32 n = emit_qlen(S, depth, w.k, a.photons);
+    u32 *cnt_out = emit_qcounter(S.qcount, depth + 1, w.k);
+    u32 *qout = S.queue[(depth + 1) & 1] + (size_t)w.k * S.slice;
+    for (u32 base = w.first; base < n; base += w.step) {
+        u32 i = base + threadIdx.x;
         bool keep = false;
         u32 li = 0;
-        if (qi < n_live) {
-            li = depth == 0 ? qi : qin[qi];
+        if (i < n) {
+            li = emit_qentry(S, depth, w.k, i);
             keep = emit_advance<B>(a.sc, S, a.g, li, depth, a.max_bounces, true, stk);
         }
-        wave_append(keep, li, &S.qcount[depth + 1], qout);
+        wave_append(keep, li, cnt_out, qout);
     }
 }
 
@@ -529,11 +541,124 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) void k_e
     __shared__ u32 s_stack[BVH_STACK_SIZE * B];
     u32 *stk = s_stack + threadIdx.x;
     const EmitWfState &S = a.st;
-    u32 n_live = a.depth == 0 ? a.photons : S.qcount[a.depth];
-    const u32 *qin = S.queue[a.depth & 1];
-    for (u32 qi = blockIdx.x * B + threadIdx.x; qi < n_live; qi += gridDim.x * B) {
-        u32 li = a.depth == 0 ? qi : qin[qi];
+    const EmitQueueWalk w = emit_queue_walk(B);
+    const u32 n = emit_qlen(S, a.depth, w.k, a.photons);
+    for (u32 i = w.first + threadIdx.x; i < n; i += w.step) {
+        u32 li = emit_qentry(S, a.depth, w.k, i);
         emit_advance<B>(a.sc, S, a.g, li, a.depth, a.max_bounces, false, stk);
+    }
+}
+
+/* Split bounce, first half: the closest hit of every live photon's ray on
+ * the pair tree.  Persistent waves, each serving sub-queue (wave index mod
+ * EMIT_NQ), take entries from that sub-queue's ticket EMIT_TRACE_CHUNK at a
+ * time into a wave-private pool, and a lane whose search ends stores its hit
+ * and takes the next entry of the pool once EMIT_TRACE_REFILL lanes of its
+ * wave are idle: photon rays need from a handful to hundreds of steps, and a
+ * wave that waited for its slowest ray would leave most lanes idle (measured
+ * on the atrium: 17.5 steps per ray, 99 per wave).  Every wave ends: its
+ * ticket only grows, a wave stops fetching once the ticket has passed the
+ * sub-queue's length, and every search ends. */
+constexpr u32 EMIT_TRACE_CHUNK = 64;
+constexpr u32 EMIT_TRACE_REFILL = 16;
+/* a round of triangle tests runs when waiting lanes × weight exceed those
+   waiting for a pair step × theirs */
+#ifndef EMIT_TRACE_TRI_WEIGHT
+#define EMIT_TRACE_TRI_WEIGHT 1u
+#endif
+#ifndef EMIT_TRACE_NODE_WEIGHT
+#define EMIT_TRACE_NODE_WEIGHT 1u
+#endif
+
+template <u32 B> __global__ __launch_bounds__(B) void k_emit_trace(EmitWfArgs a) {
+    __shared__ u32 s_stack[BVH_STACK_SIZE * B];
+    u32 *stk = s_stack + threadIdx.x;
+    const EmitWfState &S = a.st;
+    const u32 depth = a.depth;
+    const u32 k = (blockIdx.x * (B / 64u) + (threadIdx.x >> 6)) % EMIT_NQ;
+    const u32 n = emit_qlen(S, depth, k, a.photons);
+    u32 *ticket = emit_qcounter(S.ticket, depth, k);
+    phx_float4 *hits = S.hit + (size_t)k * S.slice;
+    const u32 lane = threadIdx.x & 63u;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    PairSearch sr;
+    bool act = false;
+    u32 qi = 0;
+    /* the wave's pool [pnext, pend) of entries; dry: the ticket has passed n
+       (both wave-uniform) */
+    u32 pnext = 0, pend = 0;
+    bool dry = n == 0;
+    for (;;) {
+        unsigned long long idle = __ballot(!act);
+        u32 cnt = (u32)__popcll(idle);
+        if (cnt >= EMIT_TRACE_REFILL && (pnext < pend || !dry)) {
+            if (pnext >= pend) {
+                u32 base = 0;
+                u32 first = (u32)__ffsll((long long)idle) - 1u;
+                if (lane == first)
+                    base = atomicAdd(ticket, EMIT_TRACE_CHUNK);
+                base = __builtin_amdgcn_readfirstlane(__shfl(base, (int)first, 64));
+                if (base >= n) {
+                    dry = true;
+                } else {
+                    pnext = base;
+                    pend = n - base < EMIT_TRACE_CHUNK ? n : base + EMIT_TRACE_CHUNK;
+                    if (pend == n)
+                        dry = true;
+                }
+            }
+            u32 rank = (u32)__popcll(idle & lt_mask);
+            u32 avail = pend - pnext;
+            if (!act && rank < avail) {
+                qi = pnext + rank;
+                u32 li = emit_qentry(S, depth, k, qi);
+                pair_search_begin(sr, a.sc, ld(S.origin + li), ld(S.dir + li), EPS, INF);
+                act = true;
+            }
+            pnext += cnt < avail ? cnt : avail;
+        }
+        /* one round of pair steps or one round of triangle tests, whichever
+           more lanes are waiting for */
+        const bool at_leaf = act && (sr.ref & PHX_PAIR_LEAF) != 0;
+        const u32 n_act = (u32)__popcll(__ballot(act));
+        if (n_act == 0)
+            break;
+        const u32 n_tri = (u32)__popcll(__ballot(at_leaf));
+        const bool tri_round = n_tri * EMIT_TRACE_TRI_WEIGHT > (n_act - n_tri) * EMIT_TRACE_NODE_WEIGHT;
+        bool more = true;
+        if (tri_round) {
+            if (at_leaf)
+                more = pair_search_tri(sr, a.sc, stk, B);
+        } else if (act && !at_leaf) {
+            more = pair_search_node(sr, a.sc, stk, B);
+        }
+        if (act && !more) {
+            st(hits + qi, mk(sr.rec.t, sr.rec.u, sr.rec.v, phx_as_float(sr.found ? sr.rec.tri_index : EMIT_MISS)));
+            act = false;
+        }
+    }
+}
+
+/* Split bounce, second half: one lane per queue entry shades its photon at
+ * the hit k_emit_trace found, stores the deposit and the new state, and
+ * appends survivors to the next queue (emit_after_hit). */
+template <u32 B> __global__ __launch_bounds__(B) void k_emit_shade(EmitWfArgs a) {
+    const EmitWfState &S = a.st;
+    const u32 depth = a.depth;
+    const EmitQueueWalk w = emit_queue_walk(B);
+    const u32 n = emit_qlen(S, depth, w.k, a.photons);
+    u32 *cnt_out = emit_qcounter(S.qcount, depth + 1, w.k);
+    u32 *qout = S.queue[(depth + 1) & 1] + (size_t)w.k * S.slice;
+    const phx_float4 *hits = S.hit + (size_t)w.k * S.slice;
+    for (u32 base = w.first; base < n; base += w.step) {
+        u32 i = base + threadIdx.x;
+        bool keep = false;
+        u32 li = 0;
+        if (i < n) {
+            li = emit_qentry(S, depth, w.k, i);
+            keep = emit_after_hit(a.sc, S, a.g, li, depth, a.max_bounces, ld(hits + i));
+        }
+        wave_append(keep, li, cnt_out, qout);
     }
 }
 
@@ -556,12 +681,18 @@ EmitWfState emit_wf_layout(char *&p, u32 photons, u32 max_bounces) {
     p += align_up(n * 4, 256);
     S.ior = (f32 *)p;
     p += align_up(n * IOR_STACK_SIZE * 4, 256);
+    S.slice = (u32)((n + EMIT_NQ - 1) / EMIT_NQ);
+    size_t qn = (size_t)S.slice * EMIT_NQ;
     S.queue[0] = (u32 *)p;
-    p += align_up(n * 4, 256);
+    p += align_up(qn * 4, 256);
     S.queue[1] = (u32 *)p;
-    p += align_up(n * 4, 256);
+    p += align_up(qn * 4, 256);
     S.qcount = (u32 *)p;
-    p += align_up(((size_t)max_bounces + 2) * 4, 256);
+    p += align_up(emit_counter_words(max_bounces) * 4, 256);
+    S.hit = (phx_float4 *)p;
+    p += align_up(qn * 16, 256);
+    S.ticket = (u32 *)p;
+    p += align_up(emit_counter_words(max_bounces) * 4, 256);
     return S;
 }
 
@@ -576,8 +707,10 @@ size_t emit_workspace_bytes(u32 photons, u32 max_bounces) {
     b += 256;                                       /* total */
     b += align_up(scan_workspace_bytes(photons), 256);
     /* wavefront state */
-    b += 4 * align_up(n * 16, 256) + 5 * align_up(n * 4, 256) + align_up(n * IOR_STACK_SIZE * 4, 256) +
-         align_up(((size_t)max_bounces + 2) * 4, 256);
+    size_t qn = (n + EMIT_NQ - 1) / EMIT_NQ * EMIT_NQ;
+    b += 4 * align_up(n * 16, 256) + 3 * align_up(n * 4, 256) + align_up(n * IOR_STACK_SIZE * 4, 256);
+    b += 2 * align_up(qn * 4, 256) + align_up(qn * 16, 256);              /* queues, hits */
+    b += 2 * align_up(emit_counter_words(max_bounces) * 4, 256);        /* counters, tickets */
     return b;
 }
 
@@ -622,8 +755,13 @@ hipError_t launch_emit(const SceneArgs &sc, u32 max_photons, u32 offset, u32 pho
         return hipSuccess;
     EmitLayout L = emit_layout(ws, photons, max_bounces);
     const EmitWfState &S = L.st;
-    hipError_t e = launch_fill_u32(S.qcount, 0u, (size_t)max_bounces + 2, s);
+    hipError_t e = launch_fill_u32(S.qcount, 0u, emit_counter_words(max_bounces), s);
     if (e != hipSuccess)
+        return e;
+    /* split bounces need the pair tree (the reference tree keeps the fused
+       bounce kernel) */
+    const bool split = sc.near_first != 0 && (!shape || shape->emit_split);
+    if (split && (e = launch_fill_u32(S.ticket, 0u, emit_counter_words(max_bounces), s)) != hipSuccess)
         return e;
     EmitWfArgs a;
     a.sc = sc;
@@ -643,16 +781,32 @@ hipError_t launch_emit(const SceneArgs &sc, u32 max_photons, u32 offset, u32 pho
     u32 eb = shape ? shape->emit : (u32)EMIT_WF_BLOCK;
     if (eb != 64 && eb != 256)
         eb = 128;
-    u32 blocks = (photons + eb - 1) / eb;
-    if (blocks > 4096 * 128 / eb)
-        blocks = 4096 * 128 / eb;
+    u32 blocks = emit_queue_blocks(eb, S.slice);
     /* bounce-synchronous launches while most photons are alive, then one
        launch for the rest of every path */
     if (wave_bounces == 0)
         wave_bounces = EMIT_WF_BOUNCES;
     wave_bounces = max_bounces < wave_bounces ? max_bounces : wave_bounces;
+    /* persistent trace waves: a multiple of EMIT_NQ (every sub-queue is
+       served), enough to fill every CU whatever the count */
+    u32 tblocks = emit_trace_blocks(eb, photons);
     for (u32 depth = 0; depth < wave_bounces; depth++) {
         a.depth = depth;
+        if (split) {
+            if (eb == 64) {
+                hipLaunchKernelGGL(k_emit_trace<64>, dim3(tblocks), dim3(64), 0, s, a);
+                hipLaunchKernelGGL(k_emit_shade<64>, dim3(blocks), dim3(64), 0, s, a);
+            } else if (eb == 256) {
+                hipLaunchKernelGGL(k_emit_trace<256>, dim3(tblocks), dim3(256), 0, s, a);
+                hipLaunchKernelGGL(k_emit_shade<256>, dim3(blocks), dim3(256), 0, s, a);
+            } else {
+                hipLaunchKernelGGL(k_emit_trace<128>, dim3(tblocks), dim3(128), 0, s, a);
+                hipLaunchKernelGGL(k_emit_shade<128>, dim3(blocks), dim3(128), 0, s, a);
+            }
+            if ((e = hipGetLastError()) != hipSuccess)
+                return e;
+            continue;
+        }
         if (eb == 64)
             hipLaunchKernelGGL(k_emit_wf_bounce<64>, dim3(blocks), dim3(64), 0, s, a);
         else if (eb == 256)
@@ -684,6 +838,8 @@ bool kernel_info_trace(PhxKernelInfo *out, u32 max, u32 &n) {
     PHX_INFO("k_trace_resolve", k_trace_resolve, 256)
     PHX_INFO("k_emit_wf_bounce", k_emit_wf_bounce<128>, EMIT_WF_BLOCK)
     PHX_INFO("k_emit_wf_finish", k_emit_wf_finish<128>, EMIT_WF_BLOCK)
+    PHX_INFO("k_emit_trace", k_emit_trace<128>, EMIT_WF_BLOCK)
+    PHX_INFO("k_emit_shade", k_emit_shade<128>, EMIT_WF_BLOCK)
     return ok;
 }
 

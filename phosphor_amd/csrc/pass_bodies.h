@@ -397,9 +397,63 @@ struct EmitWfState {
     phx_float4 *origin, *dir, *power, *thr;
     u32 *rng, *ior_cur, *count;
     f32 *ior; /* [depth][photon] */
+    /* live photons, as EMIT_NQ sub-queues of `slice` entries: photon slot
+       li = k·slice + i starts in sub-queue k, and a photon's survivors stay
+       in its sub-queue, whose capacity is therefore `slice` */
     u32 *queue[2];
-    u32 *qcount; /* [bounce] live photons entering bounce k */
+    u32 *qcount; /* sub-queue k's length entering bounce d: emit_qcounter */
+    /* split bounces (k_emit_trace, then k_emit_shade): the hit of entry i of
+       sub-queue k at [k·slice + i] (t, u, v, position in the tree's triangle
+       order, EMIT_MISS when the ray leaves the scene) and, per bounce and
+       sub-queue, the ticket from which the trace kernel's waves take
+       entries */
+    phx_float4 *hit;
+    u32 *ticket;
+    u32 slice;
 };
+
+constexpr u32 EMIT_MISS = 0xFFFFFFFFu;
+/* Sub-queues of the live-photon queue.  A queue append or a ticket is a
+ * device-scope atomic, and atomics on one address serialise at ≈ 11 ns each
+ * on MI355X (65 536 waves appending to one counter take ≈ 0.75 ms), so each
+ * sub-queue has its own counter and ticket, one 128-byte line apart. */
+constexpr u32 EMIT_NQ = 64;
+constexpr u32 EMIT_QSTRIDE = 32; /* u32 between two counters */
+
+PHX_DEV u32 *emit_qcounter(u32 *base, u32 depth, u32 k) {
+    return base + ((size_t)depth * EMIT_NQ + k) * EMIT_QSTRIDE;
+}
+/* entries of sub-queue k entering bounce `depth` */
+PHX_DEV u32 emit_qlen(const EmitWfState &S, u32 depth, u32 k, u32 photons) {
+    if (depth == 0) {
+        u32 lo = k * S.slice;
+        return lo >= photons ? 0u : (photons - lo < S.slice ? photons - lo : S.slice);
+    }
+    return *emit_qcounter(S.qcount, depth, k);
+}
+/* u32 words of the per-bounce, per-sub-queue counters (and of the tickets) */
+inline size_t emit_counter_words(u32 max_bounces) { return ((size_t)max_bounces + 2) * EMIT_NQ * EMIT_QSTRIDE; }
+/* blocks of B threads for a per-entry emission kernel: a multiple of
+ * EMIT_NQ, at most 512 Ki threads */
+inline u32 emit_queue_blocks(u32 B, u32 slice) {
+    u32 per_q = (slice + B - 1) / B;
+    u32 cap = 4096u * 128u / B / EMIT_NQ;
+    per_q = per_q < 1 ? 1 : (per_q > cap ? cap : per_q);
+    return per_q * EMIT_NQ;
+}
+/* blocks of B threads for k_emit_trace: whole groups of EMIT_NQ waves, one
+ * wave per 64 photons, at most 8192 waves */
+inline u32 emit_trace_blocks(u32 B, u32 photons) {
+    u32 group = EMIT_NQ * 64u / B; /* blocks holding EMIT_NQ waves */
+    u32 waves = (photons + 63u) / 64u;
+    u32 groups = (waves + EMIT_NQ - 1) / EMIT_NQ;
+    groups = groups < 1 ? 1 : (groups > 8192u / EMIT_NQ ? 8192u / EMIT_NQ : groups);
+    return groups * group;
+}
+/* photon slot of entry i of sub-queue k at bounce `depth` */
+PHX_DEV u32 emit_qentry(const EmitWfState &S, u32 depth, u32 k, u32 i) {
+    return depth == 0 ? k * S.slice + i : S.queue[depth & 1][(size_t)k * S.slice + i];
+}
 
 /* deposit staging, [k][photon] with `stride` photons per row */
 struct EmitStaging {
@@ -516,6 +570,40 @@ PHX_DEV bool emit_advance(const SceneView &sc, const EmitWfState &S, const EmitS
     if (n != n0)
         S.count[li] = n;
     return false;
+}
+
+/* The part of bounce `depth` of photon li after its ray has been traced on
+ * the pair tree (k_emit_trace): h = (t, u, v, tree position or EMIT_MISS).
+ * Returns whether the photon is still live; same state and deposits as
+ * emit_advance's single bounce. */
+PHX_DEV bool emit_after_hit(const SceneView &sc, const EmitWfState &S, const EmitStaging &g, u32 li, u32 depth,
+                            u32 max_bounces, F4 h) {
+    u32 j = phx_as_uint(h.w);
+    if (j == EMIT_MISS)
+        return false;
+    HitRecord rec;
+    rec.t = h.x;
+    rec.u = h.y;
+    rec.v = h.z;
+    rec.tri_index = sc.tri_order[j];
+    rec.mat_index = sc.tmat[rec.tri_index];
+    F4 origin = ld(S.origin + li), dir = ld(S.dir + li);
+    F4 power = ld(S.power + li), throughput = ld(S.thr + li);
+    u32 rng = S.rng[li];
+    IorStack ior{S.ior + li, g.stride, S.ior_cur[li]};
+    u32 n = S.count[li], n0 = n;
+    bool keep = emit_shade(sc, g, li, depth, max_bounces, rec, origin, dir, power, throughput, rng, ior, n);
+    if (keep) {
+        st(S.origin + li, origin);
+        st(S.dir + li, dir);
+        st(S.power + li, power);
+        st(S.thr + li, throughput);
+        S.rng[li] = rng;
+        S.ior_cur[li] = ior.cur;
+    }
+    if (n != n0)
+        S.count[li] = n;
+    return keep;
 }
 
 /* appends li to a queue when keep, one atomic per wave; every lane of the

@@ -579,6 +579,104 @@ PHX_DEV bool traverse_pairs(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_ma
     return found;
 }
 
+/* The closest-hit search of traverse_pairs<false, ...> held as lane state
+ * and advanced one operation at a time (one pair step, or one triangle test),
+ * so that a persistent trace kernel can give a lane a new ray as soon as its
+ * own search ends instead of idling until the slowest ray of its wave is
+ * done, and can run the lanes that need a pair step and those that need a
+ * triangle test in separate rounds.  The hit is the same whatever the order
+ * of the operations (the argument of traverse_pairs: every leaf holding a
+ * candidate passes every box test that includes [t_min, closest]); leaf tests
+ * done later than the plain search would do them only leave `closest` larger
+ * meanwhile, which costs visits, not results.
+ * While the search runs rec.tri_index is the hit's position in the tree's
+ * triangle order (sc.tri_order maps it to the triangle index). */
+struct PairSearch {
+    RayInv ri;
+    F4 o, d;
+    f32 t_min, closest;
+    HitRecord rec;
+    int q;
+    u32 ref;
+    bool found;
+};
+
+PHX_DEV void pair_search_begin(PairSearch &s, const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max) {
+    s.ri = ray_inv(o, d);
+    s.o = o;
+    s.d = d;
+    s.t_min = t_min;
+    s.closest = t_max;
+    s.found = false;
+    s.q = -1;
+    s.ref = sc.root_ref;
+    s.rec.t = 0.0f;
+    s.rec.u = 0.0f;
+    s.rec.v = 0.0f;
+    s.rec.tri_index = 0;
+    s.rec.mat_index = 0;
+}
+
+/* Pops the next pending child; false when none is left (the search ends). */
+PHX_DEV bool pair_search_pop(PairSearch &s, u32 *stk, u32 stride) {
+    if (s.q < 0)
+        return false;
+    s.ref = stk[s.q * stride];
+    s.q--;
+    return true;
+}
+
+/* One step at the pair s.ref names (s.ref has no leaf bit): both children's
+ * boxes tested, the nearer hit child entered and the other pushed; false when
+ * the search has ended.  Only called with t_min < t_max (the reference's
+ * root test; callers check it). */
+PHX_DEV bool pair_search_node(PairSearch &s, const SceneView &sc, u32 *stk, u32 stride) {
+    const PhxBvhPair *p = sc.pairs + s.ref;
+    F4 lmin = ld(&p->lmin), lmax = ld(&p->lmax), rmin = ld(&p->rmin), rmax = ld(&p->rmax);
+    f32 llo, lhi, rlo, rhi;
+    box_span(s.ri, lmin, lmax, s.t_min, s.closest, llo, lhi);
+    box_span(s.ri, rmin, rmax, s.t_min, s.closest, rlo, rhi);
+    bool hl = llo <= lhi, hr = rlo <= rhi;
+    u32 lref = phx_as_uint(lmin.w), rref = phx_as_uint(rmin.w);
+    if (hl && hr) {
+        bool right_first = rlo < llo;
+        s.q++;
+        stk[s.q * stride] = right_first ? lref : rref;
+        s.ref = right_first ? rref : lref;
+        return true;
+    }
+    if (hl || hr) {
+        s.ref = hl ? lref : rref;
+        return true;
+    }
+    return pair_search_pop(s, stk, stride);
+}
+
+/* The first triangle left of the leaf s.ref names (PHX_PAIR_LEAF | j << 2 |
+ * count − 1): tested with traverse_pairs' rules, then s.ref names the rest of
+ * the leaf or the next pending child; false when the search has ended. */
+PHX_DEV bool pair_search_tri(PairSearch &s, const SceneView &sc, u32 *stk, u32 stride) {
+    const u32 j = (s.ref & ~PHX_PAIR_LEAF) >> 2, rest = s.ref & 3u;
+    HitRecord cand;
+    if (is_triangle_hit(sc.tvo + j, j, s.o, s.d, s.t_min, s.closest, cand)) {
+        bool take = !s.found || cand.t < s.closest;
+        if (!take) { /* cand.t == closest: the reference's order decides */
+            u32 kc = sc.tri_key[j], kb = sc.tri_key[s.rec.tri_index];
+            take = (cand.t == s.t_min) ? kc < kb : kc > kb;
+        }
+        if (take) {
+            s.closest = cand.t;
+            s.rec = cand;
+            s.found = true;
+        }
+    }
+    if (rest > 0) {
+        s.ref = PHX_PAIR_LEAF | ((j + 1u) << 2) | (rest - 1u);
+        return true;
+    }
+    return pair_search_pop(s, stk, stride);
+}
+
 /* any-hit test of shadow rays in opaque scenes: the reference tree
  * depth-first, or the SAH tree */
 PHX_DEV bool scene_any_hit(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
