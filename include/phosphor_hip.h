@@ -476,10 +476,11 @@ typedef struct PhxRenderArgs {
  * the sorted photon map then alternate between two sets of buffers (twice
  * their device memory).  Results are identical either way. */
 #define PHX_PASS_SERIAL 4u
-/* Trace and shade each photon bounce in one kernel per bounce instead of a
- * persistent trace kernel (lanes take new rays as their searches end) and a
- * shading kernel.  Results are identical either way. */
-#define PHX_EMIT_FUSED 8u
+/* Trace and shade each photon bounce in one kernel, one lane per photon,
+ * instead of a persistent trace kernel whose lanes take a new ray as soon as
+ * their search ends followed by a shading kernel.  Results are identical
+ * either way. */
+#define PHX_FUSED_TRACE 8u
 
 typedef struct PhxRenderer PhxRenderer;
 

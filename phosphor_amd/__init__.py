@@ -69,7 +69,7 @@ class PassConfig:
     reference_bvh: bool = False  # reference median BVH + depth-first order instead of SAH + near-first; same results
     bvh_leaf_size: int = 0  # triangles per leaf of the SAH tree (0 = library default 3); same results
     serial: bool = False  # each gather ends before the next pass starts (no second buffer set); same results
-    emit_fused: bool = False  # one trace+shade kernel per photon bounce instead of trace and shade kernels; same results
+    fused_trace: bool = False  # photon bounces traced and shaded in one kernel (no ray replacement); same results
     camera_block: int = 0  # threads per block of the camera kernel, 64/128/256 (0 = 128); same results
     shadow_block: int = 0  # of the shadow kernel (0 = 128); same results
     emit_block: int = 0  # of the emission bounce kernels (0 = 128); same results
@@ -80,7 +80,7 @@ class PassConfig:
                           self.max_photon_bounces, self.grid_res, self.direct_samples, self.seed,
                           self.sppm_alpha, self.initial_radius, self.trace_mode,
                           (abi.PHX_PASS_GRAPH if self.graph else 0) | (abi.PHX_REFERENCE_BVH if self.reference_bvh else 0)
-                          | (abi.PHX_PASS_SERIAL if self.serial else 0) | (abi.PHX_EMIT_FUSED if self.emit_fused else 0),
+                          | (abi.PHX_PASS_SERIAL if self.serial else 0) | (abi.PHX_FUSED_TRACE if self.fused_trace else 0),
                           self.emit_wave_bounces, self.bvh_leaf_size, self.camera_block, self.shadow_block,
                           self.emit_block, self.gather_tile_width)
 

@@ -71,7 +71,7 @@ class RenderArgs(C.Structure):
 PHX_PASS_GRAPH = 1
 PHX_REFERENCE_BVH = 2
 PHX_PASS_SERIAL = 4
-PHX_EMIT_FUSED = 8
+PHX_FUSED_TRACE = 8
 PHX_PAIR_MAX_LEAF = 4
 
 

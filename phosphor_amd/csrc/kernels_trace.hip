@@ -621,8 +621,12 @@ template <u32 B> __global__ __launch_bounds__(B) void k_emit_trace(EmitWfArgs a)
            more lanes are waiting for */
         const bool at_leaf = act && (sr.ref & PHX_PAIR_LEAF) != 0;
         const u32 n_act = (u32)__popcll(__ballot(act));
-        if (n_act == 0)
-            break;
+        if (n_act == 0) {
+            /* every lane idle: done once the ticket and the pool are used up */
+            if (dry && pnext >= pend)
+                break;
+            continue;
+        }
         const u32 n_tri = (u32)__popcll(__ballot(at_leaf));
         const bool tri_round = n_tri * EMIT_TRACE_TRI_WEIGHT > (n_act - n_tri) * EMIT_TRACE_NODE_WEIGHT;
         bool more = true;

@@ -654,11 +654,18 @@ PHX_DEV bool pair_search_node(PairSearch &s, const SceneView &sc, u32 *stk, u32 
 
 /* The first triangle left of the leaf s.ref names (PHX_PAIR_LEAF | j << 2 |
  * count − 1): tested with traverse_pairs' rules, then s.ref names the rest of
- * the leaf or the next pending child; false when the search has ended. */
+ * the leaf or the next pending child; false when the search has ended.  ANY:
+ * the search ends at the first hit (traverse_pairs<true, ...>). */
+template <bool ANY = false>
 PHX_DEV bool pair_search_tri(PairSearch &s, const SceneView &sc, u32 *stk, u32 stride) {
     const u32 j = (s.ref & ~PHX_PAIR_LEAF) >> 2, rest = s.ref & 3u;
     HitRecord cand;
     if (is_triangle_hit(sc.tvo + j, j, s.o, s.d, s.t_min, s.closest, cand)) {
+        if (ANY) {
+            s.rec = cand;
+            s.found = true;
+            return false;
+        }
         bool take = !s.found || cand.t < s.closest;
         if (!take) { /* cand.t == closest: the reference's order decides */
             u32 kc = sc.tri_key[j], kb = sc.tri_key[s.rec.tri_index];

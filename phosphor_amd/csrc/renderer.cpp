@@ -55,7 +55,7 @@ int check_render_args(const PhxRenderArgs &a, const char *who) {
         return fail(PHX_ERR_INVALID_ARG, w + ": trace_mode must be 0 or 1");
     if (a.direct_samples == 0 || a.direct_samples > 128)
         return fail(PHX_ERR_INVALID_ARG, w + ": direct_samples outside 1..128 (cmd_args.hpp:59)");
-    if (a.flags & ~(uint32_t)(PHX_PASS_GRAPH | PHX_REFERENCE_BVH | PHX_PASS_SERIAL | PHX_EMIT_FUSED))
+    if (a.flags & ~(uint32_t)(PHX_PASS_GRAPH | PHX_REFERENCE_BVH | PHX_PASS_SERIAL | PHX_FUSED_TRACE))
         return fail(PHX_ERR_INVALID_ARG, w + ": unknown flags");
     if (a.emit_wave_bounces > 32)
         return fail(PHX_ERR_INVALID_ARG, w + ": emit_wave_bounces outside 0..32");
@@ -478,7 +478,7 @@ int phx_renderer_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, Ph
         r->shape.emit = args->emit_block;
     if (args->gather_tile_width)
         r->shape.gather_tile_w = args->gather_tile_width;
-    r->shape.emit_split = (args->flags & PHX_EMIT_FUSED) ? 0u : 1u;
+    r->shape.emit_split = (args->flags & PHX_FUSED_TRACE) ? 0u : 1u;
     hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
     if (e == hipSuccess)
         e = hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking);

@@ -229,7 +229,7 @@ def test_batch_rejects_bad_arguments(phx_lib):
 
 
 def test_renderer_flags_checked_before_the_device(phx_lib):
-    """PHX_PASS_GRAPH, PHX_REFERENCE_BVH, PHX_PASS_SERIAL and PHX_EMIT_FUSED pass the argument
+    """PHX_PASS_GRAPH, PHX_REFERENCE_BVH, PHX_PASS_SERIAL and PHX_FUSED_TRACE pass the argument
     check (creation then stops at the missing device); any other bit is
     refused first."""
     import os
@@ -243,7 +243,7 @@ def test_renderer_flags_checked_before_the_device(phx_lib):
     h = C.c_void_p()
     for flags, want in ((abi.PHX_PASS_SERIAL, abi.PHX_ERR_NO_DEVICE),
                         (abi.PHX_PASS_GRAPH | abi.PHX_REFERENCE_BVH | abi.PHX_PASS_SERIAL, abi.PHX_ERR_NO_DEVICE),
-                        (abi.PHX_EMIT_FUSED, abi.PHX_ERR_NO_DEVICE),
+                        (abi.PHX_FUSED_TRACE, abi.PHX_ERR_NO_DEVICE),
                         (0x10, abi.PHX_ERR_INVALID_ARG)):
         a = PassConfig(width=8, height=8, photons_per_round=16, grid_res=8).to_c()
         a.flags = flags
