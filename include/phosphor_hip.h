@@ -526,7 +526,9 @@ int phx_renderer_read_photon_map(PhxRenderer *r, phx_float4 *pos, uint32_t *powe
  * waiting for every enqueued pass; the hitpoint and photon-map pointers are
  * the last pass's set (pipelined passes alternate between two): hp_pos …
  * hp_emission, flux, photon_weight, radiance, sorted pos/power/dir, cell
- * start/end, unsorted pos/power/dir, hash keys, NULL (the reference's cell
+ * start/end, the unsorted deposits (32-byte records: position, then packed
+ * power, octahedral direction, 0, 0; they replace the reference's unsorted
+ * pos/power/dir arrays), NULL, NULL, hash keys, NULL (the reference's cell
  * counts: the pass reads cell ranges off the sorted keys instead), stream. */
 int phx_renderer_device_buffers(PhxRenderer *r, void **ptrs, uint32_t n_ptrs);
 /* Final linear estimate (output.cpp:52-83 before exposure/gamma):

@@ -133,14 +133,19 @@ hipError_t launch_emit(const SceneArgs &sc, uint32_t max_photons, uint32_t offse
                        uint32_t max_bounces, uint32_t seed, phx_float4 *photon_pos, uint32_t *photon_power,
                        uint32_t *photon_dir, uint32_t *n_photons, void *ws, hipStream_t s,
                        const uint32_t *d_seed = nullptr, uint32_t wave_bounces = 0,
-                       const struct LaunchShape *shape = nullptr);
+                       const struct LaunchShape *shape = nullptr, phx_float4 *deposit_rec = nullptr);
+/* deposit_rec: the deposits go to 32-byte records (position; packed power,
+ * octahedral direction, 0, 0) instead of photon_pos / photon_power /
+ * photon_dir */
 
 hipError_t launch_hash(const phx_float4 *pos, uint32_t n, const PhxPhotonHashData &d, uint32_t *hashes,
                        uint32_t *counts, hipStream_t s, bool octants = false);
 /* Same with the photon count read on the device (clamped to max_n); counts
  * may be null (keys only). */
 hipError_t launch_hash_dev(const phx_float4 *pos, const uint32_t *d_n, uint32_t max_n, const PhxPhotonHashData &d,
-                           uint32_t *hashes, uint32_t *counts, hipStream_t s, bool octants = false);
+                           uint32_t *hashes, uint32_t *counts, hipStream_t s, bool octants = false,
+                           uint32_t pos_stride = 1);
+/* pos_stride: float4s from one position to the next (2 for deposit records) */
 
 hipError_t launch_scatter_atomic(const phx_float4 *pos_in, const uint32_t *pow_in, const uint32_t *dir_in,
                                  const uint32_t *hashes, uint32_t *cell_end, uint32_t n, phx_float4 *pos_out,
@@ -151,7 +156,12 @@ hipError_t launch_sort(const phx_float4 *pos_in, const uint32_t *pow_in, const u
                        const uint32_t *hashes, const uint32_t *cell_start, uint32_t *cell_end, uint32_t n,
                        uint32_t n_cells, phx_float4 *pos_out, uint32_t *pow_out, uint32_t *dir_out, void *ws,
                        hipStream_t s, const uint32_t *d_n = nullptr, uint32_t *perm_out = nullptr,
-                       uint32_t *cell_start_out = nullptr, uint32_t *oct_start_out = nullptr);
+                       uint32_t *cell_start_out = nullptr, uint32_t *oct_start_out = nullptr,
+                       const phx_float4 *rec_in = nullptr, phx_float4 *dirpow_out = nullptr);
+/* rec_in: the records come as deposit records (launch_emit's deposit_rec)
+ * instead of pos_in / pow_in / dir_in, so the last pass reads each with one
+ * access; dirpow_out: the last pass also writes each record's decoded
+ * direction with its packed power in w (launch_decode_photons' output). */
 /* With cell_start_out the cell starts come from the sorted keys (binary
  * search: the same numbers as the exclusive scan of the cell counts) and
  * cell_start is ignored, so the caller needs neither counts nor scan.  With

@@ -286,13 +286,13 @@ hipError_t noalloc_scan_scratch(void **out) {
 /* ---------------- hash (hash_photons.cl) ---------------- */
 __global__ __launch_bounds__(256) void k_hash(const phx_float4 *__restrict__ pos, u32 n, const u32 *d_n, u32 max_n,
                                               PhxPhotonHashData d, u32 *__restrict__ hashes,
-                                              u32 *__restrict__ counts, u32 octants) {
+                                              u32 *__restrict__ counts, u32 octants, u32 stride) {
     if (d_n) {
         u32 c = *d_n;
         n = c < max_n ? c : max_n;
     }
     for (u32 i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        phx_float4 p = pos[i];
+        phx_float4 p = pos[(size_t)i * stride];
         u32 h = octants ? get_hash_octant(p.x, p.y, p.z, d) : get_hash(p.x, p.y, p.z, d);
         hashes[i] = h;
         if (counts)
@@ -308,19 +308,19 @@ hipError_t launch_hash(const phx_float4 *pos, u32 n, const PhxPhotonHashData &d,
     if (blocks > 8192)
         blocks = 8192;
     hipLaunchKernelGGL(k_hash, dim3(blocks), dim3(256), 0, s, pos, n, (const u32 *)nullptr, 0u, d, hashes, counts,
-                       octants ? 1u : 0u);
+                       octants ? 1u : 0u, 1u);
     return hipGetLastError();
 }
 
 hipError_t launch_hash_dev(const phx_float4 *pos, const u32 *d_n, u32 max_n, const PhxPhotonHashData &d, u32 *hashes,
-                           u32 *counts, hipStream_t s, bool octants) {
+                           u32 *counts, hipStream_t s, bool octants, u32 pos_stride) {
     u32 blocks = (max_n + 255) / 256;
     if (blocks > 8192)
         blocks = 8192;
     if (blocks == 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_hash, dim3(blocks), dim3(256), 0, s, pos, 0u, d_n, max_n, d, hashes, counts,
-                       octants ? 1u : 0u);
+                       octants ? 1u : 0u, pos_stride);
     return hipGetLastError();
 }
 
@@ -395,7 +395,8 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(
     const u32 *__restrict__ keys_in, const u32 *__restrict__ vals_in, u32 n, const u32 *d_n, u32 shift,
     const u32 *__restrict__ offs, u32 *__restrict__ keys_out, u32 *__restrict__ vals_out, int final_pass,
     const phx_float4 *__restrict__ pos_in, const u32 *__restrict__ pow_in, const u32 *__restrict__ dir_in,
-    phx_float4 *__restrict__ pos_out, u32 *__restrict__ pow_out, u32 *__restrict__ dir_out, u32 *__restrict__ perm) {
+    phx_float4 *__restrict__ pos_out, u32 *__restrict__ pow_out, u32 *__restrict__ dir_out, u32 *__restrict__ perm,
+    const phx_float4 *__restrict__ rec_in, phx_float4 *__restrict__ dirpow_out) {
     __shared__ u32 cnt[SORT_WAVES][RADIX];
     u32 t = threadIdx.x, lane = t & 63u, w = t >> 6;
     n = sort_count(n, d_n);
@@ -449,9 +450,23 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(
             u32 dst = cnt[w][d] + rank[k];
             if (final_pass) {
                 u32 src = val[k];
-                pos_out[dst] = pos_in[src];
-                pow_out[dst] = pow_in[src];
-                dir_out[dst] = dir_in[src];
+                u32 pw, dr;
+                if (rec_in) { /* one 32-byte record per deposit */
+                    F4 p = ld(rec_in + 2 * (size_t)src), q = ld(rec_in + 2 * (size_t)src + 1);
+                    st(pos_out + dst, p);
+                    pw = phx_as_uint(q.x);
+                    dr = phx_as_uint(q.y);
+                } else {
+                    pos_out[dst] = pos_in[src];
+                    pw = pow_in[src];
+                    dr = dir_in[src];
+                }
+                pow_out[dst] = pw;
+                dir_out[dst] = dr;
+                if (dirpow_out) {
+                    F4 dv = decode_oct(dr);
+                    st(dirpow_out + dst, mk(dv.x, dv.y, dv.z, phx_as_float(pw)));
+                }
                 if (perm)
                     perm[dst] = src;
                 if (final_pass > 1) /* the sorted keys are wanted too */
@@ -544,7 +559,7 @@ constexpr u32 SORT_DEV_BLOCKS = 2048;
 hipError_t launch_sort(const phx_float4 *pos_in, const u32 *pow_in, const u32 *dir_in, const u32 *hashes,
                        const u32 *cell_start, u32 *cell_end, u32 n, u32 n_cells, phx_float4 *pos_out, u32 *pow_out,
                        u32 *dir_out, void *ws, hipStream_t s, const u32 *d_n, u32 *perm_out,
-                       u32 *cell_start_out, u32 *oct_start_out) {
+                       u32 *cell_start_out, u32 *oct_start_out, const phx_float4 *rec_in, phx_float4 *dirpow_out) {
     hipError_t e;
     const u32 *sorted_keys = hashes; /* n == 0: nothing is read */
     if (n > 0) {
@@ -589,9 +604,7 @@ hipError_t launch_sort(const phx_float4 *pos_in, const u32 *pow_in, const u32 *d
             u32 *kout = kbuf[pass & 1], *vout = vbuf[pass & 1];
             hipLaunchKernelGGL(k_radix_scatter, dim3(grid), dim3(SORT_BLOCK), 0, s, kin, vin, n, d_n, shift, offs,
                                kout, vout, last ? (cell_start_out || oct_start_out ? 2 : 1) : 0, pos_in, pow_in,
-                               dir_in, pos_out,
-                               pow_out, dir_out,
-                               perm_out);
+                               dir_in, pos_out, pow_out, dir_out, perm_out, rec_in, dirpow_out);
             if ((e = hipGetLastError()) != hipSuccess)
                 return e;
             kin = kout;

@@ -451,13 +451,16 @@ hipError_t launch_trace_wavefront(const PhxCameraParams &cam, u32 w, u32 h, u32 
 
 /* ---------------- photon emission ---------------- */
 /* Moves photon li's deposits to base + offset[li] + k (base = the counter
- * value before this batch), dropping those at or beyond max_photons. */
+ * value before this batch), dropping those at or beyond max_photons: to the
+ * reference's three arrays, or, with `rec`, to 32-byte deposit records
+ * (position; packed power, octahedral direction, 0, 0), which the sort's
+ * last pass reads with one access per record. */
 __global__ __launch_bounds__(256) void k_emit_compact(const phx_float4 *__restrict__ st_pos,
                                                       const u32 *__restrict__ st_power, const u32 *__restrict__ st_dir,
                                                       const u32 *__restrict__ count, const u32 *__restrict__ offs,
                                                       u32 photons, const u32 *__restrict__ counter, u32 max_photons,
                                                       phx_float4 *__restrict__ pos, u32 *__restrict__ power,
-                                                      u32 *__restrict__ dirs) {
+                                                      u32 *__restrict__ dirs, phx_float4 *__restrict__ rec) {
     u32 li = blockIdx.x * 256 + threadIdx.x;
     if (li >= photons)
         return;
@@ -467,9 +470,14 @@ __global__ __launch_bounds__(256) void k_emit_compact(const phx_float4 *__restri
         if (dst >= max_photons)
             break;
         size_t slot = (size_t)k * photons + li;
-        pos[dst] = st_pos[slot];
-        power[dst] = st_power[slot];
-        dirs[dst] = st_dir[slot];
+        if (rec) {
+            rec[2 * dst] = st_pos[slot];
+            st(rec + 2 * dst + 1, mk(phx_as_float(st_power[slot]), phx_as_float(st_dir[slot]), 0.0f, 0.0f));
+        } else {
+            pos[dst] = st_pos[slot];
+            power[dst] = st_power[slot];
+            dirs[dst] = st_dir[slot];
+        }
     }
 }
 
@@ -560,7 +568,9 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) void k_e
  * ticket only grows, a wave stops fetching once the ticket has passed the
  * sub-queue's length, and every search ends. */
 constexpr u32 EMIT_TRACE_CHUNK = 64;
-constexpr u32 EMIT_TRACE_REFILL = 16;
+#ifndef EMIT_TRACE_REFILL
+#define EMIT_TRACE_REFILL 16u
+#endif
 /* a round of triangle tests runs when waiting lanes × weight exceed those
    waiting for a pair step × theirs */
 #ifndef EMIT_TRACE_TRI_WEIGHT
@@ -740,12 +750,14 @@ EmitLayout emit_layout(void *ws, u32 photons, u32 max_bounces) {
 }
 
 hipError_t launch_emit_compact(const EmitLayout &L, u32 photons, u32 max_photons, phx_float4 *photon_pos,
-                               u32 *photon_power, u32 *photon_dir, u32 *n_photons, hipStream_t s) {
+                               u32 *photon_power, u32 *photon_dir, u32 *n_photons, hipStream_t s,
+                               phx_float4 *deposit_rec) {
     hipError_t e = launch_scan(L.st.count, photons, L.offs, nullptr, L.total, L.scan_ws, s);
     if (e != hipSuccess)
         return e;
     hipLaunchKernelGGL(k_emit_compact, dim3((photons + 255) / 256), dim3(256), 0, s, L.g.pos, L.g.power, L.g.dir,
-                       L.st.count, L.offs, photons, n_photons, max_photons, photon_pos, photon_power, photon_dir);
+                       L.st.count, L.offs, photons, n_photons, max_photons, photon_pos, photon_power, photon_dir,
+                       deposit_rec);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(1), 0, s, n_photons, L.total);
@@ -754,7 +766,8 @@ hipError_t launch_emit_compact(const EmitLayout &L, u32 photons, u32 max_photons
 
 hipError_t launch_emit(const SceneArgs &sc, u32 max_photons, u32 offset, u32 photons, u32 max_bounces, u32 seed,
                        phx_float4 *photon_pos, u32 *photon_power, u32 *photon_dir, u32 *n_photons, void *ws,
-                       hipStream_t s, const u32 *d_seed, u32 wave_bounces, const LaunchShape *shape) {
+                       hipStream_t s, const u32 *d_seed, u32 wave_bounces, const LaunchShape *shape,
+                       phx_float4 *deposit_rec) {
     if (photons == 0)
         return hipSuccess;
     EmitLayout L = emit_layout(ws, photons, max_bounces);
@@ -831,7 +844,8 @@ hipError_t launch_emit(const SceneArgs &sc, u32 max_photons, u32 offset, u32 pho
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
     }
-    return launch_emit_compact(L, photons, max_photons, photon_pos, photon_power, photon_dir, n_photons, s);
+    return launch_emit_compact(L, photons, max_photons, photon_pos, photon_power, photon_dir, n_photons, s,
+                               deposit_rec);
 }
 
 bool kernel_info_trace(PhxKernelInfo *out, u32 max, u32 &n) {

@@ -1107,7 +1107,8 @@ EmitLayout emit_layout(void *ws, u32 photons, u32 max_bounces);
 /* scan of the per-photon deposit counts, compaction to *n_photons + offset,
  * counter advance */
 hipError_t launch_emit_compact(const EmitLayout &L, u32 photons, u32 max_photons, phx_float4 *photon_pos,
-                               u32 *photon_power, u32 *photon_dir, u32 *n_photons, hipStream_t s);
+                               u32 *photon_power, u32 *photon_dir, u32 *n_photons, hipStream_t s,
+                               phx_float4 *deposit_rec = nullptr);
 
 } // namespace phx
 

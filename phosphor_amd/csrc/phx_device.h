@@ -684,12 +684,49 @@ PHX_DEV bool pair_search_tri(PairSearch &s, const SceneView &sc, u32 *stk, u32 s
     return pair_search_pop(s, stk, stride);
 }
 
+/* traverse_pairs<ANY, false> run as rounds of pair steps or rounds of
+ * triangle tests, whichever more lanes of the wave are waiting for (lanes
+ * whose searches end stay idle until the wave's last search ends).  Same
+ * hits as traverse_pairs (the argument of PairSearch). */
+template <bool ANY>
+PHX_DEV bool traverse_pairs_rounds(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
+                                   u32 stride) {
+    PairSearch sr;
+    bool act = t_min < t_max; /* the reference's root test */
+    if (act)
+        pair_search_begin(sr, sc, o, d, t_min, t_max);
+    for (;;) {
+        const bool at_leaf = act && (sr.ref & PHX_PAIR_LEAF) != 0;
+        const u32 n_act = (u32)__popcll(__ballot(act));
+        if (n_act == 0)
+            break;
+        const u32 n_tri = (u32)__popcll(__ballot(at_leaf));
+        if (n_tri > n_act - n_tri) {
+            if (at_leaf)
+                act = pair_search_tri<ANY>(sr, sc, stk, stride);
+        } else if (act && !at_leaf) {
+            act = pair_search_node(sr, sc, stk, stride);
+        }
+    }
+    if (!(t_min < t_max) || !sr.found)
+        return false;
+    rec = sr.rec;
+    rec.tri_index = sc.tri_order[rec.tri_index];
+    rec.mat_index = sc.tmat[rec.tri_index];
+    return true;
+}
+
+#ifndef PHX_ROUNDS_TRAVERSAL
+#define PHX_ROUNDS_TRAVERSAL 0
+#endif
+
 /* any-hit test of shadow rays in opaque scenes: the reference tree
  * depth-first, or the SAH tree */
 PHX_DEV bool scene_any_hit(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
                            u32 stride) {
     if (sc.near_first)
-        return traverse_pairs<true, false>(sc, o, d, t_min, t_max, rec, stk, stride);
+        return PHX_ROUNDS_TRAVERSAL ? traverse_pairs_rounds<true>(sc, o, d, t_min, t_max, rec, stk, stride)
+                                    : traverse_pairs<true, false>(sc, o, d, t_min, t_max, rec, stk, stride);
     return traverse<true>(sc, o, d, t_min, t_max, rec, stk, stride);
 }
 
@@ -697,7 +734,9 @@ template <bool GROUP_LEAVES = false>
 PHX_DEV bool scene_intersect(const SceneView &sc, F4 o, F4 d, f32 t_min, f32 t_max, HitRecord &rec, u32 *stk,
                              u32 stride) {
     if (sc.near_first)
-        return traverse_pairs<false, GROUP_LEAVES>(sc, o, d, t_min, t_max, rec, stk, stride);
+        return (PHX_ROUNDS_TRAVERSAL && !GROUP_LEAVES)
+                   ? traverse_pairs_rounds<false>(sc, o, d, t_min, t_max, rec, stk, stride)
+                   : traverse_pairs<false, GROUP_LEAVES>(sc, o, d, t_min, t_max, rec, stk, stride);
     return traverse<false>(sc, o, d, t_min, t_max, rec, stk, stride);
 }
 

@@ -116,9 +116,7 @@ int PhxRenderer::set_scene(const PhxSceneDesc &sc) {
     if (buf > 0xFFFFFFFFull)
         return fail(PHX_ERR_TOO_LARGE, "set_scene: photons_per_round * max_bounces overflows u32");
     photon_buf_size = (uint32_t)buf;
-    PHX_TRY(d_photon_pos.alloc((size_t)photon_buf_size * sizeof(phx_float4)), "alloc photons");
-    PHX_TRY(d_photon_power.alloc((size_t)photon_buf_size * sizeof(uint32_t)), "alloc photons");
-    PHX_TRY(d_photon_dir.alloc((size_t)photon_buf_size * sizeof(uint32_t)), "alloc photons");
+    PHX_TRY(d_deposit_rec.alloc((size_t)photon_buf_size * 2 * sizeof(phx_float4)), "alloc photons");
     PHX_TRY(d_batch_size.alloc(sizeof(uint32_t)), "alloc counter");
     PhotonBuffers &p = buffers.photon_buf;
     PHX_TRY(p.d_photon_pos.alloc((size_t)photon_buf_size * sizeof(phx_float4)), "alloc photon map");
@@ -201,9 +199,9 @@ int PhxRenderer::emit_photons(hipStream_t s) {
     SceneArgs sc = buffers.scene_args();
     for (uint32_t off = 0; off < ppr; off += per_batch) {
         uint32_t to_emit = std::min(per_batch, ppr - off);
-        PHX_TRY(launch_emit(sc, photon_buf_size, off, to_emit, args.max_photon_bounces, 0u,
-                            d_photon_pos.as<phx_float4>(), d_photon_power.as<uint32_t>(), d_photon_dir.as<uint32_t>(),
-                            d_batch_size.as<uint32_t>(), d_emit_ws.p, s, seed_ptr(), args.emit_wave_bounces, &shape),
+        PHX_TRY(launch_emit(sc, photon_buf_size, off, to_emit, args.max_photon_bounces, 0u, nullptr, nullptr, nullptr,
+                            d_batch_size.as<uint32_t>(), d_emit_ws.p, s, seed_ptr(), args.emit_wave_bounces, &shape,
+                            d_deposit_rec.as<phx_float4>()),
                 "emit_photons");
     }
     PHX_TRY(launch_note_count(d_batch_size.as<uint32_t>(), d_info.as<uint32_t>(), s), "emit_photons: count");
@@ -220,16 +218,16 @@ int PhxRenderer::hash_and_sort_photons(hipStream_t s, int set) {
     /* keys only: the stable sort orders the records, and the cell ranges
        (the scan of the cell counts, renderer.cpp:134-147) are read off the
        sorted keys */
-    PHX_TRY(launch_hash_dev(d_photon_pos.as<phx_float4>(), d_n, n_max, photon_hash_info,
-                            p.d_photon_hashes.as<uint32_t>(), nullptr, s, true),
+    const phx_float4 *rec = d_deposit_rec.as<phx_float4>();
+    PHX_TRY(launch_hash_dev(rec, d_n, n_max, photon_hash_info, p.d_photon_hashes.as<uint32_t>(), nullptr, s, true, 2u),
             "hash_photons");
-    PHX_TRY(launch_sort(d_photon_pos.as<phx_float4>(), d_photon_power.as<uint32_t>(), d_photon_dir.as<uint32_t>(),
-                        p.d_photon_hashes.as<uint32_t>(), nullptr, m[5]->as<uint32_t>(), n_max, bucket_count,
-                        m[0]->as<phx_float4>(), m[1]->as<uint32_t>(), m[2]->as<uint32_t>(), d_sort_ws.p, s, d_n,
-                        sort_perm, m[4]->as<uint32_t>(), m[6]->as<uint32_t>()),
+    /* the last radix pass moves the records and writes each one's decoded
+       direction (the gather's input) */
+    PHX_TRY(launch_sort(nullptr, nullptr, nullptr, p.d_photon_hashes.as<uint32_t>(), nullptr, m[5]->as<uint32_t>(),
+                        n_max, bucket_count, m[0]->as<phx_float4>(), m[1]->as<uint32_t>(), m[2]->as<uint32_t>(),
+                        d_sort_ws.p, s, d_n, sort_perm, m[4]->as<uint32_t>(), m[6]->as<uint32_t>(), rec,
+                        m[3]->as<phx_float4>()),
             "sort_photons");
-    PHX_TRY(launch_decode_photons(m[1]->as<uint32_t>(), m[2]->as<uint32_t>(), d_n, n_max, m[3]->as<phx_float4>(), s),
-            "decode_photons");
     return PHX_OK;
 }
 
@@ -657,8 +655,8 @@ int phx_renderer_device_buffers(PhxRenderer *r, void **ptrs, uint32_t n_ptrs) {
     void *all[] = {i.d_hp_pos.p,       i.d_hp_normal.p,     i.d_hp_throughput.p, i.d_hp_base_color.p,
                    i.d_hp_direct.p,    i.d_hp_emission.p,   i.d_flux.p,          i.d_photon_weight.p,
                    i.d_radiance.p,     p.d_photon_pos.p,    p.d_photon_power.p,  p.d_photon_dir.p,
-                   p.d_cell_start.p,   p.d_cell_end.p,      r->d_photon_pos.p,   r->d_photon_power.p,
-                   r->d_photon_dir.p,  p.d_photon_hashes.p, nullptr,             (void *)r->stream};
+                   p.d_cell_start.p,   p.d_cell_end.p,      r->d_deposit_rec.p,  nullptr,
+                   nullptr,            p.d_photon_hashes.p, nullptr,             (void *)r->stream};
     uint32_t total = (uint32_t)(sizeof(all) / sizeof(all[0]));
     for (uint32_t k = 0; k < n_ptrs && k < total; k++)
         ptrs[k] = all[k];

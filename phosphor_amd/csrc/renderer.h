@@ -84,8 +84,10 @@ struct PhxRenderer {
     PhxPhotonHashData photon_hash_info{};
     uint32_t bucket_count = 0;
     uint32_t photon_buf_size = 0;
-    /* Renderer's own unsorted photon buffers (renderer.hpp:48-51) */
-    DevBuf d_photon_pos, d_photon_power, d_photon_dir, d_batch_size;
+    /* Renderer's own unsorted photon buffers (renderer.hpp:48-51), kept as
+       32-byte records (position; packed power, octahedral direction, 0, 0):
+       the sort's last pass reads each deposit with one access */
+    DevBuf d_deposit_rec, d_batch_size;
     DevBuf d_emit_ws, d_scan_ws, d_sort_ws, d_trace_ws;
     DevBuf d_rng;  /* [0] PCG state of the round seeds, [1] and [2] the round-seed slots */
     DevBuf d_info; /* [0] last deposit count, [1] passes without deposits */
