@@ -452,7 +452,9 @@ typedef struct PhxRenderArgs {
     uint32_t trace_mode;            /* 0 = wavefront trace (default), 1 = one-kernel trace; same results */
     uint32_t flags;                 /* PHX_PASS_* */
     uint32_t emit_wave_bounces;     /* bounce-synchronous emission launches before the finishing one
-                                       (0 = default 5); same results for every value */
+                                       (0 = default: 4 split trace + shade bounces, 5 with
+                                       PHX_FUSED_TRACE or the reference tree); same results for
+                                       every value */
     uint32_t bvh_leaf_size;         /* triangles per leaf of the SAH tree rays are traced on, 1 ..
                                        PHX_PAIR_MAX_LEAF (0 = default 3); same results for every value */
     /* Launch shapes (the reference's --workgroup, cmd_args.hpp:44, per

@@ -802,7 +802,7 @@ hipError_t launch_emit(const SceneArgs &sc, u32 max_photons, u32 offset, u32 pho
     /* bounce-synchronous launches while most photons are alive, then one
        launch for the rest of every path */
     if (wave_bounces == 0)
-        wave_bounces = EMIT_WF_BOUNCES;
+        wave_bounces = split ? EMIT_SPLIT_BOUNCES : EMIT_WF_BOUNCES;
     wave_bounces = max_bounces < wave_bounces ? max_bounces : wave_bounces;
     /* persistent trace waves: a multiple of EMIT_NQ (every sub-queue is
        served), enough to fill every CU whatever the count */
