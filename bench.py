@@ -569,7 +569,8 @@ def main() -> int:
                      "duration": {"ms": g_ms, "source": "HIP events around the gather on its stream in the "
                                   "stage-timing passes (the kernel alone)"},
                      "timed_region": timed_gather},
-        "map_stage": {"kernels": "hash (octant keys), LSD radix sort, cell and octant starts, record decode",
+        "map_stage": {"kernels": "hash (octant keys) of the 32-byte deposit records, LSD radix sort whose last pass "
+                                 "moves the records and writes their decoded directions, cell and octant starts",
                       "ms": float(mean_stage[2]),
                       "GB_s": sum(map_bytes(n_photons, buckets).values()) / (float(mean_stage[2]) * 1e-3) / 1e9,
                       "note": "algorithmic bytes of hash + scan + scatter over the whole map stage's time"},
