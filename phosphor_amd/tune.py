@@ -4,8 +4,10 @@ Two kinds of knobs:
 
 * exact knobs change how the pass is launched, never its results: trace form
   (wavefront or one kernel), BVH (SAH tree visited near-first, or the
-  reference's median tree visited depth-first), the SAH tree's leaf size (1
-  to 4 triangles; the reference's is 1, bvh.cpp:24), emission batch size
+  reference's median tree visited depth-first), photon bounces traced with
+  ray replacement and shaded apart or fused (PHX_FUSED_TRACE), the SAH
+  tree's leaf size (1 to 4 triangles; the reference's is 1, bvh.cpp:24),
+  emission batch size
   (``--batch-size``, cmd_args.hpp:42), the number of bounce-synchronous
   emission launches, and the launch shapes (the reference's ``--workgroup``,
   cmd_args.hpp:44, per kernel): threads per block of the camera, shadow and
@@ -102,6 +104,8 @@ def autotune(cfg: PassConfig, scene, passes: int = 3, warmup: int = 1, wave_boun
 
     consider("trace_mode", trace_modes, True)
     consider("reference_bvh", (False, True), True)
+    if not best.reference_bvh:
+        consider("fused_trace", (True,), True)
     if not best.reference_bvh:
         consider("bvh_leaf_size", [v for v in leaf_sizes if v != (best.bvh_leaf_size or 3)], True)
     consider("emit_wave_bounces", [w for w in wave_bounces if w <= cfg.max_photon_bounces], True)

@@ -16,13 +16,13 @@ def test_autotune_small_scene():
     res = autotune(cfg, scene, passes=2, warmup=1, wave_bounces=(1, 2, 4), cell_factors=(1.155, 2.0))
     for knob, value, ms, exact, rmse, ok in res.trials:
         print(knob, value, f"{ms:.3f} ms", "exact" if exact else f"rmse {rmse:.2e}", "accepted" if ok else "rejected")
-        if knob in ("trace_mode", "reference_bvh", "bvh_leaf_size", "emit_wave_bounces", "max_photons_per_batch",
-                    "camera_block", "shadow_block", "emit_block", "gather_tile_width"):
+        if knob in ("trace_mode", "reference_bvh", "fused_trace", "bvh_leaf_size", "emit_wave_bounces",
+                    "max_photons_per_batch", "camera_block", "shadow_block", "emit_block", "gather_tile_width"):
             assert exact, (knob, value)
         if knob == "grid_res":
             assert rmse < 1e-3, (value, rmse)  # cells >= one radius find the same photons
     tried = {t[0] for t in res.trials}
-    assert {"emit_block", "gather_tile_width"} <= tried
+    assert {"emit_block", "gather_tile_width", "fused_trace"} <= tried
     assert res.ms_per_pass > 0 and res.baseline_ms > 0
     accepted = [t for t in res.trials if t[5]]
     assert any(abs(t[2] - res.ms_per_pass) < 1e-9 for t in accepted)
@@ -30,10 +30,11 @@ def test_autotune_small_scene():
 
 @pytest.mark.parametrize("case", ["point_light", "caustic"])
 def test_launch_shapes_bit_exact(case):
-    """Every block size of the camera, shadow and emission kernels and
-    several gather tiles give the oracle's pass bit for bit, on a scene lit by
-    one point light (one shadow slot per event) and on the caustic scene
-    (area light, one slot per sample)."""
+    """Every block size of the camera, shadow and emission kernels, several
+    gather tiles, fused photon bounces (PHX_FUSED_TRACE) and split bounce
+    counts give the oracle's pass bit for bit, on a scene lit by one point
+    light (one shadow slot per event) and on the caustic scene (area light,
+    one slot per sample)."""
     import oracle
     from phosphor_amd import Renderer
     from tests.test_gpu_pass import _compare_pass
@@ -56,7 +57,9 @@ def test_launch_shapes_bit_exact(case):
                     max_photon_bounces=12, grid_res=60, seed=8, initial_radius=0.012)
     shapes = [dict(camera_block=64, shadow_block=256, emit_block=64, gather_tile_width=16),
               dict(camera_block=256, shadow_block=64, emit_block=256, gather_tile_width=64),
-              dict(gather_tile_width=1), dict(gather_tile_width=4)]
+              dict(gather_tile_width=1), dict(gather_tile_width=4),
+              dict(fused_trace=True), dict(fused_trace=True, emit_block=64, emit_wave_bounces=3),
+              dict(emit_wave_bounces=1), dict(emit_wave_bounces=12, emit_block=256)]
     ref = oracle.RefRenderer(PassConfig(**base), sc)
     assert ref.run_pass() == 0
     for shape in shapes:
