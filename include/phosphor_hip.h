@@ -478,9 +478,10 @@ typedef struct PhxRenderArgs {
  * the sorted photon map then alternate between two sets of buffers (twice
  * their device memory).  Results are identical either way. */
 #define PHX_PASS_SERIAL 4u
-/* Trace and shade each photon bounce in one kernel, one lane per photon,
- * instead of a persistent trace kernel whose lanes take a new ray as soon as
- * their search ends followed by a shading kernel.  Results are identical
+/* Trace the primary camera rays inside the camera kernel, and trace and
+ * shade each photon bounce in one kernel, one lane per photon, instead of
+ * persistent trace kernels whose lanes take a new ray as soon as their search
+ * ends, ahead of the kernels that shade the hits.  Results are identical
  * either way. */
 #define PHX_FUSED_TRACE 8u
 

@@ -62,9 +62,11 @@ struct SceneArgs {
  * result. */
 struct LaunchShape {
     uint32_t camera = 128, shadow = 128, emit = 128, gather_tile_w = 8;
-    /* photon bounces as a persistent trace kernel with ray replacement plus
-       a shading kernel (pair tree only); 0: one fused kernel per bounce */
-    uint32_t emit_split = 1;
+    /* primary camera rays and photon bounces traced by persistent kernels
+       with ray replacement ahead of the kernels that shade them (pair tree
+       only); 0: traced inside the camera kernel and one fused kernel per
+       bounce */
+    uint32_t split_trace = 1;
 };
 
 /* ---- the SAH pair tree (accel.cpp) ---- */

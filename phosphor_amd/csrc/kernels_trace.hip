@@ -279,6 +279,7 @@ hipError_t launch_trace(const PhxCameraParams &cam, u32 w, u32 h, u32 seed, u32 
 struct WfCamArgs {
     PhxCameraParams cam;
     u32 w, h, seed, n;
+    u32 tiles_x; /* 8×8 pixel tiles per image row */
     const u32 *d_seed;
     SceneView sc;
     WfState st;
@@ -290,11 +291,66 @@ struct WfCamArgs {
 template <u32 B> __global__ __launch_bounds__(B) void k_trace_camera(WfCamArgs a) {
     __shared__ u32 s_stack[BVH_STACK_SIZE * B];
     __shared__ f32 s_ior[IOR_STACK_SIZE * B];
-    u32 tid = blockIdx.x * B + threadIdx.x;
-    bool live = tid < a.w * a.h;
+    /* one wave per 8×8 pixel tile: the primary rays of a wave leave the
+       camera in a narrow cone, so its lanes visit much the same nodes */
+    u32 wave = blockIdx.x * (B / 64u) + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    u32 x = (wave % a.tiles_x) * 8u + (lane & 7u), y = (wave / a.tiles_x) * 8u + (lane >> 3);
+    bool live = x < a.w && y < a.h;
+    u32 pix = live ? y * a.w + x : 0u;
     u32 seed = a.d_seed ? *a.d_seed : a.seed;
-    camera_path(a.sc, a.cam, a.w, a.h, seed, a.n, live, tid, tid, tid, 0u, a.st, a.o, s_stack + threadIdx.x,
-                s_ior + threadIdx.x, B, single_delta_light(a.sc) ? 1u : a.n);
+    u32 slots = single_delta_light(a.sc) ? 1u : a.n;
+    IorStack ior{s_ior + threadIdx.x, B, 0u};
+    CamPath p = camera_path_begin(a.cam, a.w, a.h, seed, live, pix, ior);
+    if (!a.st.cq_state) {
+        camera_path_run(a.sc, a.n, pix, pix, 0u, a.st, a.o, s_stack + threadIdx.x, B, slots, p, ior, MAX_RAY_BOUNCES);
+        camera_path_end(a.st, a.o, live, pix, pix, p);
+        return;
+    }
+    /* The first bounce here; the paths that go on (specular events, a few
+       in a hundred on diffuse scenes) continue in k_trace_camera_rest,
+       packed densely into waves, instead of holding a wave for the few
+       lanes whose paths are still going. */
+    camera_path_run(a.sc, a.n, pix, pix, 0u, a.st, a.o, s_stack + threadIdx.x, B, slots, p, ior, 1u);
+    unsigned long long m = __ballot(p.active);
+    if (m) {
+        u32 first = (u32)__ffsll((long long)m) - 1u;
+        u32 base = 0;
+        if (lane == first)
+            base = atomicAdd(a.st.cq_count, (u32)__popcll(m));
+        base = __shfl(base, (int)first, 64);
+        if (p.active) {
+            u32 q = base + (u32)__popcll(m & ((1ull << lane) - 1ull));
+            a.st.cq_pix[q] = pix;
+            camera_path_save(a.st, q, p, ior);
+        }
+    }
+    if (!p.active)
+        camera_path_end(a.st, a.o, live, pix, pix, p);
+}
+
+/* The camera paths k_trace_camera queued after their first bounce, 64 to a
+ * wave, followed to their ends. */
+template <u32 B> __global__ __launch_bounds__(B) void k_trace_camera_rest(WfCamArgs a) {
+    __shared__ u32 s_stack[BVH_STACK_SIZE * B];
+    __shared__ f32 s_ior[IOR_STACK_SIZE * B];
+    const u32 count = *a.st.cq_count;
+    const u32 lane = threadIdx.x & 63u, waves = gridDim.x * (B / 64u);
+    u32 slots = single_delta_light(a.sc) ? 1u : a.n;
+    for (u32 base = (blockIdx.x * (B / 64u) + (threadIdx.x >> 6)) * 64u; base < count; base += waves * 64u) {
+        u32 q = base + lane;
+        bool live = q < count;
+        IorStack ior{s_ior + threadIdx.x, B, 0u};
+        CamPath p;
+        u32 pix = 0;
+        if (live) {
+            pix = a.st.cq_pix[q];
+            p = camera_path_load(a.st, q, ior);
+        } else {
+            p.active = false;
+        }
+        camera_path_run(a.sc, a.n, pix, pix, 0u, a.st, a.o, s_stack + threadIdx.x, B, slots, p, ior, MAX_RAY_BOUNCES);
+        camera_path_end(a.st, a.o, live, pix, pix, p);
+    }
 }
 
 struct WfShadowArgs {
@@ -345,6 +401,17 @@ WfState wf_layout(void *ws, u32 n_pixels, u32 n, bool with_scene) {
     WfState S;
     S.counter = (u32 *)p;
     p += 256;
+    /* the camera queue's count shares the counter block (one fill clears
+       both), 128 bytes from the shadow-slot counter */
+    S.cq_count = S.counter + 32;
+    S.cq_pix = nullptr;
+    S.cq_state = nullptr;
+    if (!with_scene) {
+        S.cq_pix = (u32 *)p;
+        p += align_up((size_t)n_pixels * sizeof(u32), 256);
+        S.cq_state = (phx_float4 *)p;
+        p += align_up((size_t)n_pixels * 4u * sizeof(phx_float4), 256);
+    }
     S.rec_a = (phx_float4 *)p;
     p += align_up(rec * sizeof(phx_float4), 256);
     S.rec_thr = (phx_float4 *)p;
@@ -380,7 +447,11 @@ WfState wf_layout(void *ws, u32 n_pixels, u32 n, bool with_scene) {
 size_t trace_wavefront_workspace_bytes(u32 n_pixels, u32 n, bool with_scene) {
     size_t rec = (size_t)n_pixels * MAX_RAY_BOUNCES;
     size_t cap = rec * n;
-    return 256 + 2 * align_up(rec * sizeof(phx_float4), 256) + align_up(rec * sizeof(u32), 256) +
+    return 256 +
+           (with_scene ? 0
+                       : align_up((size_t)n_pixels * sizeof(u32), 256) +
+                             align_up((size_t)n_pixels * 4u * sizeof(phx_float4), 256)) +
+           2 * align_up(rec * sizeof(phx_float4), 256) + align_up(rec * sizeof(u32), 256) +
            align_up((size_t)n_pixels * sizeof(u32), 256) + align_up((size_t)n_pixels * sizeof(phx_float4), 256) +
            align_up((size_t)n_pixels * sizeof(f32), 256) + 5 * align_up(rec * sizeof(phx_float4), 256) +
            2 * align_up(cap * sizeof(u32), 256) + align_up(cap * sizeof(phx_float4), 256) +
@@ -396,7 +467,13 @@ hipError_t launch_trace_wavefront(const PhxCameraParams &cam, u32 w, u32 h, u32 
     if (n == 0)
         return hipSuccess;
     WfState S = wf_layout(ws, n, direct_samples, false);
-    hipError_t e = launch_fill_u32(S.counter, 0u, 1, s);
+    /* paths that go on after their first bounce in a kernel of their own */
+    const bool split = !shape || shape->split_trace;
+    if (!split) {
+        S.cq_pix = nullptr;
+        S.cq_state = nullptr;
+    }
+    hipError_t e = launch_fill_u32(S.counter, 0u, 64u, s);
     if (e != hipSuccess)
         return e;
     WfCamArgs ca;
@@ -409,15 +486,31 @@ hipError_t launch_trace_wavefront(const PhxCameraParams &cam, u32 w, u32 h, u32 
     ca.sc = sc;
     ca.st = S;
     ca.o = CamOut{hp_pos, hp_normal, hp_throughput, hp_base_color};
+    ca.tiles_x = (w + 7u) / 8u;
     u32 cb = shape ? shape->camera : (u32)WF_CAM_BLOCK;
+    cb = cb == 64 || cb == 256 ? cb : 128u;
+    u32 tiles = ca.tiles_x * ((h + 7u) / 8u), per = cb / 64u;
     if (cb == 64)
-        hipLaunchKernelGGL(k_trace_camera<64>, dim3((n + 63) / 64), dim3(64), 0, s, ca);
+        hipLaunchKernelGGL(k_trace_camera<64>, dim3((tiles + per - 1) / per), dim3(64), 0, s, ca);
     else if (cb == 256)
-        hipLaunchKernelGGL(k_trace_camera<256>, dim3((n + 255) / 256), dim3(256), 0, s, ca);
+        hipLaunchKernelGGL(k_trace_camera<256>, dim3((tiles + per - 1) / per), dim3(256), 0, s, ca);
     else
-        hipLaunchKernelGGL(k_trace_camera<128>, dim3((n + 127) / 128), dim3(128), 0, s, ca);
+        hipLaunchKernelGGL(k_trace_camera<128>, dim3((tiles + per - 1) / per), dim3(128), 0, s, ca);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
+    if (split) {
+        /* at most 8192 waves, each taking 64 queued paths at a time */
+        u32 rb = (u32)(((size_t)n + cb - 1) / cb), cap = 8192u * 64u / cb;
+        rb = rb < cap ? rb : cap;
+        if (cb == 64)
+            hipLaunchKernelGGL(k_trace_camera_rest<64>, dim3(rb), dim3(64), 0, s, ca);
+        else if (cb == 256)
+            hipLaunchKernelGGL(k_trace_camera_rest<256>, dim3(rb), dim3(256), 0, s, ca);
+        else
+            hipLaunchKernelGGL(k_trace_camera_rest<128>, dim3(rb), dim3(128), 0, s, ca);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+    }
     WfShadowArgs sa;
     sa.sc = sc;
     sa.st = S;
@@ -580,15 +673,13 @@ constexpr u32 EMIT_TRACE_CHUNK = 64;
 #define EMIT_TRACE_NODE_WEIGHT 1u
 #endif
 
-template <u32 B> __global__ __launch_bounds__(B) void k_emit_trace(EmitWfArgs a) {
-    __shared__ u32 s_stack[BVH_STACK_SIZE * B];
-    u32 *stk = s_stack + threadIdx.x;
-    const EmitWfState &S = a.st;
-    const u32 depth = a.depth;
-    const u32 k = (blockIdx.x * (B / 64u) + (threadIdx.x >> 6)) % EMIT_NQ;
-    const u32 n = emit_qlen(S, depth, k, a.photons);
-    u32 *ticket = emit_qcounter(S.ticket, depth, k);
-    phx_float4 *hits = S.hit + (size_t)k * S.slice;
+/* The persistent search loop of k_emit_trace: the calling
+ * wave serves one sub-queue of n entries whose ticket is `ticket`; src.ray(i,
+ * o, d) gives entry i's ray, src.hit(i, h) stores its hit (t, u, v, position
+ * in the tree's triangle order, EMIT_MISS for none).  Every lane of the wave
+ * calls it. */
+template <u32 B, typename Src>
+PHX_DEV void replace_trace(const SceneView &sc, u32 n, u32 *ticket, u32 *stk, Src &src) {
     const u32 lane = threadIdx.x & 63u;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     PairSearch sr;
@@ -621,8 +712,9 @@ template <u32 B> __global__ __launch_bounds__(B) void k_emit_trace(EmitWfArgs a)
             u32 avail = pend - pnext;
             if (!act && rank < avail) {
                 qi = pnext + rank;
-                u32 li = emit_qentry(S, depth, k, qi);
-                pair_search_begin(sr, a.sc, ld(S.origin + li), ld(S.dir + li), EPS, INF);
+                F4 o, d;
+                src.ray(qi, o, d);
+                pair_search_begin(sr, sc, o, d, EPS, INF);
                 act = true;
             }
             pnext += cnt < avail ? cnt : avail;
@@ -642,15 +734,36 @@ template <u32 B> __global__ __launch_bounds__(B) void k_emit_trace(EmitWfArgs a)
         bool more = true;
         if (tri_round) {
             if (at_leaf)
-                more = pair_search_tri(sr, a.sc, stk, B);
+                more = pair_search_tri(sr, sc, stk, B);
         } else if (act && !at_leaf) {
-            more = pair_search_node(sr, a.sc, stk, B);
+            more = pair_search_node(sr, sc, stk, B);
         }
         if (act && !more) {
-            st(hits + qi, mk(sr.rec.t, sr.rec.u, sr.rec.v, phx_as_float(sr.found ? sr.rec.tri_index : EMIT_MISS)));
+            src.hit(qi, mk(sr.rec.t, sr.rec.u, sr.rec.v, phx_as_float(sr.found ? sr.rec.tri_index : EMIT_MISS)));
             act = false;
         }
     }
+}
+
+struct EmitRaySrc {
+    const EmitWfState &S;
+    u32 depth, k;
+    phx_float4 *hits;
+    __device__ void ray(u32 qi, F4 &o, F4 &d) const {
+        u32 li = emit_qentry(S, depth, k, qi);
+        o = ld(S.origin + li);
+        d = ld(S.dir + li);
+    }
+    __device__ void hit(u32 qi, F4 h) const { st(hits + qi, h); }
+};
+
+template <u32 B> __global__ __launch_bounds__(B) void k_emit_trace(EmitWfArgs a) {
+    __shared__ u32 s_stack[BVH_STACK_SIZE * B];
+    const EmitWfState &S = a.st;
+    const u32 k = (blockIdx.x * (B / 64u) + (threadIdx.x >> 6)) % EMIT_NQ;
+    EmitRaySrc src{S, a.depth, k, S.hit + (size_t)k * S.slice};
+    replace_trace<B>(a.sc, emit_qlen(S, a.depth, k, a.photons), emit_qcounter(S.ticket, a.depth, k),
+                     s_stack + threadIdx.x, src);
 }
 
 /* Split bounce, second half: one lane per queue entry shades its photon at
@@ -777,7 +890,7 @@ hipError_t launch_emit(const SceneArgs &sc, u32 max_photons, u32 offset, u32 pho
         return e;
     /* split bounces need the pair tree (the reference tree keeps the fused
        bounce kernel) */
-    const bool split = sc.near_first != 0 && (!shape || shape->emit_split);
+    const bool split = sc.near_first != 0 && (!shape || shape->split_trace);
     if (split && (e = launch_fill_u32(S.ticket, 0u, emit_counter_words(max_bounces), s)) != hipSuccess)
         return e;
     EmitWfArgs a;
@@ -852,6 +965,7 @@ bool kernel_info_trace(PhxKernelInfo *out, u32 max, u32 &n) {
     PhxKernelInfo k;
     bool ok = true;
     PHX_INFO("k_trace_camera", k_trace_camera<128>, WF_CAM_BLOCK)
+    PHX_INFO("k_trace_camera_rest", k_trace_camera_rest<128>, WF_CAM_BLOCK)
     PHX_INFO("k_trace_shadow", k_trace_shadow<128>, WF_SHADOW_BLOCK)
     PHX_INFO("k_trace_resolve", k_trace_resolve, 256)
     PHX_INFO("k_emit_wf_bounce", k_emit_wf_bounce<128>, EMIT_WF_BLOCK)

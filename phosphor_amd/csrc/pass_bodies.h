@@ -41,6 +41,9 @@ constexpr u32 EMIT_SPLIT_BOUNCES = 4;
 constexpr u32 NEE_NONE = 0, NEE_DIFFUSE = 1, NEE_SPECULAR = 2;
 
 constexpr u32 NO_EVENT = 0xFFFFFFFFu;
+/* hit records of the persistent trace kernels: (t, u, v, position in the
+   tree's triangle order), w = EMIT_MISS when the ray leaves the scene */
+constexpr u32 EMIT_MISS = 0xFFFFFFFFu;
 
 struct WfState {
     phx_float4 *rec_a, *rec_thr; /* [pixel · 5 + bounce] */
@@ -57,6 +60,12 @@ struct WfState {
     u32 *sh_ev;                  /* event of the slot; NO_EVENT: no sample is drawn (adds BLACK) */
     phx_float4 *sh_tr;           /* NEE weight × shadow transmittance (BLACK without a shadow ray) */
     u32 *sh_scene;               /* scene of each slot (multi-scene queues only, else null) */
+    /* camera paths that go on after their first bounce, handed from
+       k_trace_camera to k_trace_camera_rest (null: k_trace_camera follows
+       every path to its end): their count, pixels and state, four float4
+       per path (camera_path_save) */
+    u32 *cq_count, *cq_pix;
+    phx_float4 *cq_state;
     u32 cap;
 };
 
@@ -74,32 +83,59 @@ PHX_DEV bool single_delta_light(const SceneView &sc) {
     return sc.n_lights == 1 && sc.lights[0].kind != PHX_LIGHT_TEXTURED;
 }
 
-/* One camera path of trace_rays.cl up to its first diffuse hit, with every
- * NEE event's fragments drawn and its shadow rays queued.  `pix` is the pixel
- * within its image (PCG stream and raster position), `out` its index in the
+/* A camera path of trace_rays.cl between bounces: the ray it follows, its
+ * throughput and PCG state, the bounce it is at, the NEE kinds recorded so
+ * far (2 bits per bounce), whether the envmap term was stored, whether it
+ * ended at a diffuse hit and whether it goes on.  The IOR stack lives in
+ * LDS beside it (IorStack). */
+struct CamPath {
+    F4 origin, dir, throughput;
+    u32 rng, depth, kinds, env_flag;
+    bool diffuse, active;
+};
+
+/* The path of pixel `pix` before its first bounce (trace_rays.cl:47-62):
+ * make_ray from the pixel's PCG stream, white throughput, air on the IOR
+ * stack. */
+PHX_DEV CamPath camera_path_begin(const PhxCameraParams &cam, u32 w, u32 h, u32 seed, bool live, u32 pix,
+                                  IorStack &ior) {
+    CamPath p;
+    u32 px = live ? pix % w : 0, py = live ? pix / w : 0;
+    p.rng = pcg_seed(seed + pix);
+    p.origin = ZERO;
+    p.dir = ZERO;
+    if (live)
+        make_ray(cam, px, py, w, h, p.rng, p.origin, p.dir);
+    p.throughput = WHITE;
+    init_ior_stack(ior, AIR_IOR);
+    p.depth = 0;
+    p.kinds = 0;
+    p.env_flag = 0;
+    p.diffuse = false;
+    p.active = live;
+    return p;
+}
+
+/* The bounces of a camera path (trace_rays.cl:64-161) from p.depth until it
+ * ends or reaches bounce `stop_depth`, with every NEE event's fragments
+ * drawn and its shadow rays queued.  `out` is the path's index in the
  * hitpoint arrays, `wf` its index in the wavefront state.  Every lane of the
  * wave must call this together (the queue slots are allocated wave-wide);
- * lanes with live = false only take part in the allocation.  `slots` queue
- * slots per event: n, or 1 for a scene with a single delta light (the slot
- * gets the state before the first sample). */
-PHX_DEV void camera_path(const SceneView &sc, const PhxCameraParams &cam, u32 w, u32 h, u32 seed, u32 n, bool live,
-                         u32 pix, u32 out, u32 wf, u32 tag, const WfState &S, const CamOut &o, u32 *stk, f32 *s_ior,
-                         u32 block, u32 slots) {
-    u32 px = live ? pix % w : 0, py = live ? pix / w : 0;
-    u32 rng = pcg_seed(seed + pix);
-    F4 origin = ZERO, dir = ZERO;
-    if (live)
-        make_ray(cam, px, py, w, h, rng, origin, dir);
-    F4 throughput = WHITE;
-    IorStack ior{s_ior, block, 0u};
-    init_ior_stack(ior, AIR_IOR);
-    u32 depth = 0, kinds = 0, env_flag = 0;
-    bool diffuse = false;
+ * lanes whose path is not active only take part in the allocation.  `slots`
+ * queue slots per event: n, or 1 for a scene with a single delta light (the
+ * slot gets the state before the first sample). */
+PHX_DEV void camera_path_run(const SceneView &sc, u32 n, u32 out, u32 wf, u32 tag, const WfState &S, const CamOut &o,
+                             u32 *stk, u32 block, u32 slots, CamPath &p, IorStack &ior, u32 stop_depth) {
+    F4 &origin = p.origin, &dir = p.dir, &throughput = p.throughput;
+    u32 &rng = p.rng, &depth = p.depth, &kinds = p.kinds, &env_flag = p.env_flag;
+    bool &diffuse = p.diffuse;
     size_t rbase = (size_t)wf * MAX_RAY_BOUNCES;
-    /* the loop runs while any lane of the wave is live, so the wave-wide slot
-       allocation below sees every lane */
-    bool active = live;
-    while (__any(active)) {
+    /* the loop runs while any lane of the wave has a bounce to take, so the
+       wave-wide slot allocation below sees every lane */
+    for (;;) {
+        bool active = p.active && depth < stop_depth;
+        if (!__any(active))
+            break;
         bool nee = false, spec = false;
         F4 hit_pos = ZERO, sn = ZERO, bc = ZERO, f0 = ZERO, view = ZERO;
         f32 rough = 0.0f;
@@ -117,7 +153,7 @@ PHX_DEV void camera_path(const SceneView &sc, const PhxCameraParams &cam, u32 w,
                     st(S.env + wf, throughput * sample_envmap(sc, dir));
                     env_flag = 1;
                 }
-                active = false;
+                active = p.active = false;
             } else {
                 SurfaceHit sh = process_hit(sc, rec, origin, dir);
                 const PhxMaterial &mat = sc.materials[sh.mat_index];
@@ -200,7 +236,7 @@ PHX_DEV void camera_path(const SceneView &sc, const PhxCameraParams &cam, u32 w,
             }
         }
         if (diffuse)
-            active = false;
+            active = p.active = false;
         if (active) {
             /* trace_rays.cl:149-160 */
             throughput = throughput * bs.throughput;
@@ -220,22 +256,70 @@ PHX_DEV void camera_path(const SceneView &sc, const PhxCameraParams &cam, u32 w,
             }
             depth++;
             if (killed || depth >= MAX_RAY_BOUNCES)
-                active = false;
+                active = p.active = false;
         }
     }
+}
+
+/* The outputs of a camera path that has ended (trace_rays.cl:163-168): the
+ * hitpoint of a path that found no diffuse surface, and the path's record
+ * count, NEE kinds and envmap flag. */
+PHX_DEV void camera_path_end(const WfState &S, const CamOut &o, bool live, u32 out, u32 wf, const CamPath &p) {
     if (!live)
         return;
-    if (!diffuse) {
+    if (!p.diffuse) {
         st(o.hp_pos + out, ZERO);
         st(o.hp_normal + out, ZERO);
         st(o.hp_base_color + out, BLACK);
-        st(o.hp_throughput + out, throughput);
+        st(o.hp_throughput + out, p.throughput);
     }
     /* records exist for every bounce that hit: 0..depth at a diffuse end,
        0..depth-1 after a miss (nothing recorded for the missing bounce), a
        roulette kill or the bounce limit (depth already advanced) */
-    u32 nrec = diffuse ? depth + 1 : depth;
-    S.meta[wf] = nrec | (kinds << 3) | (env_flag << 15);
+    u32 nrec = p.diffuse ? p.depth + 1 : p.depth;
+    S.meta[wf] = nrec | (p.kinds << 3) | (p.env_flag << 15);
+}
+
+/* A path that goes on after its first bounce, as entry q of the camera queue:
+ * its ray, throughput, PCG state, NEE kinds and IOR stack.  After one bounce
+ * the stack holds air and at most one more entry (one update per bounce). */
+PHX_DEV void camera_path_save(const WfState &S, u32 q, const CamPath &p, const IorStack &ior) {
+    phx_float4 *d = S.cq_state + (size_t)q * 4u;
+    st(d, p.origin);
+    st(d + 1, p.dir);
+    st(d + 2, p.throughput);
+    st(d + 3, mk(phx_as_float(p.rng), phx_as_float(p.kinds), phx_as_float(ior.cur),
+                 ior.cur ? ior.ior[ior.stride] : 0.0f));
+}
+/* the path camera_path_save stored as entry q, at bounce 1 */
+PHX_DEV CamPath camera_path_load(const WfState &S, u32 q, IorStack &ior) {
+    const phx_float4 *d = S.cq_state + (size_t)q * 4u;
+    CamPath p;
+    p.origin = ld(d);
+    p.dir = ld(d + 1);
+    p.throughput = ld(d + 2);
+    F4 m = ld(d + 3);
+    p.rng = phx_as_uint(m.x);
+    p.kinds = phx_as_uint(m.y);
+    init_ior_stack(ior, AIR_IOR);
+    ior.cur = phx_as_uint(m.z);
+    if (ior.cur)
+        ior.ior[ior.stride] = m.w;
+    p.depth = 1;
+    p.env_flag = 0;
+    p.diffuse = false;
+    p.active = true;
+    return p;
+}
+
+/* The whole camera path of pixel `pix` (camera_path_begin, _run, _end). */
+PHX_DEV void camera_path(const SceneView &sc, const PhxCameraParams &cam, u32 w, u32 h, u32 seed, u32 n, bool live,
+                         u32 pix, u32 out, u32 wf, u32 tag, const WfState &S, const CamOut &o, u32 *stk, f32 *s_ior,
+                         u32 block, u32 slots) {
+    IorStack ior{s_ior, block, 0u};
+    CamPath p = camera_path_begin(cam, w, h, seed, live, pix, ior);
+    camera_path_run(sc, n, out, wf, tag, S, o, stk, block, slots, p, ior, MAX_RAY_BOUNCES);
+    camera_path_end(S, o, live, out, wf, p);
 }
 
 /* The light sample of a shadow slot: draws it from the PCG state the camera
@@ -416,7 +500,6 @@ struct EmitWfState {
     u32 slice;
 };
 
-constexpr u32 EMIT_MISS = 0xFFFFFFFFu;
 /* Sub-queues of the live-photon queue.  A queue append or a ticket is a
  * device-scope atomic, and atomics on one address serialise at ≈ 11 ns each
  * on MI355X (65 536 waves appending to one counter take ≈ 0.75 ms), so each

@@ -476,7 +476,7 @@ int phx_renderer_create(const PhxRenderArgs *args, const PhxSceneDesc *scene, Ph
         r->shape.emit = args->emit_block;
     if (args->gather_tile_width)
         r->shape.gather_tile_w = args->gather_tile_width;
-    r->shape.emit_split = (args->flags & PHX_FUSED_TRACE) ? 0u : 1u;
+    r->shape.split_trace = (args->flags & PHX_FUSED_TRACE) ? 0u : 1u;
     hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
     if (e == hipSuccess)
         e = hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking);
