@@ -288,7 +288,10 @@ struct WfCamArgs {
 
 /* B: threads per block (64, 128 or 256; the LDS stacks are laid out
    [depth][thread] with stride B) */
-template <u32 B> __global__ __launch_bounds__(B) void k_trace_camera(WfCamArgs a) {
+/* SPLIT: paths that go on after the first bounce are queued for
+   k_trace_camera_rest; otherwise each lane follows its path to the end */
+template <u32 B, bool SPLIT>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_trace_camera(WfCamArgs a) {
     __shared__ u32 s_stack[BVH_STACK_SIZE * B];
     __shared__ f32 s_ior[IOR_STACK_SIZE * B];
     /* one wave per 8×8 pixel tile: the primary rays of a wave leave the
@@ -301,7 +304,7 @@ template <u32 B> __global__ __launch_bounds__(B) void k_trace_camera(WfCamArgs a
     u32 slots = single_delta_light(a.sc) ? 1u : a.n;
     IorStack ior{s_ior + threadIdx.x, B, 0u};
     CamPath p = camera_path_begin(a.cam, a.w, a.h, seed, live, pix, ior);
-    if (!a.st.cq_state) {
+    if (!SPLIT) {
         camera_path_run(a.sc, a.n, pix, pix, 0u, a.st, a.o, s_stack + threadIdx.x, B, slots, p, ior, MAX_RAY_BOUNCES);
         camera_path_end(a.st, a.o, live, pix, pix, p);
         return;
@@ -330,7 +333,8 @@ template <u32 B> __global__ __launch_bounds__(B) void k_trace_camera(WfCamArgs a
 
 /* The camera paths k_trace_camera queued after their first bounce, 64 to a
  * wave, followed to their ends. */
-template <u32 B> __global__ __launch_bounds__(B) void k_trace_camera_rest(WfCamArgs a) {
+template <u32 B>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_trace_camera_rest(WfCamArgs a) {
     __shared__ u32 s_stack[BVH_STACK_SIZE * B];
     __shared__ f32 s_ior[IOR_STACK_SIZE * B];
     const u32 count = *a.st.cq_count;
@@ -490,12 +494,15 @@ hipError_t launch_trace_wavefront(const PhxCameraParams &cam, u32 w, u32 h, u32 
     u32 cb = shape ? shape->camera : (u32)WF_CAM_BLOCK;
     cb = cb == 64 || cb == 256 ? cb : 128u;
     u32 tiles = ca.tiles_x * ((h + 7u) / 8u), per = cb / 64u;
+    dim3 cg((tiles + per - 1) / per);
+    void (*cam_kernel)(WfCamArgs);
     if (cb == 64)
-        hipLaunchKernelGGL(k_trace_camera<64>, dim3((tiles + per - 1) / per), dim3(64), 0, s, ca);
+        cam_kernel = split ? k_trace_camera<64, true> : k_trace_camera<64, false>;
     else if (cb == 256)
-        hipLaunchKernelGGL(k_trace_camera<256>, dim3((tiles + per - 1) / per), dim3(256), 0, s, ca);
+        cam_kernel = split ? k_trace_camera<256, true> : k_trace_camera<256, false>;
     else
-        hipLaunchKernelGGL(k_trace_camera<128>, dim3((tiles + per - 1) / per), dim3(128), 0, s, ca);
+        cam_kernel = split ? k_trace_camera<128, true> : k_trace_camera<128, false>;
+    hipLaunchKernelGGL(cam_kernel, cg, dim3(cb), 0, s, ca);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     if (split) {
@@ -964,7 +971,7 @@ hipError_t launch_emit(const SceneArgs &sc, u32 max_photons, u32 offset, u32 pho
 bool kernel_info_trace(PhxKernelInfo *out, u32 max, u32 &n) {
     PhxKernelInfo k;
     bool ok = true;
-    PHX_INFO("k_trace_camera", k_trace_camera<128>, WF_CAM_BLOCK)
+    PHX_INFO("k_trace_camera", (k_trace_camera<128, true>), WF_CAM_BLOCK)
     PHX_INFO("k_trace_camera_rest", k_trace_camera_rest<128>, WF_CAM_BLOCK)
     PHX_INFO("k_trace_shadow", k_trace_shadow<128>, WF_SHADOW_BLOCK)
     PHX_INFO("k_trace_resolve", k_trace_resolve, 256)
