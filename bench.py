@@ -409,7 +409,7 @@ def main() -> int:
     ap.add_argument("--serial", action="store_true",
                     help="each pass's gather ends before the next pass starts (default: pipelined; same results)")
     ap.add_argument("--stage-passes", type=int, default=4,
-                    help="passes after the timed region, each waited for, that give the stage timings")
+                    help="passes after the timed region, each waited for, that give the stage timings (at least 1)")
     ap.add_argument("--abi-passes", type=int, default=2,
                     help="passes of the reference's sequence through the stage-level C ABI, timed per stage "
                          "(0 = skip)")
@@ -452,7 +452,7 @@ def main() -> int:
     # stage timings: further passes, each waited for before the next is enqueued
     stage = []
     counts = []
-    for i in range(args.stage_passes):
+    for i in range(max(1, args.stage_passes)):
         r.run_pass()
         stage.append(r.stage_ms())
         counts.append(r.last_photon_count)

@@ -210,7 +210,7 @@ PHX_DEV void camera_path_run(const SceneView &sc, u32 n, u32 out, u32 wf, u32 ta
             /* The light samples are drawn by the shadow kernel, one lane per
                sample, from the event recorded here and the PCG state before
                each sample; this lane only steps its stream past the draws
-               the samples take (skip_light_fragment), so the path goes on
+               the samples take (skip_light_fragments), so the path goes on
                with the state the reference has after direct lighting. */
             u32 base = wave_base + rank * slots;
             size_t e = rbase + depth;
@@ -223,17 +223,17 @@ PHX_DEV void camera_path_run(const SceneView &sc, u32 n, u32 out, u32 wf, u32 ta
             /* direct_lighting_specular returns BLACK before sampling when
                n·v < EPS: no draws */
             bool draws = !spec || !(dot(view, sn) < EPS);
-            for (u32 d = 0; d < n; d++) {
-                if (d < slots) {
-                    size_t slot = (size_t)base + d;
-                    S.sh_rng[slot] = rng;
-                    S.sh_ev[slot] = draws ? (u32)e : NO_EVENT;
-                    if (S.sh_scene)
-                        S.sh_scene[slot] = tag;
-                }
+            for (u32 d = 0; d < slots; d++) {
+                size_t slot = (size_t)base + d;
+                S.sh_rng[slot] = rng;
+                S.sh_ev[slot] = draws ? (u32)e : NO_EVENT;
+                if (S.sh_scene)
+                    S.sh_scene[slot] = tag;
                 if (draws)
-                    skip_light_fragment(rng, sc);
+                    skip_light_fragments(rng, sc, 1u);
             }
+            if (draws && n > slots)
+                skip_light_fragments(rng, sc, n - slots);
         }
         if (diffuse)
             active = p.active = false;

@@ -1251,6 +1251,44 @@ PHX_DEV void skip_light_fragment(u32 &rng, const SceneView &sc) {
     (void)random_u32(rng);
 }
 
+/* The PCG state `steps` draws later, by the LCG jump-ahead of F. Brown,
+ * "Random number generation with arbitrary strides" (1994): the map
+ * x -> a·x + c applied k times is x -> A·x + C with A and C formed from the
+ * binary digits of k by squaring the map (all mod 2^32). */
+PHX_DEV u32 pcg_advance(u32 state, u32 steps) {
+    u32 acc_mult = 1u, acc_plus = 0u, cur_mult = 747796405u, cur_plus = 2891336453u;
+    while (steps) {
+        if (steps & 1u) {
+            acc_mult *= cur_mult;
+            acc_plus = acc_plus * cur_mult + cur_plus;
+        }
+        cur_plus = (cur_mult + 1u) * cur_plus;
+        cur_mult *= cur_mult;
+        steps >>= 1;
+    }
+    return acc_mult * state + acc_plus;
+}
+
+/* skip_light_fragment `count` times.  With one light every fragment selects
+ * it and takes the same number of draws (one, plus for an emissive mesh of
+ * area >= EPS one for the triangle when it has several and two for the
+ * point), so the state jumps by count times that many steps at once. */
+PHX_DEV void skip_light_fragments(u32 &rng, const SceneView &sc, u32 count) {
+    if (sc.n_lights == 0 || sc.total_lum < EPS)
+        return;
+    if (sc.n_lights == 1) {
+        const PhxLight &l = sc.lights[0];
+        u32 per = 1u;
+        if (l.kind != PHX_LIGHT_POINT && l.kind != PHX_LIGHT_SPOT && l.kind != PHX_LIGHT_DIRECTIONAL &&
+            !(l.aux.w < EPS))
+            per += (phx_as_uint(l.aux.z) > 1u && l.aux.w > EPS) ? 3u : 2u;
+        rng = pcg_advance(rng, count * per);
+        return;
+    }
+    for (u32 i = 0; i < count; i++)
+        skip_light_fragment(rng, sc);
+}
+
 /* light_sampling.h:231-300 */
 PHX_DEV bool sample_light_fragment(u32 &rng, const SceneView &sc, F4 pos, F4 normal, LightFragment &out) {
     if (sc.n_lights == 0 || sc.total_lum < EPS)
