@@ -69,7 +69,7 @@ class PassConfig:
     reference_bvh: bool = False  # reference median BVH + depth-first order instead of SAH + near-first; same results
     bvh_leaf_size: int = 0  # triangles per leaf of the SAH tree (0 = library default 3); same results
     serial: bool = False  # each gather ends before the next pass starts (no second buffer set); same results
-    fused_trace: bool = False  # photon bounces traced and shaded in one kernel (no ray replacement); same results
+    fused_trace: bool = False  # camera paths in one kernel, photon bounces traced and shaded in one kernel; same results
     camera_block: int = 0  # threads per block of the camera kernel, 64/128/256 (0 = 128); same results
     shadow_block: int = 0  # of the shadow kernel (0 = 128); same results
     emit_block: int = 0  # of the emission bounce kernels (0 = 128); same results

@@ -4,8 +4,9 @@ Two kinds of knobs:
 
 * exact knobs change how the pass is launched, never its results: trace form
   (wavefront or one kernel), BVH (SAH tree visited near-first, or the
-  reference's median tree visited depth-first), photon bounces traced with
-  ray replacement and shaded apart or fused (PHX_FUSED_TRACE), the SAH
+  reference's median tree visited depth-first), camera paths continued in
+  a second kernel and photon bounces traced with ray replacement and shaded
+  apart, or both fused (PHX_FUSED_TRACE), the SAH
   tree's leaf size (1 to 4 triangles; the reference's is 1, bvh.cpp:24),
   emission batch size
   (``--batch-size``, cmd_args.hpp:42), the number of bounce-synchronous

@@ -224,8 +224,9 @@ def test_kernel_info_reports_registers_and_occupancy():
     per SIMD, the block sizes the launches use."""
     info = P.kernel_info()
     print(info)
-    for name in ("k_trace_camera", "k_trace_shadow", "k_trace_resolve", "k_emit_wf_bounce", "k_emit_wf_finish",
-                 "k_hash", "k_radix_scatter", "k_decode_photons", "k_gather_flat"):
+    for name in ("k_trace_camera", "k_trace_camera_rest", "k_trace_shadow", "k_trace_resolve", "k_emit_wf_bounce",
+                 "k_emit_wf_finish", "k_emit_trace", "k_emit_shade", "k_hash", "k_radix_scatter", "k_decode_photons",
+                 "k_gather_flat"):
         k = info[name]
         assert 0 < k["vgprs"] <= 512 and k["waves_per_simd"] >= 1, (name, k)
     assert info["k_trace_shadow"]["block"] == 128 and info["k_emit_wf_bounce"]["block"] == 128
