@@ -1129,34 +1129,36 @@ PHX_DEV void gather_pixel_flat(const GatherArgs &a, bool valid, u32 tid, const P
         u32 n1 = k < e ? (e - k < B ? e - k : B) : 0u;
         u32 n2 = (n1 < B && k2 < e2) ? (e2 - k2 < B - n1 ? e2 - k2 : B - n1) : 0u;
         u32 nt = n1 + n2;
-        F4 pp[B], pd[B];
         bool work = nt > 0;
-        u32 kl = n2 ? k2 + n2 - 1 : k + n1 - 1;
-        u32 pdr[DECODED ? 1 : B];
-#pragma unroll
-        for (u32 q = 0; q < B; q++) {
-            u32 kk = q < n1 ? k + q : (q < nt ? k2 + (q - n1) : kl);
-            pp[q] = work ? ld(a.ppos + kk) : ZERO;
-            if (DECODED) {
-                pd[q] = work ? ld(a.pdirpow + kk) : ZERO;
-            } else {
-                pd[q].w = phx_as_float(work ? a.ppow[kk] : 0u);
-                pdr[q] = work ? a.pdir[kk] : 0u;
-            }
-        }
-#pragma unroll
-        for (u32 q = 0; q < B; q++)
-            keep_loaded(DECODED ? pd[q] : mk(pd[q].w, phx_as_float(pdr[q]), 0.0f, 0.0f));
-        if (!DECODED) {
-            /* records not decoded once per pass (phx_gather_flux): the same
-               decode_oct at every visit */
+        if (work) {
+            /* the batch's records, all loaded before any is used (a lane
+               without records loads nothing: its registers stay unset) */
+            F4 pp[B], pd[B];
+            u32 kl = n2 ? k2 + n2 - 1 : k + n1 - 1;
+            u32 pdr[DECODED ? 1 : B];
 #pragma unroll
             for (u32 q = 0; q < B; q++) {
-                F4 dv = decode_oct(pdr[q]);
-                pd[q] = mk(dv.x, dv.y, dv.z, pd[q].w);
+                u32 kk = q < n1 ? k + q : (q < nt ? k2 + (q - n1) : kl);
+                pp[q] = ld(a.ppos + kk);
+                if (DECODED) {
+                    pd[q] = ld(a.pdirpow + kk);
+                } else {
+                    pd[q].w = phx_as_float(a.ppow[kk]);
+                    pdr[q] = a.pdir[kk];
+                }
             }
-        }
-        if (work) {
+#pragma unroll
+            for (u32 q = 0; q < B; q++)
+                keep_loaded(DECODED ? pd[q] : mk(pd[q].w, phx_as_float(pdr[q]), 0.0f, 0.0f));
+            if (!DECODED) {
+                /* records not decoded once per pass (phx_gather_flux): the
+                   same decode_oct at every visit */
+#pragma unroll
+                for (u32 q = 0; q < B; q++) {
+                    F4 dv = decode_oct(pdr[q]);
+                    pd[q] = mk(dv.x, dv.y, dv.z, pd[q].w);
+                }
+            }
 #pragma unroll
             for (u32 q = 0; q < B; q++) {
                 if (q >= nt)
