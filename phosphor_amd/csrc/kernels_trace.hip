@@ -408,6 +408,7 @@ WfState wf_layout(void *ws, u32 n_pixels, u32 n, bool with_scene) {
     size_t cap = rec * n;
     char *p = (char *)ws;
     WfState S;
+    S.n_paths = n_pixels;
     S.counter = (u32 *)p;
     p += 256;
     /* the camera queue's count shares the counter block (one fill clears

@@ -46,13 +46,14 @@ constexpr u32 NO_EVENT = 0xFFFFFFFFu;
 constexpr u32 EMIT_MISS = 0xFFFFFFFFu;
 
 struct WfState {
-    phx_float4 *rec_a, *rec_thr; /* [pixel · 5 + bounce] */
+    u32 n_paths;                 /* paths the layout holds: the stride of the per-bounce arrays */
+    phx_float4 *rec_a, *rec_thr; /* [bounce · n_paths + path] (rec_at) */
     u32 *rec_base;               /* first shadow slot of the NEE event of that bounce */
     u32 *meta;                   /* per pixel: bits 0-2 records, 3-12 NEE kinds, 15 envmap term */
     phx_float4 *env;             /* per pixel: throughput · envmap on a miss */
     f32 *occl;                   /* per pixel: occlusion at the diffuse hit */
     u32 *counter;                /* shadow slots handed out */
-    /* NEE events [pixel · 5 + bounce]: hit position, shading normal, view
+    /* NEE events [bounce · n_paths + path]: hit position, shading normal, view
        direction, base colour (diffuse) or F0 (specular), (roughness,
        specular flag) */
     phx_float4 *ev_pos, *ev_n, *ev_view, *ev_col, *ev_misc;
@@ -68,6 +69,11 @@ struct WfState {
     phx_float4 *cq_state;
     u32 cap;
 };
+
+/* Index of bounce `depth` of path `wf` in the per-bounce arrays: bounce-major,
+ * so the lanes of a wave, which hold neighbouring paths at the same bounce,
+ * write and read neighbouring entries. */
+PHX_DEV size_t rec_at(const WfState &S, u32 wf, u32 depth) { return (size_t)depth * S.n_paths + wf; }
 
 /* hitpoint outputs written by the camera path */
 struct CamOut {
@@ -129,7 +135,6 @@ PHX_DEV void camera_path_run(const SceneView &sc, u32 n, u32 out, u32 wf, u32 ta
     F4 &origin = p.origin, &dir = p.dir, &throughput = p.throughput;
     u32 &rng = p.rng, &depth = p.depth, &kinds = p.kinds, &env_flag = p.env_flag;
     bool &diffuse = p.diffuse;
-    size_t rbase = (size_t)wf * MAX_RAY_BOUNCES;
     /* the loop runs while any lane of the wave has a bounce to take, so the
        wave-wide slot allocation below sees every lane */
     for (;;) {
@@ -161,7 +166,7 @@ PHX_DEV void camera_path_run(const SceneView &sc, u32 n, u32 out, u32 wf, u32 ta
                 if (curr_ior != AIR_IOR && mat.thickness > EPS && mat.att_dist > EPS)
                     throughput = throughput * beer_lambert(from(mat.att_color), mat.att_dist, sh.t);
                 Ctx c = evaluate_material(sc, mat, sh.mat_index, rec, sh);
-                st(S.rec_a + rbase + depth, throughput * c.emissive);
+                st(S.rec_a + rec_at(S, wf, depth), throughput * c.emissive);
                 view = -dir;
                 bs = sample_bsdf(rng, c, c.shading_normal, sh.normal, view, ior, sh.front_face);
                 hit_pos = sh.position;
@@ -191,7 +196,7 @@ PHX_DEV void camera_path_run(const SceneView &sc, u32 n, u32 out, u32 wf, u32 ta
                     }
                 }
                 if (nee)
-                    st(S.rec_thr + rbase + depth, throughput);
+                    st(S.rec_thr + rec_at(S, wf, depth), throughput);
             }
         }
         /* one atomic per wave for the shadow slots of this bounce */
@@ -213,7 +218,7 @@ PHX_DEV void camera_path_run(const SceneView &sc, u32 n, u32 out, u32 wf, u32 ta
                the samples take (skip_light_fragments), so the path goes on
                with the state the reference has after direct lighting. */
             u32 base = wave_base + rank * slots;
-            size_t e = rbase + depth;
+            size_t e = rec_at(S, wf, depth);
             S.rec_base[e] = base;
             st(S.ev_pos + e, hit_pos);
             st(S.ev_n + e, sn);
@@ -444,15 +449,14 @@ PHX_DEV void resolve_pixel(const WfState &S, u32 n, u32 wf, u32 out, phx_float4 
                            bool summed = false, bool shared = false) {
     u32 meta = S.meta[wf];
     u32 nrec = meta & 7u;
-    size_t rbase = (size_t)wf * MAX_RAY_BOUNCES;
     F4 e = BLACK;
     F4 direct = BLACK;
     for (u32 k = 0; k < nrec; k++) {
-        e = e + ld(S.rec_a + rbase + k);
+        e = e + ld(S.rec_a + rec_at(S, wf, k));
         u32 kind = (meta >> (3 + 2 * k)) & 3u;
         if (kind == NEE_NONE)
             continue;
-        size_t base = S.rec_base[rbase + k];
+        size_t base = S.rec_base[rec_at(S, wf, k)];
         /* light_sampling.h: sum of weight × transmittance over the samples,
            BLACK for a sample without a shadow ray (products formed by
            shadow_one) */
@@ -466,7 +470,7 @@ PHX_DEV void resolve_pixel(const WfState &S, u32 n, u32 wf, u32 out, phx_float4 
         else
             for (u32 d = 0; d < n; d++)
                 sum = sum + ld(S.sh_tr + base + d);
-        F4 thr = ld(S.rec_thr + rbase + k);
+        F4 thr = ld(S.rec_thr + rec_at(S, wf, k));
         if (kind == NEE_SPECULAR) {
             e = e + thr * (sum / (f32)n);
         } else {
