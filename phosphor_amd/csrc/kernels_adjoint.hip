@@ -267,7 +267,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adj_photons(AdjPhotonArgs a) {
         f32 inv_e = (em != 0xFFFFFFFFu) ? a.am[em].inv_emis : 0.0f;
         F4 throughput = mk(1.0f, 1.0f, 1.0f, 0.0f);
         IorStack ior{s_ior + t, (u32)ADJ_BLOCK, 0u};
-        init_ior_stack(ior, AIR_IOR);
+        init_ior_stack(ior);
         u32 n = 0, base = a.offs[li];
         /* emit_photons.cl:49-113 */
         for (u32 depth = 0; depth < a.max_bounces; depth++) {
@@ -389,7 +389,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adj_camera(AdjCameraArgs a) {
         F4 lam_f = ld(a.lam_tau + tid) * a.R[tid];
         F4 throughput = WHITE, emission = BLACK, direct = BLACK;
         IorStack ior{s_ior + t, (u32)ADJ_BLOCK, 0u};
-        init_ior_stack(ior, AIR_IOR);
+        init_ior_stack(ior);
         f32 inv_n = 1.0f / (f32)a.n;
         for (u32 depth = 0;;) {
             HitRecord rec;

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(TraceArgs a) {
 
     F4 position = ZERO, normal = ZERO, throughput = WHITE, base_color = BLACK, direct = BLACK, emission = BLACK;
     IorStack ior{s_ior + threadIdx.x, (u32)TRACE_BLOCK, 0u};
-    init_ior_stack(ior, AIR_IOR);
+    init_ior_stack(ior);
 
     u32 depth = 0;
     /* state of the current hit (kept while its NEE loop runs) */

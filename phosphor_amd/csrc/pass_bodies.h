@@ -113,7 +113,7 @@ PHX_DEV CamPath camera_path_begin(const PhxCameraParams &cam, u32 w, u32 h, u32 
     if (live)
         make_ray(cam, px, py, w, h, p.rng, p.origin, p.dir);
     p.throughput = WHITE;
-    init_ior_stack(ior, AIR_IOR);
+    init_ior_stack(ior);
     p.depth = 0;
     p.kinds = 0;
     p.env_flag = 0;
@@ -306,7 +306,7 @@ PHX_DEV CamPath camera_path_load(const WfState &S, u32 q, IorStack &ior) {
     F4 m = ld(d + 3);
     p.rng = phx_as_uint(m.x);
     p.kinds = phx_as_uint(m.y);
-    init_ior_stack(ior, AIR_IOR);
+    init_ior_stack(ior);
     ior.cur = phx_as_uint(m.z);
     if (ior.cur)
         ior.ior[ior.stride] = m.w;

@@ -849,15 +849,18 @@ struct IorStack {
     u32 stride;
     u32 cur;
 };
-PHX_DEV void init_ior_stack(IorStack &s, f32 v) {
+/* Every stack starts with air (shading.h:30-33) and entry 0 is never
+ * overwritten (a push writes above it), so the bottom entry is read as the
+ * constant: a path that never enters a medium reads no stack memory. */
+PHX_DEV void init_ior_stack(IorStack &s) {
     s.cur = 0;
-    s.ior[0] = v;
+    s.ior[0] = AIR_IOR;
 }
-PHX_DEV f32 current_ior(const IorStack &s) { return s.ior[s.cur * s.stride]; }
+PHX_DEV f32 current_ior(const IorStack &s) { return s.cur == 0 ? AIR_IOR : s.ior[s.cur * s.stride]; }
 PHX_DEV f32 target_ior(const IorStack &s, f32 inside, bool front) {
     if (front)
         return inside;
-    return (s.cur == 0) ? AIR_IOR : s.ior[(s.cur - 1) * s.stride];
+    return (s.cur <= 1) ? AIR_IOR : s.ior[(s.cur - 1) * s.stride];
 }
 PHX_DEV void update_ior_stack(IorStack &s, f32 inside, bool front) {
     if (front) {
