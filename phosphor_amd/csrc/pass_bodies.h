@@ -774,7 +774,10 @@ PHX_DEV f32 gauss_weight(f32 dsq, f32 two_r_sq, f32 inv_two_r_sq) {
  * to nf / found in storage order.  DECODED reads the records' decoded
  * direction (with the packed power) instead of decoding it per visit: the
  * direction test dot(decode_oct(d), sn) is ((x·sn.x + y·sn.y) + z·sn.z) +
- * 0·sn.w, the last term (decode_oct's w is +0) being the lane's constant wz. */
+ * 0·sn.w, the last term (decode_oct's w is +0) being the lane's constant wz.
+ * wz is +0, −0 or NaN, so the test `s + wz > 0` is `s > wz`: for a zero wz
+ * the sum is s (or +0 when s is −0, also not above zero), for NaN both are
+ * false. */
 template <bool DECODED>
 PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32 r_sq, f32 two_r_sq,
                          f32 inv_two_r_sq, F4 &nf, f32 &found) {
@@ -799,7 +802,7 @@ PHX_DEV void gather_cell(const GatherArgs &a, u32 s0, u32 e0, F4 pos, F4 sn, f32
                 f32 dsq = dot(dd, dd);
                 if (dsq >= r_sq)
                     continue;
-                if (((pd[j].x * sn.x + pd[j].y * sn.y) + pd[j].z * sn.z) + wz > 0.0f)
+                if (((pd[j].x * sn.x + pd[j].y * sn.y) + pd[j].z * sn.z) > wz)
                     continue;
                 f32 wgt = gauss_weight(dsq, two_r_sq, inv_two_r_sq);
                 nf = nf + wgt * decode_rgbe(phx_as_uint(pd[j].w));
@@ -1171,7 +1174,8 @@ PHX_DEV void gather_pixel_flat(const GatherArgs &a, bool valid, u32 tid, const P
                    one branch, so the sums have one merge point */
                 F4 dd = pos - pp[q];
                 f32 dsq = dot(dd, dd);
-                bool acc = dsq < r_sq && !(((pd[q].x * sn.x + pd[q].y * sn.y) + pd[q].z * sn.z) + wz > 0.0f);
+                /* `s + wz > 0` as `s > wz` (gather_cell) */
+                bool acc = dsq < r_sq && !(((pd[q].x * sn.x + pd[q].y * sn.y) + pd[q].z * sn.z) > wz);
                 if (acc) {
                     f32 wgt = gauss_weight(dsq, two_r_sq, inv_two_r_sq);
                     nf = nf + wgt * decode_rgbe_ldexp(phx_as_uint(pd[q].w));
