@@ -666,7 +666,11 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) void k_e
     const u32 n = emit_qlen(S, a.depth, w.k, a.photons);
     for (u32 i = w.first + threadIdx.x; i < n; i += w.step) {
         u32 li = emit_qentry(S, a.depth, w.k, i);
-        emit_advance<B>(a.sc, S, a.g, li, a.depth, a.max_bounces, false, stk);
+        if (S.dense)
+            emit_advance<B>(a.sc, S, a.g, li, a.depth, a.max_bounces, false, stk, &S.ps[a.depth & 1],
+                            (size_t)w.k * S.slice + i);
+        else
+            emit_advance<B>(a.sc, S, a.g, li, a.depth, a.max_bounces, false, stk);
     }
 }
 
@@ -769,10 +773,13 @@ struct EmitRaySrc {
     const EmitWfState &S;
     u32 depth, k;
     phx_float4 *hits;
+    /* the split bounces keep the ray state dense: entry qi's at slot
+       k·slice + qi of set depth & 1 */
     __device__ void ray(u32 qi, F4 &o, F4 &d) const {
-        u32 li = emit_qentry(S, depth, k, qi);
-        o = ld(S.origin + li);
-        d = ld(S.dir + li);
+        const PhotonSet &ps = S.ps[depth & 1];
+        size_t at = (size_t)k * S.slice + qi;
+        o = ld(ps.origin + at);
+        d = ld(ps.dir + at);
     }
     __device__ void hit(u32 qi, F4 h) const { st(hits + qi, h); }
 };
@@ -800,39 +807,63 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) void k_e
     u32 *cnt_out = emit_qcounter(S.qcount, depth + 1, w.k);
     u32 *qout = S.queue[(depth + 1) & 1] + (size_t)w.k * S.slice;
     const phx_float4 *hits = S.hit + (size_t)w.k * S.slice;
+    const size_t row = (size_t)w.k * S.slice;
+    const PhotonSet &in = S.ps[depth & 1], &out = S.ps[(depth + 1) & 1];
     for (u32 base = w.first; base < n; base += w.step) {
         u32 i = base + threadIdx.x;
         bool keep = false;
         u32 li = 0;
+        PhotonRay r;
         if (i < n) {
             li = emit_qentry(S, depth, w.k, i);
-            keep = emit_after_hit(a.sc, S, a.g, li, depth, a.max_bounces, ld(hits + i));
+            keep = emit_after_hit(a.sc, S, a.g, li, depth, a.max_bounces, ld(hits + i), in, row + i, r);
         }
-        wave_append(keep, li, cnt_out, qout);
+        /* the survivor's state goes to the slot of its entry in the next
+           queue */
+        u32 pos = wave_append(keep, li, cnt_out, qout);
+        if (keep) {
+            size_t at = row + pos;
+            st(out.origin + at, r.origin);
+            st(out.dir + at, r.dir);
+            st(out.power + at, r.power);
+            st(out.thr + at, r.thr);
+            out.rng[at] = r.rng;
+            out.ior_cur[at] = r.ior_cur;
+        }
     }
 }
 
 EmitWfState emit_wf_layout(char *&p, u32 photons, u32 max_bounces) {
     EmitWfState S;
     size_t n = photons;
-    S.origin = (phx_float4 *)p;
-    p += align_up(n * 16, 256);
-    S.dir = (phx_float4 *)p;
-    p += align_up(n * 16, 256);
-    S.power = (phx_float4 *)p;
-    p += align_up(n * 16, 256);
-    S.thr = (phx_float4 *)p;
-    p += align_up(n * 16, 256);
-    S.rng = (u32 *)p;
-    p += align_up(n * 4, 256);
-    S.ior_cur = (u32 *)p;
-    p += align_up(n * 4, 256);
+    S.slice = (u32)((n + EMIT_NQ - 1) / EMIT_NQ);
+    size_t qn = (size_t)S.slice * EMIT_NQ;
+    /* two sets of ray state, one entry per queue position */
+    for (int k = 0; k < 2; k++) {
+        S.ps[k].origin = (phx_float4 *)p;
+        p += align_up(qn * 16, 256);
+        S.ps[k].dir = (phx_float4 *)p;
+        p += align_up(qn * 16, 256);
+        S.ps[k].power = (phx_float4 *)p;
+        p += align_up(qn * 16, 256);
+        S.ps[k].thr = (phx_float4 *)p;
+        p += align_up(qn * 16, 256);
+        S.ps[k].rng = (u32 *)p;
+        p += align_up(qn * 4, 256);
+        S.ps[k].ior_cur = (u32 *)p;
+        p += align_up(qn * 4, 256);
+    }
+    S.origin = S.ps[0].origin;
+    S.dir = S.ps[0].dir;
+    S.power = S.ps[0].power;
+    S.thr = S.ps[0].thr;
+    S.rng = S.ps[0].rng;
+    S.ior_cur = S.ps[0].ior_cur;
+    S.dense = 0;
     S.count = (u32 *)p;
     p += align_up(n * 4, 256);
     S.ior = (f32 *)p;
     p += align_up(n * IOR_STACK_SIZE * 4, 256);
-    S.slice = (u32)((n + EMIT_NQ - 1) / EMIT_NQ);
-    size_t qn = (size_t)S.slice * EMIT_NQ;
     S.queue[0] = (u32 *)p;
     p += align_up(qn * 4, 256);
     S.queue[1] = (u32 *)p;
@@ -858,7 +889,8 @@ size_t emit_workspace_bytes(u32 photons, u32 max_bounces) {
     b += align_up(scan_workspace_bytes(photons), 256);
     /* wavefront state */
     size_t qn = (n + EMIT_NQ - 1) / EMIT_NQ * EMIT_NQ;
-    b += 4 * align_up(n * 16, 256) + 3 * align_up(n * 4, 256) + align_up(n * IOR_STACK_SIZE * 4, 256);
+    b += 2 * (4 * align_up(qn * 16, 256) + 2 * align_up(qn * 4, 256)); /* two sets of ray state */
+    b += align_up(n * 4, 256) + align_up(n * IOR_STACK_SIZE * 4, 256);
     b += 2 * align_up(qn * 4, 256) + align_up(qn * 16, 256);              /* queues, hits */
     b += 2 * align_up(emit_counter_words(max_bounces) * 4, 256);        /* counters, tickets */
     return b;
@@ -925,6 +957,7 @@ hipError_t launch_emit(const SceneArgs &sc, u32 max_photons, u32 offset, u32 pho
     a.d_seed = d_seed;
     a.depth = 0;
     a.st = S;
+    a.st.dense = split ? 1u : 0u; /* split bounces move each survivor's state to its queue slot */
     a.g = L.g;
     u32 sb = (photons + 255) / 256;
     sb = sb > EMIT_START_BLOCKS ? EMIT_START_BLOCKS : sb;

@@ -485,10 +485,25 @@ PHX_DEV void resolve_pixel(const WfState &S, u32 n, u32 wf, u32 out, phx_float4 
 }
 
 /* ---------------- photon emission ---------------- */
-struct EmitWfState {
+/* The ray state of live photons: ray, power, throughput, PCG state and IOR
+ * stack depth, one entry per slot. */
+struct PhotonSet {
     phx_float4 *origin, *dir, *power, *thr;
-    u32 *rng, *ior_cur, *count;
-    f32 *ior; /* [depth][photon] */
+    u32 *rng, *ior_cur;
+};
+struct EmitWfState {
+    /* photon li's ray state at slot li of set 0 (origin ... ior_cur alias
+       ps[0]).  With `dense` (split bounces) the state of the photon at
+       entry i of sub-queue k entering bounce d is instead at slot
+       k·slice + i of set d & 1: the shading kernel writes each survivor's
+       state where it appends the survivor, so every bounce reads and writes
+       its state in queue order (contiguous across a wave) */
+    phx_float4 *origin, *dir, *power, *thr;
+    u32 *rng, *ior_cur;
+    PhotonSet ps[2];
+    u32 dense;
+    u32 *count; /* [photon]: deposits so far */
+    f32 *ior;   /* [depth][photon] */
     /* live photons, as EMIT_NQ sub-queues of `slice` entries: photon slot
        li = k·slice + i starts in sub-queue k, and a photon's survivors stay
        in its sub-queue, whose capacity is therefore `slice` */
@@ -621,39 +636,45 @@ PHX_DEV bool emit_step(const SceneView &sc, const EmitStaging &g, u32 li, u32 de
     return emit_shade(sc, g, li, depth, max_bounces, rec, origin, dir, power, throughput, rng, ior, n);
 }
 
-/* Loads photon li's state, runs bounces depth .. (depth + 1 if one_bounce,
- * else to the end of its path), stores what changed; returns whether the
- * photon is still live after a single bounce. */
+/* Loads photon li's state (its ray state from slot `at` of set ps, li's own
+ * slot of set 0 unless the state is dense), runs bounces depth .. (depth + 1
+ * if one_bounce, else to the end of its path), stores what changed; returns
+ * whether the photon is still live after a single bounce. */
 template <u32 SB = EMIT_WF_BLOCK>
 PHX_DEV bool emit_advance(const SceneView &sc, const EmitWfState &S, const EmitStaging &g, u32 li, u32 depth,
-                          u32 max_bounces, bool one_bounce, u32 *stk) {
-    F4 origin = ld(S.origin + li), dir = ld(S.dir + li);
+                          u32 max_bounces, bool one_bounce, u32 *stk, const PhotonSet *ps = nullptr,
+                          size_t at = 0) {
+    if (!ps) {
+        ps = &S.ps[0];
+        at = li;
+    }
+    F4 origin = ld(ps->origin + at), dir = ld(ps->dir + at);
     if (one_bounce) {
         /* the rest of the state is loaded after the traversal, which then
            runs with fewer live registers */
         HitRecord rec;
         if (!scene_intersect<true>(sc, origin, dir, EPS, INF, rec, stk, SB))
             return false;
-        F4 power = ld(S.power + li), throughput = ld(S.thr + li);
-        u32 rng = S.rng[li];
-        IorStack ior{S.ior + li, g.stride, S.ior_cur[li]};
+        F4 power = ld(ps->power + at), throughput = ld(ps->thr + at);
+        u32 rng = ps->rng[at];
+        IorStack ior{S.ior + li, g.stride, ps->ior_cur[at]};
         u32 n = S.count[li], n0 = n;
         bool keep = emit_shade(sc, g, li, depth, max_bounces, rec, origin, dir, power, throughput, rng, ior, n);
         if (keep) {
-            st(S.origin + li, origin);
-            st(S.dir + li, dir);
-            st(S.power + li, power);
-            st(S.thr + li, throughput);
-            S.rng[li] = rng;
-            S.ior_cur[li] = ior.cur;
+            st(ps->origin + at, origin);
+            st(ps->dir + at, dir);
+            st(ps->power + at, power);
+            st(ps->thr + at, throughput);
+            ps->rng[at] = rng;
+            ps->ior_cur[at] = ior.cur;
         }
         if (n != n0)
             S.count[li] = n;
         return keep;
     }
-    F4 power = ld(S.power + li), throughput = ld(S.thr + li);
-    u32 rng = S.rng[li];
-    IorStack ior{S.ior + li, g.stride, S.ior_cur[li]};
+    F4 power = ld(ps->power + at), throughput = ld(ps->thr + at);
+    u32 rng = ps->rng[at];
+    IorStack ior{S.ior + li, g.stride, ps->ior_cur[at]};
     u32 n = S.count[li], n0 = n;
     for (u32 d = depth; d < max_bounces; d++)
         if (!emit_step<SB>(sc, g, li, d, max_bounces, origin, dir, power, throughput, rng, ior, n, stk))
@@ -663,12 +684,20 @@ PHX_DEV bool emit_advance(const SceneView &sc, const EmitWfState &S, const EmitS
     return false;
 }
 
+/* A surviving photon's ray state between the shading of one bounce and its
+ * store at the slot its queue append gives it. */
+struct PhotonRay {
+    F4 origin, dir, power, thr;
+    u32 rng, ior_cur;
+};
+
 /* The part of bounce `depth` of photon li after its ray has been traced on
- * the pair tree (k_emit_trace): h = (t, u, v, tree position or EMIT_MISS).
- * Returns whether the photon is still live; same state and deposits as
+ * the pair tree (k_emit_trace): h = (t, u, v, tree position or EMIT_MISS),
+ * the ray state read from slot `at` of set `in`.  Returns whether the photon
+ * is still live, its new ray state in r; same state and deposits as
  * emit_advance's single bounce. */
 PHX_DEV bool emit_after_hit(const SceneView &sc, const EmitWfState &S, const EmitStaging &g, u32 li, u32 depth,
-                            u32 max_bounces, F4 h) {
+                            u32 max_bounces, F4 h, const PhotonSet &in, size_t at, PhotonRay &r) {
     u32 j = phx_as_uint(h.w);
     if (j == EMIT_MISS)
         return false;
@@ -678,39 +707,38 @@ PHX_DEV bool emit_after_hit(const SceneView &sc, const EmitWfState &S, const Emi
     rec.v = h.z;
     rec.tri_index = sc.tri_order[j];
     rec.mat_index = sc.tmat[rec.tri_index];
-    F4 origin = ld(S.origin + li), dir = ld(S.dir + li);
-    F4 power = ld(S.power + li), throughput = ld(S.thr + li);
-    u32 rng = S.rng[li];
-    IorStack ior{S.ior + li, g.stride, S.ior_cur[li]};
+    r.origin = ld(in.origin + at);
+    r.dir = ld(in.dir + at);
+    r.power = ld(in.power + at);
+    r.thr = ld(in.thr + at);
+    r.rng = in.rng[at];
+    IorStack ior{S.ior + li, g.stride, in.ior_cur[at]};
     u32 n = S.count[li], n0 = n;
-    bool keep = emit_shade(sc, g, li, depth, max_bounces, rec, origin, dir, power, throughput, rng, ior, n);
-    if (keep) {
-        st(S.origin + li, origin);
-        st(S.dir + li, dir);
-        st(S.power + li, power);
-        st(S.thr + li, throughput);
-        S.rng[li] = rng;
-        S.ior_cur[li] = ior.cur;
-    }
+    bool keep = emit_shade(sc, g, li, depth, max_bounces, rec, r.origin, r.dir, r.power, r.thr, r.rng, ior, n);
+    r.ior_cur = ior.cur;
     if (n != n0)
         S.count[li] = n;
     return keep;
 }
 
-/* appends li to a queue when keep, one atomic per wave; every lane of the
- * wave must call it */
-PHX_DEV void wave_append(bool keep, u32 li, u32 *qcount, u32 *queue) {
+/* appends li to a queue when keep, one atomic per wave, and returns the
+ * position it took (undefined without keep); every lane of the wave must
+ * call it */
+PHX_DEV u32 wave_append(bool keep, u32 li, u32 *qcount, u32 *queue) {
     u32 lane = threadIdx.x & 63u;
     unsigned long long m = __ballot(keep);
+    u32 pos = 0;
     if (m) {
         u32 first = (u32)__ffsll((long long)m) - 1u;
         u32 base = 0;
         if (lane == first)
             base = atomicAdd(qcount, (u32)__popcll(m));
         base = __shfl(base, (int)first, 64);
+        pos = base + (u32)__popcll(m & ((1ull << lane) - 1ull));
         if (keep)
-            queue[base + (u32)__popcll(m & ((1ull << lane) - 1ull))] = li;
+            queue[pos] = li;
     }
+    return pos;
 }
 
 /* ---------------- gather (gather_flux.cl + photon_hash.h:56-103) ---------------- */
