@@ -796,8 +796,10 @@ template <u32 B> __global__ __launch_bounds__(B) void k_emit_trace(EmitWfArgs a)
 /* Split bounce, second half: one lane per queue entry shades its photon at
  * the hit k_emit_trace found, stores the deposit and the new state, and
  * appends survivors to the next queue (emit_after_hit). */
-/* five waves per SIMD (96 VGPRs, 28 bytes of spills per lane) against four
-   (120 VGPRs): atrium 5.72 -> 5.67 ms/pass, caustic within noise */
+/* five waves per SIMD (96 VGPRs) against four (120 VGPRs): atrium 5.72 ->
+   5.67 ms/pass, caustic within noise, when the cap still cost 28 bytes of
+   spills per lane; with the photon state read in queue order it spills
+   nothing */
 template <u32 B>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) void k_emit_shade(EmitWfArgs a) {
     const EmitWfState &S = a.st;
